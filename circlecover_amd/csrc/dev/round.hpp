@@ -1,0 +1,297 @@
+// Directed-rounding FP64/FP32 primitives for gfx950 prover kernels.
+//
+// Execution model.  Every prover kernel ("ccrd_*" symbols) starts its waves
+// with MODE.FP_ROUND = round-toward-negative-infinity for f32 and f64: the
+// build sets `.amdhsa_float_round_mode_32/_16_64 2` in the kernel descriptor
+// (see build.py), so no instruction of the kernel can ever run in another
+// mode by accident.  In that mode:
+//   * a round-down add/mul/fma is ONE VALU instruction;
+//   * a round-up op is the exact IEEE identity RU(x op y) = -RD((-x) op y)
+//     (for add: -RD((-x)+(-y))), realised with VALU negation modifiers;
+//   * div and sqrt start from a Newton approximation (rcp/rsq seeds, all in
+//     round-down) that is within two ulps, and are then stepped to the exact
+//     directed result by the sign of the residual a - q*b (resp. x - s*s),
+//     which one FMA yields with the correct sign and zero-ness whenever the
+//     residual's granularity does not underflow (guaranteed on the fast path's
+//     exponent range; the slow path rescales by powers of two first);
+//   * the few round-to-nearest operations of the reference (bisection
+//     midpoints and get_subinterval) run inside a single asm block that
+//     switches MODE to nearest-even, computes, and switches back.
+// All rounding arithmetic is inline asm so the compiler, which assumes
+// round-to-nearest, can neither constant-fold nor algebraically rewrite it.
+// Denormals stay enabled (descriptor default), as IEEE directed rounding
+// requires.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define CCD __device__ __forceinline__
+
+namespace ccd {
+
+CCD uint64_t bits(double x) { return (uint64_t)__double_as_longlong(x); }
+CCD double from_bits(uint64_t b) { return __longlong_as_double((long long)b); }
+
+// ---------------------------------------------------------------- add / mul
+CCD double add_rd(double a, double b) {
+  double r;
+  asm("v_add_f64 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+  return r;
+}
+CCD double add_ru(double a, double b) {
+  double r;
+  asm("v_add_f64 %0, -%1, -%2" : "=v"(r) : "v"(a), "v"(b));
+  return -r;
+}
+CCD double mul_rd(double a, double b) {
+  double r;
+  asm("v_mul_f64 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+  return r;
+}
+CCD double mul_ru(double a, double b) {
+  double r;
+  asm("v_mul_f64 %0, -%1, %2" : "=v"(r) : "v"(a), "v"(b));
+  return -r;
+}
+// a*b + c rounded once, downward.
+CCD double fma_rd(double a, double b, double c) {
+  double r;
+  asm("v_fma_f64 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+  return r;
+}
+// c - a*b rounded once, downward.
+CCD double fnma_rd(double a, double b, double c) {
+  double r;
+  asm("v_fma_f64 %0, -%1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+  return r;
+}
+
+// ------------------------------------------------------------ round-nearest
+// 0.5*(a+b) with both operations rounded to nearest-even.
+CCD double rn_mid(double a, double b) {
+  double r;
+  asm volatile(
+      "s_nop 1\n\t"
+      "s_setreg_imm32_b32 hwreg(HW_REG_MODE, 0, 4), 0\n\t"
+      "s_nop 1\n\t"
+      "v_add_f64 %0, %1, %2\n\t"
+      "v_mul_f64 %0, 0.5, %0\n\t"
+      "s_nop 1\n\t"
+      "s_setreg_imm32_b32 hwreg(HW_REG_MODE, 0, 4), 10\n\t"
+      "s_nop 1"
+      : "=&v"(r)
+      : "v"(a), "v"(b));
+  return r;
+}
+// fma(d, k, lo) rounded to nearest-even: the reference's `lb + diff*i`
+// as contracted by nvcc's default device code generation.
+CCD double rn_fma(double d, double k, double lo) {
+  double r;
+  asm volatile(
+      "s_nop 1\n\t"
+      "s_setreg_imm32_b32 hwreg(HW_REG_MODE, 0, 4), 0\n\t"
+      "s_nop 1\n\t"
+      "v_fma_f64 %0, %1, %2, %3\n\t"
+      "s_nop 1\n\t"
+      "s_setreg_imm32_b32 hwreg(HW_REG_MODE, 0, 4), 10\n\t"
+      "s_nop 1"
+      : "=&v"(r)
+      : "v"(d), "v"(k), "v"(lo));
+  return r;
+}
+// (hi - lo) / n rounded to nearest-even (the diff of get_subinterval): the
+// standard scaled Newton division sequence executed in nearest-even mode.
+CCD double rn_span_div(double hi, double lo, double n) {
+  double r;
+  double t0, t1, t2, t3, t4;
+  asm volatile(
+      "s_nop 1\n\t"
+      "s_setreg_imm32_b32 hwreg(HW_REG_MODE, 0, 4), 0\n\t"
+      "s_nop 1\n\t"
+      "v_add_f64 %[a], %[hi], -%[lo]\n\t"
+      "v_div_scale_f64 %[d], vcc, %[n], %[n], %[a]\n\t"
+      "v_rcp_f64 %[y], %[d]\n\t"
+      "v_div_scale_f64 %[s], vcc, %[a], %[n], %[a]\n\t"
+      "v_fma_f64 %[t], -%[d], %[y], 1.0\n\t"
+      "v_fma_f64 %[y], %[y], %[t], %[y]\n\t"
+      "v_fma_f64 %[t], -%[d], %[y], 1.0\n\t"
+      "v_fma_f64 %[y], %[y], %[t], %[y]\n\t"
+      "v_mul_f64 %[t], %[s], %[y]\n\t"
+      "v_fma_f64 %[d], -%[d], %[t], %[s]\n\t"
+      "s_nop 4\n\t"
+      "v_div_fmas_f64 %[d], %[d], %[y], %[t]\n\t"
+      "v_div_fixup_f64 %[r], %[d], %[n], %[a]\n\t"
+      "s_nop 1\n\t"
+      "s_setreg_imm32_b32 hwreg(HW_REG_MODE, 0, 4), 10\n\t"
+      "s_nop 1"
+      : [r] "=&v"(r), [a] "=&v"(t0), [d] "=&v"(t1), [y] "=&v"(t2), [s] "=&v"(t3), [t] "=&v"(t4)
+      : [hi] "v"(hi), [lo] "v"(lo), [n] "v"(n)
+      : "vcc");
+  return r;
+}
+
+// ------------------------------------------------------------ neighbours
+// Next representable double toward +inf / -inf for finite nonzero x.
+CCD double next_up(double x) {
+  uint64_t b = bits(x);
+  return from_bits((int64_t)b >= 0 ? b + 1 : b - 1);
+}
+CCD double next_down(double x) {
+  uint64_t b = bits(x);
+  return from_bits((int64_t)b >= 0 ? b - 1 : b + 1);
+}
+CCD double pow2(int k) {  // 2^k for -1022 <= k <= 1023
+  return from_bits((uint64_t)(k + 1023) << 52);
+}
+CCD int exponent_of(double x) {  // unbiased exponent of a normal double
+  return (int)((bits(x) >> 52) & 0x7ff) - 1023;
+}
+// Reciprocal / reciprocal-square-root seeds.  They are transcendental ops:
+// on gfx950 a dependent VALU instruction must not issue in the next cycle
+// (the hardware does not interlock it), and the compiler's hazard tracking
+// does not look into the inline asm that consumes these seeds, so the wait
+// states are part of the asm block itself.
+CCD double rcp_seed(double x) {
+  double r;
+  asm volatile("v_rcp_f64 %0, %1\n\ts_nop 1" : "=v"(r) : "v"(x));
+  return r;
+}
+CCD double rsq_seed(double x) {
+  double r;
+  asm volatile("v_rsq_f64 %0, %1\n\ts_nop 1" : "=v"(r) : "v"(x));
+  return r;
+}
+
+// ------------------------------------------------------------------ division
+// Round-down quotient for b > 0, a and b finite with |a|,|b| in
+// [2^-500, 2^500]: Newton approximation then residual stepping.
+CCD double div_rd_fast_pos(double a, double b) {
+  double y = rcp_seed(b);
+  double e = fnma_rd(b, y, 1.0);
+  y = fma_rd(y, e, y);
+  e = fnma_rd(b, y, 1.0);
+  y = fma_rd(y, e, y);
+  double q = mul_rd(a, y);
+  double r = fnma_rd(b, q, a);
+  q = fma_rd(r, y, q);
+  // sign(a - q*b) == sign(a/b - q); one FMA gives it exactly here.
+  r = fnma_rd(q, b, a);
+  if (r < 0.0) {
+    q = next_down(q);
+    r = fnma_rd(q, b, a);
+    if (r < 0.0) q = next_down(q);
+  } else {
+    double qn = next_up(q);
+    double rn = fnma_rd(qn, b, a);
+    if (rn >= 0.0) {
+      q = qn;
+      qn = next_up(q);
+      rn = fnma_rd(qn, b, a);
+      if (rn >= 0.0) q = qn;
+    }
+  }
+  return q;
+}
+
+__device__ __noinline__ double div_rd_slow(double a, double b);
+
+CCD bool div_fast_ok(double a, double b) {
+  // both exponents within [-500, 499]: |x| in [2^-500, 2^500)
+  uint32_t ea = (uint32_t)((bits(a) >> 52) & 0x7ff);
+  uint32_t eb = (uint32_t)((bits(b) >> 52) & 0x7ff);
+  return (ea - (1023u - 500u)) < 1000u && (eb - (1023u - 500u)) < 1000u;
+}
+
+CCD double div_rd(double a, double b) {
+  if (div_fast_ok(a, b)) return b > 0.0 ? div_rd_fast_pos(a, b) : div_rd_fast_pos(-a, -b);
+  return div_rd_slow(a, b);
+}
+CCD double div_ru(double a, double b) { return -div_rd(-a, b); }
+CCD double rcp_rd(double x) { return div_rd(1.0, x); }
+CCD double rcp_ru(double x) { return div_ru(1.0, x); }
+
+// Exact-sign slow path: special operands by IEEE rules, extreme exponents by
+// power-of-two rescaling (round-down compositions on nested grids are exact).
+__device__ __noinline__ double div_rd_slow(double a, double b) {
+  const double inf = __builtin_inf();
+  const bool neg = ((bits(a) ^ bits(b)) >> 63) != 0;
+  if (a != a || b != b) return __builtin_nan("");
+  if (b == 0.0) return a == 0.0 ? __builtin_nan("") : (neg ? -inf : inf);
+  if (__builtin_isinf(a)) return __builtin_isinf(b) ? __builtin_nan("") : (neg ? -inf : inf);
+  if (__builtin_isinf(b) || a == 0.0) return neg ? -0.0 : 0.0;
+  // finite nonzero: a = ma*2^ka, b = mb*2^kb with |ma|,|mb| in [1/2,1)
+  int ka, kb;
+  double ma = __builtin_frexp(a, &ka);  // frexp is exact (power-of-two scaling)
+  double mb = __builtin_frexp(b, &kb);
+  double q = mb > 0.0 ? div_rd_fast_pos(ma, mb) : div_rd_fast_pos(-ma, -mb);
+  int k = ka - kb;
+  while (k > 1023) {
+    q = mul_rd(q, pow2(1023));
+    k -= 1023;
+  }
+  while (k < -1022) {
+    q = mul_rd(q, pow2(-1022));
+    k += 1022;
+  }
+  return mul_rd(q, pow2(k));
+}
+
+// --------------------------------------------------------------------- sqrt
+// Round-down square root for x in [2^-960, DBL_MAX].
+CCD double sqrt_rd_fast(double x) {
+  double y = rsq_seed(x);
+  double g = mul_rd(x, y);
+  double h = mul_rd(y, 0.5);
+  double r = fnma_rd(g, h, 0.5);
+  g = fma_rd(g, r, g);
+  h = fma_rd(h, r, h);
+  double d = fnma_rd(g, g, x);
+  g = fma_rd(d, h, g);
+  d = fnma_rd(g, g, x);
+  g = fma_rd(d, h, g);
+  // sign(x - g*g) == sign(sqrt(x) - g)
+  d = fnma_rd(g, g, x);
+  if (d < 0.0) {
+    g = next_down(g);
+    d = fnma_rd(g, g, x);
+    if (d < 0.0) g = next_down(g);
+  } else {
+    double gn = next_up(g);
+    double dn = fnma_rd(gn, gn, x);
+    if (dn >= 0.0) {
+      g = gn;
+      gn = next_up(g);
+      dn = fnma_rd(gn, gn, x);
+      if (dn >= 0.0) g = gn;
+    }
+  }
+  return g;
+}
+
+__device__ __noinline__ double sqrt_rd_slow(double x);
+
+CCD bool sqrt_fast_ok(double x) {
+  // x in [2^-960, +inf): biased exponent in [63, 2046] and sign clear
+  uint64_t b = bits(x);
+  return (b >> 52) - 63u < 1984u;
+}
+CCD double sqrt_rd(double x) { return sqrt_fast_ok(x) ? sqrt_rd_fast(x) : sqrt_rd_slow(x); }
+CCD double sqrt_ru(double x) {
+  double s = sqrt_rd(x);
+  if (!sqrt_fast_ok(x)) {
+    // exact for 0, -0, +inf, NaN and negative inputs; tiny x rescaled
+    if (!(x > 0.0) || __builtin_isinf(x)) return s;
+    double xs = mul_rd(x, pow2(1000));
+    double ss = mul_rd(s, pow2(500));
+    return fnma_rd(ss, ss, xs) == 0.0 ? s : next_up(s);
+  }
+  return fnma_rd(s, s, x) == 0.0 ? s : next_up(s);
+}
+__device__ __noinline__ double sqrt_rd_slow(double x) {
+  if (x != x || x < 0.0) return __builtin_nan("");
+  if (x == 0.0 || __builtin_isinf(x)) return x;
+  // 0 < x < 2^-960: sqrt(x*2^1000) = sqrt(x)*2^500, both scalings exact
+  return mul_rd(sqrt_rd_fast(mul_rd(x, pow2(1000))), pow2(-500));
+}
+
+}  // namespace ccd

@@ -1,0 +1,197 @@
+"""ctypes bindings of libcirclecover_amd.so (the C ABI in include/circlecover_amd.h)."""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+library_path = os.path.join(_PKG, "libcirclecover_amd.so")
+
+CC_OK, CC_E_HIP, CC_E_OVERFLOW, CC_E_TRAP, CC_E_INTERRUPTED, CC_E_ARG, CC_E_NODEVICE = range(7)
+CC_BATCH_C2 = 0
+CC_PREC_FP64, CC_PREC_FP32_PRE = 0, 1
+
+# Criterion names of the ten-criterion 4-D box set, in evaluation order
+# (bit k of a verdict mask).
+C2_CRITERIA = (
+    "tr.r1_strategies",
+    "tr.r1_r2_strategies_simple",
+    "tr.r1_r2_strategies_stacked",
+    "tr.r1_r2_r3_strategy_horizontal",
+    "tr.r1_r2_r3_strategy_cover_strip",
+    "rsb.shortcut_even_split_recursion",
+    "rsb.shortcut_uneven_split_recursion/3",
+    "rsb.shortcut_uneven_split_recursion/4",
+    "rsb.shortcut_r1_r2_opposite_corners_recursion",
+    "rsb.shortcut_r1_r2_vertical_strip",
+)
+
+
+class CCError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"circlecover_amd error {code}: {msg}")
+        self.code = code
+
+
+def _load() -> ctypes.CDLL:
+    if not os.path.exists(library_path):
+        raise ImportError(f"{library_path} is not built; run `python circlecover_amd/build.py`")
+    return ctypes.CDLL(library_path)
+
+
+lib = _load()
+
+_vp, _sz, _i, _u64 = ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int, ctypes.c_uint64
+_P = ctypes.POINTER
+lib.cc_init.argtypes = [_i, _P(_vp)]
+lib.cc_destroy.argtypes = [_vp]
+lib.cc_last_error.argtypes = [_vp]
+lib.cc_last_error.restype = ctypes.c_char_p
+lib.cc_synchronize.argtypes = [_vp]
+lib.cc_dev_alloc.argtypes = [_vp, _sz, _P(_vp)]
+lib.cc_dev_free.argtypes = [_vp, _vp]
+lib.cc_h2d.argtypes = [_vp, _vp, _vp, _sz]
+lib.cc_d2h.argtypes = [_vp, _vp, _vp, _sz]
+lib.cc_timer_start.argtypes = [_vp]
+lib.cc_timer_stop.argtypes = [_vp, _P(ctypes.c_float)]
+lib.cc_eval_batch.argtypes = [_vp, _i, _i, _vp, _sz, _sz, _vp, _vp]
+lib.cc_eval_batch_async.argtypes = [_vp, _i, _i, _vp, _sz, _sz, _vp, _vp]
+lib.cc_gen_c2_boxes.argtypes = [_u64, _sz, _vp]
+lib.cc_gen_c2_boxes.restype = None
+lib.cc_prim_selftest.argtypes = [_vp, _vp, _vp, _sz, _vp]
+lib.cc_prim_rn_selftest.argtypes = [_vp, _vp, _vp, _vp, _sz, _vp]
+lib.cc_debug_trace_strip.argtypes = [_vp, _vp, _vp, _i]
+lib.cc_rsb_eval_cells.argtypes = [_vp, _vp, _sz, _vp, _vp]
+lib.cc_rsb_eval_cells_async.argtypes = [_vp, _vp, _sz, _vp, _vp]
+lib.cc_rsb_eval_masks.argtypes = [_vp, _vp, _sz, _vp]
+
+
+def _ptr(a: np.ndarray) -> int:
+    return a.ctypes.data
+
+
+def gen_c2_boxes(seed: int, n: int) -> np.ndarray:
+    """Synthetic 4-D boxes as an (8, n) float64 SoA array."""
+    out = np.empty((8, n), dtype=np.float64)
+    lib.cc_gen_c2_boxes(seed & 0xFFFFFFFFFFFFFFFF, n, _ptr(out))
+    return out
+
+
+class DeviceBuffer:
+    def __init__(self, prover: "Prover", nbytes: int):
+        self.prover, self.nbytes = prover, nbytes
+        p = ctypes.c_void_p()
+        prover._check(lib.cc_dev_alloc(prover.ctx, nbytes, ctypes.byref(p)))
+        self.ptr = p.value
+
+    def upload(self, a: np.ndarray) -> None:
+        assert a.nbytes <= self.nbytes and a.flags.c_contiguous
+        self.prover._check(lib.cc_h2d(self.prover.ctx, self.ptr, _ptr(a), a.nbytes))
+
+    def download(self, a: np.ndarray) -> np.ndarray:
+        assert a.nbytes <= self.nbytes and a.flags.c_contiguous
+        self.prover._check(lib.cc_d2h(self.prover.ctx, _ptr(a), self.ptr, a.nbytes))
+        return a
+
+    def free(self) -> None:
+        if self.ptr:
+            lib.cc_dev_free(self.prover.ctx, self.ptr)
+            self.ptr = None
+
+
+class Prover:
+    """One device context (HIP stream, kernel module, device constants)."""
+
+    def __init__(self, device: int = 0):
+        p = ctypes.c_void_p()
+        rc = lib.cc_init(device, ctypes.byref(p))
+        if rc != CC_OK:
+            raise CCError(rc, "cc_init failed (is a gfx950 GPU visible?)")
+        self.ctx = p.value
+
+    def close(self) -> None:
+        if self.ctx:
+            lib.cc_destroy(self.ctx)
+            self.ctx = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc: int) -> None:
+        if rc != CC_OK:
+            raise CCError(rc, lib.cc_last_error(self.ctx).decode(errors="replace"))
+
+    # ---- memory / timing
+    def alloc(self, nbytes: int) -> DeviceBuffer:
+        return DeviceBuffer(self, nbytes)
+
+    def synchronize(self) -> None:
+        self._check(lib.cc_synchronize(self.ctx))
+
+    def timer_start(self) -> None:
+        self._check(lib.cc_timer_start(self.ctx))
+
+    def timer_stop(self) -> float:
+        ms = ctypes.c_float()
+        self._check(lib.cc_timer_stop(self.ctx, ctypes.byref(ms)))
+        return ms.value
+
+    # ---- batched criteria
+    def eval_batch(self, soa: np.ndarray, precision: int = CC_PREC_FP64):
+        soa = np.ascontiguousarray(soa, dtype=np.float64)
+        assert soa.ndim == 2 and soa.shape[0] == 8
+        n = soa.shape[1]
+        mask = np.empty(n, dtype=np.uint32)
+        first = np.empty(n, dtype=np.uint8)
+        self._check(lib.cc_eval_batch(self.ctx, CC_BATCH_C2, precision, _ptr(soa), n, n, _ptr(mask), _ptr(first)))
+        return mask, first
+
+    def eval_batch_async(self, d_soa: DeviceBuffer, n: int, ld: int, d_mask: DeviceBuffer, d_first: DeviceBuffer,
+                         precision: int = CC_PREC_FP64) -> None:
+        self._check(lib.cc_eval_batch_async(self.ctx, CC_BATCH_C2, precision, d_soa.ptr, n, ld, d_mask.ptr,
+                                            d_first.ptr))
+
+    def rsb_eval_cells(self, cells: np.ndarray):
+        """cells: (n, 14) float64 (la, r1..r6 as lb,ub).  Returns (verdict int32, trap uint8)."""
+        cells = np.ascontiguousarray(cells, dtype=np.float64).reshape(-1, 14)
+        n = cells.shape[0]
+        verdict = np.empty(n, dtype=np.int32)
+        trap = np.empty(n, dtype=np.uint8)
+        self._check(lib.cc_rsb_eval_cells(self.ctx, _ptr(cells), n, _ptr(verdict), _ptr(trap)))
+        return verdict, trap
+
+    def rsb_eval_masks(self, cells: np.ndarray) -> np.ndarray:
+        cells = np.ascontiguousarray(cells, dtype=np.float64).reshape(-1, 14)
+        mask = np.empty(cells.shape[0], dtype=np.uint64)
+        self._check(lib.cc_rsb_eval_masks(self.ctx, _ptr(cells), cells.shape[0], _ptr(mask)))
+        return mask
+
+    def prim_rn_selftest(self, a: np.ndarray, b: np.ndarray, c: np.ndarray) -> np.ndarray:
+        a, b, c = (np.ascontiguousarray(x, dtype=np.float64) for x in (a, b, c))
+        out = np.empty((a.size, 3), dtype=np.float64)
+        self._check(lib.cc_prim_rn_selftest(self.ctx, _ptr(a), _ptr(b), _ptr(c), a.size, _ptr(out)))
+        return out
+
+    def debug_trace_strip(self, box: np.ndarray, cap: int = 4096):
+        box = np.ascontiguousarray(box, dtype=np.float64)
+        tr = np.empty(3 * cap + 1, dtype=np.float64)
+        self._check(lib.cc_debug_trace_strip(self.ctx, _ptr(box), _ptr(tr), cap))
+        return tr[:-1].reshape(cap, 3), bool(tr[-1])
+
+    def prim_selftest(self, a: np.ndarray, b: np.ndarray) -> np.ndarray:
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        out = np.empty((a.size, 8), dtype=np.float64)
+        self._check(lib.cc_prim_selftest(self.ctx, _ptr(a), _ptr(b), a.size, _ptr(out)))
+        return out

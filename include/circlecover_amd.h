@@ -1,0 +1,128 @@
+/* circlecover_amd — C ABI of the MI355X (gfx950) interval branch-and-bound prover.
+ *
+ * Drop-in boundary for the reference's CUDA path
+ * (phillip-keldenich/circlecover, directory Prover/).  Plain C types only:
+ * no exceptions, no C++ or torch types cross this boundary.  Every function
+ * returning int returns CC_OK (0) or one of the CC_E_* codes; the message of
+ * the last failure on a context is available through cc_last_error().
+ *
+ * Threading: a context is used by one host thread at a time.  Functions
+ * ending in _async enqueue work on the context's HIP stream and take DEVICE
+ * pointers; all other functions are synchronous and take HOST pointers.
+ */
+#ifndef CIRCLECOVER_AMD_H_
+#define CIRCLECOVER_AMD_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CC_OK 0
+#define CC_E_HIP 1         /* HIP runtime error (reference: std::system_error, ACW/error.hpp:84-98) */
+#define CC_E_OVERFLOW 2    /* output capacity too small; *n_out holds the size needed
+                              (reference: relaunch with exact size, TR/tight_rectangle.cu:281-294) */
+#define CC_E_TRAP 3        /* a geometric invariant failed in a cell (reference: device trap, ACU/exit.cuh:28-31) */
+#define CC_E_INTERRUPTED 4 /* cc_interrupt() was called (reference: SIGINT flag, P/interrupt.cpp:30-46) */
+#define CC_E_ARG 5         /* invalid argument */
+#define CC_E_NODEVICE 6    /* no usable gfx950 device */
+
+/* An FP64 interval; layout of algcuda::Interval<double> (ACU/interval.hpp:84-242). */
+typedef struct cc_iv {
+  double lb, ub;
+} cc_iv;
+
+/* tight_rectangle::Variables, 80 bytes (Prover/tight_rectangle/strategies.cuh:33-50):
+ * lambda and the four largest SQUARED radii. */
+typedef struct cc_tr_cell {
+  cc_iv la;
+  cc_iv r[4];
+} cc_tr_cell;
+
+/* rectangle_size_bound::Variables, 112 bytes (Prover/rectangle_size_bound/variables.hpp:33-110):
+ * lambda and the six largest SQUARED radii. */
+typedef struct cc_rsb_cell {
+  cc_iv la;
+  cc_iv r[6];
+} cc_rsb_cell;
+
+typedef struct cc_ctx cc_ctx;
+
+/* ---- context (replaces the implicit CUDA context and algcuda::device wrappers,
+ *      Prover/algcuda/cuda-cpp-wrapper/src/{device,memory,error}.cpp) ---- */
+int cc_init(int device, cc_ctx** out);
+void cc_destroy(cc_ctx* ctx);
+const char* cc_last_error(const cc_ctx* ctx);
+int cc_synchronize(cc_ctx* ctx);
+/* Device memory owned by the caller through the context's device. */
+int cc_dev_alloc(cc_ctx* ctx, size_t bytes, void** dptr);
+int cc_dev_free(cc_ctx* ctx, void* dptr);
+int cc_h2d(cc_ctx* ctx, void* dst, const void* src, size_t bytes);
+int cc_d2h(cc_ctx* ctx, void* dst, const void* src, size_t bytes);
+/* Stream-ordered timing: cc_timer_start records an event on the context
+ * stream, cc_timer_stop records a second one, waits for it and returns the
+ * elapsed milliseconds between the two. */
+int cc_timer_start(cc_ctx* ctx);
+int cc_timer_stop(cc_ctx* ctx, float* ms);
+
+/* ---- batched criterion evaluation (no reference counterpart: the reference
+ *      evaluates criteria only inside its enumeration kernels,
+ *      TR/tight_rectangle.cu:169-262, RSB/rectangle_size_bound.cu:161-236) ---- */
+#define CC_BATCH_C2 0      /* the ten-criterion 4-D box set (see cc_eval_batch) */
+#define CC_PREC_FP64 0     /* outward-rounded FP64 intervals */
+#define CC_PREC_FP32_PRE 1 /* FP32 outward pre-filter, undecided boxes re-run in FP64 */
+
+/* Evaluates criterion set `prover` on n boxes stored as 8 SoA FP64 arrays of
+ * stride ld >= n: (la.lb, la.ub, r1.lb, r1.ub, r2.lb, r2.ub, r3.lb, r3.ub).
+ * verdict_mask[i] bit k = criterion k proved box i; bit 16+k = criterion k
+ * hit a sqrt-domain trap.  first_ok[i] = lowest proving criterion or 255.
+ * Criteria (reference files under Prover/):
+ *   0 tight_rectangle::r1_strategies(la, r1, r2.ub)            r1_strategies.cu:27
+ *   1 r1_r2_strategies_simple(la, r1, r2, r3.ub)                r1_r2_strategies.cu:46
+ *   2 r1_r2_strategies_stacked(la, r1, r2, r3.ub)               r1_r2_strategies.cu:74
+ *   3 r1_r2_r3_strategy_horizontal(la, r1, r2, r3, ub_r4)       r1_r2_r3_strategies.cu:28
+ *   4 r1_r2_r3_strategy_cover_strip(la, r1, r2, r3, ub_r4)      r1_r2_r3_strategies.cu:69
+ *   5 rectangle_size_bound::shortcut_even_split_recursion       even_split_recursion.cu:42
+ *   6 shortcut_uneven_split_recursion(la, r1, r2)               uneven_split_recursion.cu:78
+ *   7 shortcut_uneven_split_recursion(la, r1, r2, r3)           uneven_split_recursion.cu:102
+ *   8 shortcut_r1_r2_opposite_corners_recursion                 r1_r2_opposite_corners_recursion.cu:28
+ *   9 shortcut_r1_r2_vertical_strip                             r1_r2_vertical_strip.cu:28
+ * with ub_r4 = min(r3.ub, (required_weight_for(la)-r1-r2-r3).ub), TR/tight_rectangle.cu:97-102. */
+int cc_eval_batch(cc_ctx* ctx, int prover, int precision, const double* soa, size_t n, size_t ld,
+                  uint32_t* verdict_mask, uint8_t* first_ok);
+int cc_eval_batch_async(cc_ctx* ctx, int prover, int precision, const double* d_soa, size_t n, size_t ld,
+                        uint32_t* d_verdict_mask, uint8_t* d_first_ok);
+
+/* rectangle_size_bound verdict of n 7-D cells (cc_rsb_cell layout), each
+ * evaluated exactly as the reference kernel treats a leaf cell
+ * (RSB/rectangle_size_bound.cu:162-229): shortcuts S1..S5, tighten(),
+ * Intermediate_values, criteria C1..C14 in order.  verdict[i] = index of the
+ * first succeeding step (S1..S5 = 0..4, empty/vacuous cell = 5, C1..C14 =
+ * 6..19) or -1 if unproven; trap[i] = 1 if a reference trap site was hit. */
+int cc_rsb_eval_cells(cc_ctx* ctx, const cc_rsb_cell* cells, size_t n, int32_t* verdict, uint8_t* trap);
+/* Every step of cc_rsb_eval_cells evaluated independently (no
+ * short-circuit): bit k of mask[i] = step k succeeds, bit 20+k = step k hit a
+ * reference trap site.  Used to pin each criterion separately. */
+int cc_rsb_eval_masks(cc_ctx* ctx, const cc_rsb_cell* cells, size_t n, uint64_t* mask);
+int cc_rsb_eval_cells_async(cc_ctx* ctx, const cc_rsb_cell* d_cells, size_t n, int32_t* d_verdict, uint8_t* d_trap);
+
+/* Synthetic boxes (host): splitmix64(seed); la.lb ~ U[1,2.5]; three radius
+ * lower bounds ~ U[0.02,0.7] sorted descending; each width 2^U[-20,-6];
+ * ub = lb + width rounded upward.  Writes 8*n doubles in SoA order (ld = n). */
+void cc_gen_c2_boxes(uint64_t seed, size_t n, double* soa);
+
+/* Directed-rounding primitive self-test: out[8*i+k] for k = add_rd, add_ru,
+ * mul_rd, mul_ru, div_rd, div_ru, sqrt_rd(a), sqrt_ru(a). */
+int cc_prim_selftest(cc_ctx* ctx, const double* a, const double* b, size_t n, double* out);
+/* Round-to-nearest helpers self-test: out[3*i+k] = 0.5*(a+b), fma(a,b,c), (a-b)/c. */
+int cc_prim_rn_selftest(cc_ctx* ctx, const double* a, const double* b, const double* c, size_t n, double* out);
+/* Diagnostics: lo/hi/mid of every iteration (3*cap doubles) of the cover-strip
+ * bisection (criterion 4) for one box given as 8 doubles; trace[3*cap] = verdict. */
+int cc_debug_trace_strip(cc_ctx* ctx, const double* box, double* trace, int cap);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,136 @@
+"""CPU oracle — TEST INFRASTRUCTURE ONLY.
+
+ctypes loader of oracle/liboracle.so (and its op-counting twin), the CPU
+restatement of the reference's interval arithmetic and covering criteria.
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may use
+this package, and only as the checker; the product never imports it.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_DIR = os.path.dirname(os.path.abspath(__file__))
+_P = ctypes.POINTER
+_vp, _sz, _i, _d = ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int, ctypes.c_double
+
+
+def _load(name: str) -> ctypes.CDLL:
+    path = os.path.join(_DIR, name)
+    if not os.path.exists(path):
+        raise ImportError(f"{path} missing; run `make -C oracle`")
+    L = ctypes.CDLL(path)
+    L.or_c2_eval.argtypes = [_vp, _sz, _sz, _vp, _vp, _i]
+    L.or_c2_count_ops.argtypes = [_vp, _sz, _sz, _vp]
+    L.or_prim.argtypes = [_i, _i, _d, _d]
+    L.or_prim.restype = _d
+    L.or_tr_critical_ratio.argtypes = [_d, _d, _vp]
+    L.or_tr_critical_ratio.restype = None
+    L.or_two_disks_maximize_height.argtypes = [_d, _d, _d]
+    L.or_two_disks_maximize_height.restype = _d
+    L.or_disk_satisfies_size_bound.argtypes = [_d, _d]
+    L.or_bound_required_height.argtypes = [_d]
+    L.or_bound_required_height.restype = _d
+    L.or_intersection.argtypes = [_vp, _vp, _vp]
+    L.or_iv_op.argtypes = [_i, _vp, _vp, _vp]
+    L.or_iv_op.restype = None
+    L.or_rn.argtypes = [_d, _d, _d, _vp]
+    L.or_rn.restype = None
+    L.or_trace_strip.argtypes = [_vp, _vp, _i]
+    L.or_rsb_verdicts.argtypes = [_vp, _sz, _vp, _vp, _i]
+    L.or_rsb_masks.argtypes = [_vp, _sz, _vp, _i]
+    return L
+
+
+lib = _load("liboracle.so")
+_count_lib = None
+
+
+def count_lib() -> ctypes.CDLL:
+    global _count_lib
+    if _count_lib is None:
+        _count_lib = _load("liboracle_count.so")
+    return _count_lib
+
+
+def c2_eval(soa: np.ndarray, threads: int = 0):
+    soa = np.ascontiguousarray(soa, dtype=np.float64)
+    n = soa.shape[1]
+    mask = np.empty(n, dtype=np.uint32)
+    first = np.empty(n, dtype=np.uint8)
+    lib.or_c2_eval(soa.ctypes.data, n, n, mask.ctypes.data, first.ctypes.data, threads)
+    return mask, first
+
+
+def c2_count_ops(soa: np.ndarray) -> np.ndarray:
+    """Per-criterion directed-FP64-op totals over the boxes (counting build)."""
+    soa = np.ascontiguousarray(soa, dtype=np.float64)
+    ops = np.zeros(10, dtype=np.uint64)
+    rc = count_lib().or_c2_count_ops(soa.ctypes.data, soa.shape[1], soa.shape[1], ops.ctypes.data)
+    if rc != 0:
+        raise RuntimeError("oracle built without op counting")
+    return ops
+
+
+PRIM_OPS = {"add": 0, "mul": 1, "div": 2, "sqrt": 3}
+
+
+def prim(op: str, direction: str, a: float, b: float = 0.0) -> float:
+    return lib.or_prim(PRIM_OPS[op], 1 if direction == "ru" else 0, a, b)
+
+
+def iv_op(op: int, a, b=(0.0, 0.0)):
+    x = (ctypes.c_double * 2)(*a)
+    y = (ctypes.c_double * 2)(*b)
+    out = (ctypes.c_double * 3)()
+    lib.or_iv_op(op, ctypes.addressof(x), ctypes.addressof(y), ctypes.addressof(out))
+    return (out[0], out[1]), bool(out[2])
+
+
+def rsb_verdicts(cells: np.ndarray, threads: int = 0):
+    """Reference-order verdict of (n, 14) rectangle_size_bound cells."""
+    cells = np.ascontiguousarray(cells, dtype=np.float64).reshape(-1, 14)
+    n = cells.shape[0]
+    out = np.empty(n, dtype=np.int32)
+    trap = np.empty(n, dtype=np.uint8)
+    lib.or_rsb_verdicts(cells.ctypes.data, n, out.ctypes.data, trap.ctypes.data, threads)
+    return out, trap
+
+
+def rsb_masks(cells: np.ndarray, threads: int = 0) -> np.ndarray:
+    """Every rectangle_size_bound step evaluated on each cell (bit k = step k,
+    bit 20+k = trap in step k)."""
+    cells = np.ascontiguousarray(cells, dtype=np.float64).reshape(-1, 14)
+    out = np.empty(cells.shape[0], dtype=np.uint64)
+    lib.or_rsb_masks(cells.ctypes.data, cells.shape[0], out.ctypes.data, threads)
+    return out
+
+
+def rn_helpers(a: float, b: float, c: float):
+    out = (ctypes.c_double * 3)()
+    lib.or_rn(a, b, c, ctypes.addressof(out))
+    return tuple(out)
+
+
+def trace_strip(box, cap: int = 4096):
+    box = np.ascontiguousarray(box, dtype=np.float64)
+    tr = np.zeros(3 * cap, dtype=np.float64)
+    ok = lib.or_trace_strip(box.ctypes.data, tr.ctypes.data, cap)
+    return tr.reshape(cap, 3), bool(ok)
+
+
+def tr_critical_ratio(lb: float, ub: float):
+    out = (ctypes.c_double * 2)()
+    lib.or_tr_critical_ratio(lb, ub, ctypes.addressof(out))
+    return out[0], out[1]
+
+
+def intersection(c1, c2):
+    a = (ctypes.c_double * 3)(*c1)
+    b = (ctypes.c_double * 3)(*c2)
+    out = (ctypes.c_double * 8)()
+    ok = lib.or_intersection(ctypes.addressof(a), ctypes.addressof(b), ctypes.addressof(out))
+    pts = [((out[0], out[1]), (out[2], out[3])), ((out[4], out[5]), (out[6], out[7]))]
+    return bool(ok), pts
