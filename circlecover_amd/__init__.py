@@ -6,12 +6,17 @@ is no fallback: if the native library is missing or no gfx950 device is
 present, the calls raise.
 """
 from .native import (  # noqa: F401
+    C2_CRITERIA,
+    DBL_MAX,
+    TR_LAMBDA_MAX,
     CCError,
     Prover,
     gen_c2_boxes,
     lib,
     library_path,
-    C2_CRITERIA,
+    subintervals,
+    tr_initial_cells,
 )
 
-__all__ = ["CCError", "Prover", "gen_c2_boxes", "lib", "library_path", "C2_CRITERIA"]
+__all__ = ["CCError", "Prover", "gen_c2_boxes", "lib", "library_path", "C2_CRITERIA", "DBL_MAX", "TR_LAMBDA_MAX",
+           "subintervals", "tr_initial_cells"]

@@ -1,10 +1,10 @@
 """In-tree build of the gfx950 prover library (no cmake, no JIT cache).
 
-Pipeline for the round-down kernels (csrc/prover_rd.hip):
+Pipeline for the round-down kernels (every csrc/*.hip, compiled in parallel):
   1. hipcc --cuda-device-only -S            -> gfx950 assembly
   2. set `.amdhsa_float_round_mode_32/_16_64 2` in every ccrd_* kernel
      descriptor, so each wave starts in round-toward-negative-infinity mode
-  3. clang -cc1as + ld.lld                  -> code object (.hsaco)
+  3. clang -cc1as per unit, ld.lld of all  -> one code object (.hsaco)
   4. the code object is embedded (.incbin) into libcirclecover_amd.so, which
      loads it with hipModuleLoadData.
 Host code is plain C++ over the HIP runtime API.
@@ -63,25 +63,45 @@ def patch_round_mode(asm: str) -> tuple[str, int]:
     return "".join(out), n
 
 
+def _compile_rd(src: str) -> subprocess.Popen:
+    name = os.path.splitext(os.path.basename(src))[0]
+    asm = os.path.join(BUILD, name + ".s")
+    return subprocess.Popen(["hipcc", *DEVICE_FLAGS, f"-I{CSRC}", f"-I{ROOT}/include", "--cuda-device-only", "-S",
+                             "-o", asm, src], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+
+
 def build_rd_module() -> str:
+    """Every csrc/*.hip translation unit -> patched assembly -> one object
+    each -> a single code object holding all kernels."""
     os.makedirs(BUILD, exist_ok=True)
-    src = os.path.join(CSRC, "prover_rd.hip")
-    asm = os.path.join(BUILD, "prover_rd.s")
-    _run(["hipcc", *DEVICE_FLAGS, f"-I{CSRC}", f"-I{ROOT}/include", "--cuda-device-only", "-S", "-o", asm, src])
-    text = open(asm).read()
-    patched, n = patch_round_mode(text)
-    kernels = set(re.findall(r"\.amdhsa_kernel\s+(ccrd_\S+)", text))
-    if n == 0 or n != len(kernels):
-        raise RuntimeError("round-mode patch did not reach every ccrd_ kernel")
-    if patched.count(".amdhsa_float_round_mode_16_64 2") != n:
-        raise RuntimeError("unexpected descriptor layout in compiler output")
-    pasm = os.path.join(BUILD, "prover_rd.patched.s")
-    open(pasm, "w").write(patched)
-    obj = os.path.join(BUILD, "prover_rd.o")
-    _run([os.path.join(LLVM, "clang"), "-cc1as", "-triple", "amdgcn-amd-amdhsa", "-target-cpu", "gfx950",
-          "-filetype", "obj", "-o", obj, pasm])
+    srcs = sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".hip"))
+    procs = [(src, _compile_rd(src)) for src in srcs]
+    failed = []
+    for src, p in procs:
+        out, _ = p.communicate()
+        if p.returncode != 0:
+            failed.append(src)
+            sys.stderr.write(out)
+    if failed:
+        raise RuntimeError(f"device compile failed: {failed}")
+    objs = []
+    for src in srcs:
+        name = os.path.splitext(os.path.basename(src))[0]
+        text = open(os.path.join(BUILD, name + ".s")).read()
+        patched, n = patch_round_mode(text)
+        kernels = set(re.findall(r"\.amdhsa_kernel\s+(ccrd_\S+)", text))
+        if n != len(kernels):
+            raise RuntimeError(f"round-mode patch did not reach every ccrd_ kernel of {src}")
+        if patched.count(".amdhsa_float_round_mode_16_64 2") != n:
+            raise RuntimeError("unexpected descriptor layout in compiler output")
+        pasm = os.path.join(BUILD, name + ".patched.s")
+        open(pasm, "w").write(patched)
+        obj = os.path.join(BUILD, name + ".o")
+        _run([os.path.join(LLVM, "clang"), "-cc1as", "-triple", "amdgcn-amd-amdhsa", "-target-cpu", "gfx950",
+              "-filetype", "obj", "-o", obj, pasm])
+        objs.append(obj)
     hsaco = os.path.join(BUILD, "prover_rd.hsaco")
-    _run([os.path.join(LLVM, "ld.lld"), "-shared", obj, "-o", hsaco])
+    _run([os.path.join(LLVM, "ld.lld"), "-shared", *objs, "-o", hsaco])
     return hsaco
 
 

@@ -193,7 +193,7 @@ CCD double div_rd_fast_pos(double a, double b) {
   return q;
 }
 
-__device__ __noinline__ double div_rd_slow(double a, double b);
+static __device__ __noinline__ double div_rd_slow(double a, double b);
 
 CCD bool div_fast_ok(double a, double b) {
   // both exponents within [-500, 499]: |x| in [2^-500, 2^500)
@@ -212,7 +212,7 @@ CCD double rcp_ru(double x) { return div_ru(1.0, x); }
 
 // Exact-sign slow path: special operands by IEEE rules, extreme exponents by
 // power-of-two rescaling (round-down compositions on nested grids are exact).
-__device__ __noinline__ double div_rd_slow(double a, double b) {
+static __device__ __noinline__ double div_rd_slow(double a, double b) {
   const double inf = __builtin_inf();
   const bool neg = ((bits(a) ^ bits(b)) >> 63) != 0;
   if (a != a || b != b) return __builtin_nan("");
@@ -268,7 +268,7 @@ CCD double sqrt_rd_fast(double x) {
   return g;
 }
 
-__device__ __noinline__ double sqrt_rd_slow(double x);
+static __device__ __noinline__ double sqrt_rd_slow(double x);
 
 CCD bool sqrt_fast_ok(double x) {
   // x in [2^-960, +inf): biased exponent in [63, 2046] and sign clear
@@ -287,7 +287,7 @@ CCD double sqrt_ru(double x) {
   }
   return fnma_rd(s, s, x) == 0.0 ? s : next_up(s);
 }
-__device__ __noinline__ double sqrt_rd_slow(double x) {
+static __device__ __noinline__ double sqrt_rd_slow(double x) {
   if (x != x || x < 0.0) return __builtin_nan("");
   if (x == 0.0 || __builtin_isinf(x)) return x;
   // 0 < x < 2^-960: sqrt(x*2^1000) = sqrt(x)*2^500, both scalings exact

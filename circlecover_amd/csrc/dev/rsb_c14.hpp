@@ -7,7 +7,7 @@ namespace rsbc {
 
 // ------------------------------------------------------------------ six-disk width
 // six_disks_cover_width.cu:30-93
-__device__ __noinline__ bool six_r56_center(double w, const cell7& c, Trap& tp) {
+static __device__ __noinline__ bool six_r56_center(double w, const cell7& c, Trap& tp) {
   const double r1 = c.r[0].lo, r2 = c.r[1].lo, r3 = c.r[2].lo, r4 = c.r[3].lo, r5 = c.r[4].lo, r6 = c.r[5].lo;
   double h14 = trc::two_disk_height(r1, r4, w);
   double h23 = trc::two_disk_height(r2, r3, w);
@@ -50,7 +50,7 @@ __constant__ static const unsigned char kPairs[15][6] = {
     {0, 2, 1, 5, 3, 4}, {0, 3, 1, 2, 4, 5}, {0, 3, 1, 4, 2, 5}, {0, 3, 1, 5, 2, 4}, {0, 4, 1, 2, 3, 5},
     {0, 4, 1, 3, 2, 5}, {0, 4, 1, 5, 2, 3}, {0, 5, 1, 2, 3, 4}, {0, 5, 1, 3, 2, 4}, {0, 5, 1, 4, 2, 3}};
 // six_disks_cover_width.cu:156-233
-__device__ __noinline__ double six_max_width(const cell7& c, Trap& tp) {
+static __device__ __noinline__ double six_max_width(const cell7& c, Trap& tp) {
   double lbs[6];
 #pragma unroll
   for (int i = 0; i < 6; ++i) lbs[i] = c.r[i].lo;
@@ -96,7 +96,7 @@ CCD bool r7_in_corner(const Konst& K, iv R, iv r7, double ub_w) {
   return can_recurse(K, R_no7, r7.hi, ub_w, add_ru(1.0, -sqrt_rd(h7)));
 }
 // seven_disk_strategies.cu:49-115 (ub_hrem rounded down, as the reference)
-__device__ __noinline__ bool top_strip_can_cover(double la, double t1, double t2, double t3, double a0, double a1,
+static __device__ __noinline__ bool top_strip_can_cover(double la, double t1, double t2, double t3, double a0, double a1,
                                                  double a2, double a3, double ub_max_disk, double R_no7) {
   double htop = three_height(t1, t2, t3, la);
   if (htop <= 0) return false;
@@ -138,7 +138,7 @@ CCD bool split_largest_works(const Konst& K, double ub_r7, double wrem, double w
   return can_recurse(K, rem2, ub_r7, ws, hs);
 }
 // seven_disk_strategies.cu:131-154
-__device__ __noinline__ void split_largest_r7(const Konst& K, iv& r7, double wrem, double wf, double hf, double ws,
+static __device__ __noinline__ void split_largest_r7(const Konst& K, iv& r7, double wrem, double wf, double hf, double ws,
                                               double hs) {
   if (split_largest_works(K, r7.hi, wrem, wf, hf, ws, hs)) {
     r7.lo = __builtin_inf();
@@ -156,7 +156,7 @@ __device__ __noinline__ void split_largest_r7(const Konst& K, iv& r7, double wre
   }
 }
 // seven_disk_strategies.cu:156-222
-__device__ __noinline__ void split_recursion(const Konst& K, const cell7& c, iv& r7, double wrem, int largest,
+static __device__ __noinline__ void split_recursion(const Konst& K, const cell7& c, iv& r7, double wrem, int largest,
                                              double wf, double hf, double ws, double hs) {
   if (largest > 5) {
     split_largest_r7(K, r7, wrem, wf, hf, ws, hs);
@@ -213,7 +213,7 @@ CCD r6pos place_r6(iv r6, const point& x1, const point& x2, iv lower, iv upper) 
   return pos;
 }
 // seven_disk_strategies.cu:264-316
-__device__ __noinline__ void r7_t_r5_large(const Konst& K, const cell7& c, iv& r7, const circle& c1,
+static __device__ __noinline__ void r7_t_r5_large(const Konst& K, const cell7& c, iv& r7, const circle& c1,
                                            const circle& c3, const circle& c5, double R6, double w12, double wrem,
                                            double hrem, double strip_w5, iv lower, iv upper, Trap& tp) {
   crossing x15 = intersect_circles(c1, c5, tp), x35 = intersect_circles(c3, c5, tp);
@@ -249,7 +249,7 @@ __device__ __noinline__ void r7_t_r5_large(const Konst& K, const cell7& c, iv& r
   }
 }
 // seven_disk_strategies.cu:318-518
-__device__ __noinline__ void r7_t_shaped(const Konst& K, const cell7& c, iv& r7, Trap& tp) {
+static __device__ __noinline__ void r7_t_shaped(const Konst& K, const cell7& c, iv& r7, Trap& tp) {
   double w2 = sqrt_rd(mul_rd(2.0, c.r[1].lo));
   double h12 = w2;
   double w1 = sqrt_rd(add_rd(mul_rd(4.0, c.r[0].lo), -mul_ru(h12, h12)));
@@ -370,7 +370,7 @@ __device__ __noinline__ void r7_t_shaped(const Konst& K, const cell7& c, iv& r7,
   }
 }
 // seven_disk_strategies.cu:520-554
-__device__ __noinline__ bool three_top_strip(const cell7& c, iv R, iv r7) {
+static __device__ __noinline__ bool three_top_strip(const cell7& c, iv R, iv r7) {
   const double R_no7 = add_rd(R.lo, -r7.hi);
   double d[7];
 #pragma unroll
@@ -388,7 +388,7 @@ __device__ __noinline__ bool three_top_strip(const cell7& c, iv R, iv r7) {
   return false;
 }
 // seven_disk_strategies.cu:556-718 (rem_width adds, as the reference)
-__device__ __noinline__ bool r435_strip(const Konst& K, const cell7& c, iv r7, iv R, Trap& tp) {
+static __device__ __noinline__ bool r435_strip(const Konst& K, const cell7& c, iv r7, iv R, Trap& tp) {
   double cov = three_height(c.r[2].lo, c.r[3].lo, c.r[4].lo, 1.0);
   if (cov <= 0.0) return false;
   iv wsq435{mul_rd(cov, cov), mul_ru(cov, cov)};
@@ -476,7 +476,7 @@ __device__ __noinline__ bool r435_strip(const Konst& K, const cell7& c, iv r7, i
   return can_recurse(K, add_rd(R.lo, -r7.hi), r7.hi, hr <= best.p[1].y.lo ? w67 : wide, hr);
 }
 // seven_disk_strategies.cu:961-1061 (`return false` from a double: 0.0)
-__device__ __noinline__ double five_height_23(double ub_w, double t1, double t2, double m1, double m2, double m3,
+static __device__ __noinline__ double five_height_23(double ub_w, double t1, double t2, double m1, double m2, double m3,
                                               Trap& tp) {
   strip2 bot = max_height_strip(ub_w, t1, t2, tp);
   if (bot.h <= 0.0) return 0.0;
@@ -557,7 +557,7 @@ CCD iv r7_range_for(const Konst& K, const cell7& c, iv R, double ub_w_remaining)
   return range;
 }
 // seven_disk_strategies.cu:1241-1297 (h1 not sign-checked, as the reference)
-__device__ __noinline__ bool six_222(const Konst& K, const cell7& c, iv R, iv r7) {
+static __device__ __noinline__ bool six_222(const Konst& K, const cell7& c, iv R, iv r7) {
   double c12 = trc::two_disk_height(c.r[0].lo, c.r[1].lo, 1.0);
   double c34 = trc::two_disk_height(c.r[2].lo, c.r[3].lo, 1.0);
   if (c34 <= 0.0) return false;
@@ -582,7 +582,7 @@ __device__ __noinline__ bool six_222(const Konst& K, const cell7& c, iv R, iv r7
   return can_recurse(K, smaller, r7.hi, wrect, hrem);
 }
 // seven_disk_strategies.cu:1299-1349 (h1 uses w12 unsquared, as the reference)
-__device__ __noinline__ bool diff_height_r12_22(const Konst& K, const cell7& c, iv R, iv r7) {
+static __device__ __noinline__ bool diff_height_r12_22(const Konst& K, const cell7& c, iv R, iv r7) {
   double w12 = sqrt_rd(mul_rd(2.0, c.r[1].lo));
   double h12 = add_rd(sqrt_rd(add_rd(mul_rd(4.0, c.r[0].lo), -w12)), w12);
   double wrem = add_ru(c.la.hi, -w12);
@@ -609,7 +609,7 @@ __device__ __noinline__ bool diff_height_r12_22(const Konst& K, const cell7& c, 
   return R8 >= mul_ru(area, cr);
 }
 // seven_disk_strategies.cu:1351-1394
-__device__ __noinline__ void r7_four_disk_r1_square(const Konst& K, const cell7& c, iv R, iv& r7) {
+static __device__ __noinline__ void r7_four_disk_r1_square(const Konst& K, const cell7& c, iv R, iv& r7) {
   double lbs[4] = {c.r[1].lo, c.r[2].lo, c.r[3].lo, c.r[4].lo};
   double h2345 = nd_maximize(c.la.hi, lbs, 4);
   if (h2345 <= 0.0) return;
@@ -637,7 +637,7 @@ __device__ __noinline__ void r7_four_disk_r1_square(const Konst& K, const cell7&
 // ------------------------------------------------------------------ C14
 // seven_disk_strategies.cu:1396-1496.  The reference prints a diagnostic
 // line for each failing cell; here the failing cell is simply left unproven.
-__device__ __noinline__ bool c14_seven_disk(const Konst& K, const cell7& c, const vals7& v, Trap& tp) {
+static __device__ __noinline__ bool c14_seven_disk(const Konst& K, const cell7& c, const vals7& v, Trap& tp) {
   double covered = six_max_width(c, tp);
   if (covered >= c.la.hi) return true;
   iv w_rem = c.la - covered;

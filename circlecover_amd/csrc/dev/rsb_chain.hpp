@@ -45,7 +45,7 @@ CCD iv efficient_cover_range(double lb_w, iv h) {
 }
 
 // bound_critical_ratio: can_recurse.cu:90-161
-__device__ __noinline__ double bound_crit_ratio(const Konst& K, double lb_rem, double ub_largest, double ub_w,
+static __device__ __noinline__ double bound_crit_ratio(const Konst& K, double lb_rem, double ub_largest, double ub_w,
                                                 double ub_h) {
   const double inf = __builtin_inf();
   if (lb_rem < 0.0) return inf;
@@ -74,7 +74,7 @@ __device__ __noinline__ double bound_crit_ratio(const Konst& K, double lb_rem, d
 }
 
 // recursion_bound_size: can_recurse.cu:163-218
-__device__ __noinline__ double recursion_bound(const Konst& K, double lb_rem, double ub_w, double ub_h) {
+static __device__ __noinline__ double recursion_bound(const Konst& K, double lb_rem, double ub_w, double ub_h) {
   const double inf = __builtin_inf();
   if (lb_rem <= 0.0) {
     if (lb_rem < 0.0) return 0.0;
@@ -147,7 +147,7 @@ CCD bool three_can_cover(double r1, double r2, double r3, double w, double h) {
   return add_rd(w1, add_rd(w2, w3)) >= w;
 }
 // three_disks_maximize_height: maximize_covered.cu:198-237
-__device__ __noinline__ double three_height(double r1, double r2, double r3, double ub_w) {
+static __device__ __noinline__ double three_height(double r1, double r2, double r3, double ub_w) {
   const double q1 = mul_rd(4.0, r1), q2 = mul_rd(4.0, r2), q3 = mul_rd(4.0, r3);
   double lo = 0.0, hi = mul_rd(2.0, sqrt_rd(r3));
   for (int it = 0; it < kBisectCap; ++it) {
@@ -174,7 +174,7 @@ CCD bool nd_can_cover(double ub_w, double ub_h, const double* d, int n) {
   return nd_can_cover_no_small(ub_w, ub_h, d, n);
 }
 // nd_maximize_covered: maximize_covered.cu:73-104
-__device__ __noinline__ double nd_maximize(double ub_w, const double* d, int n) {
+static __device__ __noinline__ double nd_maximize(double ub_w, const double* d, int n) {
   if (n == 1) {
     double h = add_rd(mul_rd(4.0, d[0]), -mul_ru(ub_w, ub_w));
     return h <= 0 ? 0.0 : sqrt_rd(h);
@@ -206,7 +206,7 @@ CCD strip2 max_height_strip(double la, double r1, double r2, Trap& tp) {
   return strip2{circle{{0.5 * w1, h}, pt(r1)}, circle{{la - 0.5 * w2, h}, pt(r2)}, mul_rd(2.0, h.lo)};
 }
 // two_disks_maximal_height_strip_wc_recursion: maximize_covered.cu:239-278
-__device__ __noinline__ strip2 max_height_strip_wc(double lr1, double lr2, double lb_R, double ub_w) {
+static __device__ __noinline__ strip2 max_height_strip_wc(double lr1, double lr2, double lb_R, double ub_w) {
   double lo = 0.0, hi = mul_rd(2.0, lr2);
   iv c1x = pt(0.0), c2x = pt(0.0);
   const double q1 = mul_rd(4.0, lr1), q2 = mul_rd(4.0, lr2);
@@ -357,7 +357,7 @@ CCD bool stacked_34_horizontal(const cell7& c) {
   return rem >= 1.0 || disk_fits(c.r[4].hi, rem);
 }
 // multi_disk_strip.cu:95-145
-__device__ __noinline__ bool stacked_34_56_horizontal(const cell7& c) {
+static __device__ __noinline__ bool stacked_34_56_horizontal(const cell7& c) {
   double wgt = 0.0;
   for (int j = 0; j < 6; ++j) wgt = add_ru(wgt, c.r[j].hi);
   double w = div_ru(wgt, kRsbRatio);
@@ -383,7 +383,7 @@ __device__ __noinline__ bool stacked_34_56_horizontal(const cell7& c) {
   return rem >= 1.0 || disk_fits(c.r[5].hi, rem);
 }
 // multi_disk_strip.cu:147-240 (lb_h14 / lb_h23 take sqrt twice, as the reference)
-__device__ __noinline__ bool six_strip_three_rows(const cell7& c, const vals7& v, Trap& tp) {
+static __device__ __noinline__ bool six_strip_three_rows(const cell7& c, const vals7& v, Trap& tp) {
   double wgt = 0.0;
   for (int i = 0; i < 6; ++i) wgt = add_ru(wgt, c.r[i].hi);
   double wn = div_ru(wgt, kRsbRatio);
@@ -468,7 +468,7 @@ CCD bool c4_wall_building(const Konst& K, const cell7& c, const vals7& v) {
 // r1_in_corner_explicit_recursion.cu:36-187: r1 in the corner, the other
 // disks stacked above it (right = false) or beside it (right = true) in
 // backtracked groups.
-__device__ __noinline__ bool corner_explicit(const cell7& c, const vals7& v, int nd, bool right) {
+static __device__ __noinline__ bool corner_explicit(const cell7& c, const vals7& v, int nd, bool right) {
   double lbs[6];
 #pragma unroll
   for (int i = 0; i < 6; ++i) lbs[i] = c.r[i].lo;
@@ -620,7 +620,7 @@ CCD size_check opposite_sizes(const cell7& c, const vals7& v, iv wA, iv hA, iv w
   return size_check{r7, r7, true};
 }
 // r1_r2_opposite_corners_recursion.cu:254-340
-__device__ __noinline__ bool c6_opposite_corners(const cell7& c, const vals7& v, Trap& tp) {
+static __device__ __noinline__ bool c6_opposite_corners(const cell7& c, const vals7& v, Trap& tp) {
   iv w1{sqrt_rd(mul_rd(2.0, c.r[0].lo)), sqrt_ru(mul_ru(2.0, c.r[0].hi))};
   iv w2{sqrt_rd(mul_rd(2.0, c.r[1].lo)), sqrt_ru(mul_ru(2.0, c.r[1].hi))};
   iv both = w1 + w2;

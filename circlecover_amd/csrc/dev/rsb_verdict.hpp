@@ -1,0 +1,57 @@
+// rectangle_size_bound verdict of one 7-D cell in the reference's order
+// (Prover/rectangle_size_bound/rectangle_size_bound.cu:162-229).  Return
+// codes: index of the first succeeding step (S1..S5 = 0..4, empty/vacuous
+// cell = 5, C1..C14 = 6..19) or -1 when the cell is not proven; `tp` is set
+// when a reference trap site was reached.
+#pragma once
+#include "rsb_chain.hpp"
+
+namespace ccd {
+
+// The leaf part (rectangle_size_bound.cu:186-229): tighten, the remaining
+// weight R, emptiness, then C1..C14.
+CCD int rsb_chain_verdict(const Konst& K, const cell7& in, Trap& tp) {
+  const cell7 c = rsbc::tighten(in);
+  vals7 v;
+  v.R = rsbc::remaining_weight(c);
+  if (rsbc::is_empty(c) || empty(v.R)) return 5;
+  if (rsbc::c1_even_split(c, v)) return 6;
+  if (rsbc::c2_uneven_split(c)) return 7;
+  if (rsbc::c3_multi_disk_strip(c, v, tp)) return 8;
+  if (rsbc::c4_wall_building(K, c, v)) return 9;
+  if (rsbc::c5_corner_explicit(c, v, tp)) return 10;
+  if (rsbc::c6_opposite_corners(c, v, tp)) return 11;
+  if (rsbc::c7_vertical_strip(K, c, v)) return 12;
+  const rsbc::cover22 cov = rsbc::two_by_two(c, tp);
+  v.cover_width = cov.width;
+  v.cover_c4 = cov.c4;
+  if (cov.width > 0 && (cov.width > c.la.hi || rsbc::c8_two_by_two_strip(K, c, v, tp))) return 13;
+  if (rsbc::c9_l_shaped(K, c)) return 14;
+  if (rsbc::c10_corner_wall(K, c, tp)) return 15;
+  if (rsbc::c11_advanced_strip(c)) return 16;
+  if (rsbc::c12_five_disk(K, c, v, tp)) return 17;
+  if (rsbc::c13_six_disk(K, c, v, tp)) return 18;
+  if (rsbc::c14_seven_disk(K, c, v, tp)) return 19;
+  return -1;
+}
+
+// The cell-level shortcuts S1..S5 (rectangle_size_bound.cu:162 and :174-176)
+// on the untightened cell, then the leaf part.
+CCD int rsb_cell_verdict(const Konst& K, const cell7& in, Trap& tp) {
+  if (rsbc::s1_even_split(in.la, in.r[0], in.r[1])) return 0;
+  if (rsbc::s23_uneven_split(in.la, in.r[0], in.r[1])) return 1;
+  if (rsbc::s23_uneven_split(in.la, in.r[0], in.r[2])) return 2;
+  if (rsbc::s4_opposite_corners(in.la, in.r[0], in.r[1], in.r[2])) return 3;
+  if (rsbc::s5_vertical_strip(K, in.la, in.r[0], in.r[1], in.r[2])) return 4;
+  return rsb_chain_verdict(K, in, tp);
+}
+
+CCD cell7 load_cell7(const double* __restrict__ p) {
+  cell7 c;
+  c.la = iv{p[0], p[1]};
+#pragma unroll
+  for (int k = 0; k < 6; ++k) c.r[k] = iv{p[2 + 2 * k], p[3 + 2 * k]};
+  return c;
+}
+
+}  // namespace ccd

@@ -71,7 +71,7 @@ CCD iv weight_for(const Konst& K, iv w, iv h) { return crit_ratio(K, w, h) * w *
 CCD iv weight_for(const Konst& K, iv la) { return crit_ratio(K, la) * la; }
 
 // can_recurse(ub_w, ub_h, ub_largest, lb_rem): can_recurse.cu:32-62
-__device__ __noinline__ bool can_recurse(const Konst& K, double ub_w, double ub_h, double ub_largest, double lb_rem) {
+static __device__ __noinline__ bool can_recurse(const Konst& K, double ub_w, double ub_h, double ub_largest, double lb_rem) {
   double cr = crit_ratio(K, pt(ub_w), pt(ub_h)).hi;
   double area = mul_ru(ub_w, ub_h);
   double side = ub_w < ub_h ? ub_w : ub_h;
@@ -236,6 +236,219 @@ CCD bool r123_cover_strip(const Konst& K, iv la, iv W, iv r1, iv r2, iv r3, doub
     else hi = mid;
   }
   return can_recurse(K, add_ru(la.hi, -lo), 1.0, ub_r4, (W - r1 - r2 - r3).lo);
+}
+
+CCD bool r123_strategies(const Konst& K, iv la, iv W, iv r1, iv r2, iv r3, double ub_r4) {  // :96-99
+  return r123_horizontal(K, la, W, r1, r2, r3, ub_r4) || r123_cover_strip(K, la, W, r1, r2, r3, ub_r4);
+}
+CCD bool r1_r2_strategies(const Konst& K, iv la, iv W, iv r1, iv r2, double ub_r3, Trap& tp) {  // :132-134
+  return r1_r2_simple(K, la, W, r1, r2, ub_r3, tp) || r1_r2_stacked(K, la, W, r1, r2, ub_r3);
+}
+
+// ---------------------------------------------------------------- leaf chain
+// A leaf: lambda and the four squared radii (tightened), as in
+// kernel_find_critical_intervals_in (tight_rectangle.cu:238-244).
+struct cell5 {
+  iv la;
+  iv r[4];
+};
+
+// three_disk_recursion_big_strip: r1_r2_r3_strategies.cu:101-121
+CCD bool three_big_strip(const Konst& K, double la, double r1, double r2, double r3, double ub_r4, double R4) {
+  double ssq = add_rd(add_rd(mul_rd(2.0, r2), add_rd(mul_rd(2.0, r3), -0.25)),
+                      add_rd(add_rd(-mul_ru(4.0, mul_ru(r2, r2)), mul_rd(8.0, mul_rd(r2, r3))),
+                             -mul_ru(4.0, mul_ru(r3, r3))));
+  if (ssq <= 0.0) return false;
+  double la_rem = add_ru(la, -sqrt_rd(ssq));
+  double h1sq = add_rd(mul_rd(4.0, r1), -mul_ru(la_rem, la_rem));
+  if (h1sq <= 0.0) return false;
+  if (h1sq >= 1.0) return true;
+  return can_recurse(K, la_rem, add_ru(1.0, -sqrt_rd(h1sq)), ub_r4, R4);
+}
+
+// three_disk_recursion_small_corner: r1_r2_r3_strategies.cu:123-165
+CCD bool three_small_corner(const Konst& K, double la, double r1, double r2, double r3, double ub_r4, double R4) {
+  double c3 = sqrt_rd(mul_rd(2.0, r3));
+  if (c3 >= 1.0) return false;
+  double la_rem = add_ru(la, -c3);
+  double la_half = mul_ru(la_rem, 0.5);
+  double one_rem = add_ru(1.0, -c3);
+  if (la_half <= sqrt_rd(r2)) {
+    double c1 = sqrt_rd(add_rd(mul_rd(4.0, r1), -mul_ru(one_rem, one_rem)));
+    double c2 = sqrt_rd(add_rd(mul_rd(4.0, r2), -mul_ru(la_rem, la_rem)));
+    return can_recurse(K, add_ru(la, -c1), add_ru(1.0, -c2), ub_r4, R4);
+  }
+  if (la_half >= sqrt_rd(r1)) return false;
+  if (one_rem >= mul_rd(2.0, sqrt_rd(r2))) return false;
+  double c2 = sqrt_rd(add_rd(mul_rd(4.0, r2), -mul_ru(one_rem, one_rem)));
+  double c1 = sqrt_rd(add_rd(mul_rd(4.0, r1), -mul_ru(la_rem, la_rem)));
+  return can_recurse(K, add_ru(la, -c2), add_ru(1.0, -c1), ub_r4, R4);
+}
+
+// Shared prefix of the two pocket placements (r1_r2_r3_strategies.cu:168-183
+// and :222-235): S1, then h2, then S3; false when a radius is too small.
+CCD bool pocket_chain(iv la, iv r1, iv r2, iv r3, iv& S1, iv& S3, Trap& tp) {
+  S1 = 2.0 * sqrt(r1 - 0.25, tp);
+  iv a = la - S1;
+  iv h2_rest = 0.25 * a * a;
+  if (r2.lo <= h2_rest.hi) return false;
+  iv h2 = 2.0 * sqrt(r2 - h2_rest, tp);
+  iv b = 1.0 - h2;
+  iv s3_rest = 0.25 * b * b;
+  if (r3.lo <= s3_rest.hi) return false;
+  S3 = 2.0 * sqrt(r3 - s3_rest, tp);
+  return true;
+}
+
+// three_disk_recursion_r1_r3_pocket: r1_r2_r3_strategies.cu:167-206
+CCD bool three_r1_r3_pocket(const Konst& K, iv la, iv r1, iv r2, iv r3, double ub_r4, double R4, Trap& tp) {
+  if (r1.lo <= 0.25) return false;
+  iv S1, S3;
+  if (!pocket_chain(la, r1, r2, r3, S1, S3, tp)) return false;
+  iv X = la - S1 - S3;
+  if (X.hi <= 0.0) return true;
+  iv c = la - 0.5 * S1 - S3;
+  iv y_rest = c * c;
+  if (r1.lo <= y_rest.hi) return false;
+  iv Y = 0.5 - sqrt(r1 - y_rest, tp);
+  if (Y.hi <= 0.0) return true;
+  return can_recurse(K, X.hi, Y.hi, ub_r4, R4);
+}
+
+// manually_proved_three_disk_pocket: r1_r2_r3_strategies.cu:208-247
+CCD bool three_manual_pocket(iv la, iv r1, iv r2, iv r3, Trap& tp) {
+  if (la.hi > 1.0499999999999998) return false;
+  iv lasq = square(la);
+  iv rstar = 0.0625 * lasq + 0.15625 + 0.03515625 / lasq;
+  iv d1 = rstar - r1;
+  if (d1.lo < -0.0009999999999999998 || d1.hi > 0.0009999999999999998) return false;
+  iv d2 = rstar - r2;
+  if (d2.lo < -0.0019999999999999996 || d2.hi > 0.0019999999999999996) return false;
+  iv d3 = rstar - r3;
+  if (d3.lo < -0.0029999999999999996 || d3.hi > 0.0029999999999999996) return false;
+  iv S1, S3;
+  if (!pocket_chain(la, r1, r2, r3, S1, S3, tp)) return false;
+  iv X = la - S1 - S3;
+  return !(X.hi > 0.046999999999999993);
+}
+
+// r1_r2_r3_strategies(vars, R4): r1_r2_r3_strategies.cu:249-260
+CCD bool three_disk_leaf(const Konst& K, const cell5& c, iv R4, Trap& tp) {
+  const double la = c.la.hi, r1 = c.r[0].lo, r2 = c.r[1].lo, r3 = c.r[2].lo;
+  const double ub_r4 = c.r[3].hi, R4lo = R4.lo;
+  return three_big_strip(K, la, r1, r2, r3, ub_r4, R4lo) || three_small_corner(K, la, r1, r2, r3, ub_r4, R4lo) ||
+         three_r1_r3_pocket(K, c.la, c.r[0], c.r[1], c.r[2], ub_r4, R4lo, tp) ||
+         three_manual_pocket(c.la, c.r[0], c.r[1], c.r[2], tp);
+}
+
+// two_by_two_strategies.cu:29-51
+CCD bool two_by_two_r1_large(const Konst& K, const cell5& c, iv R) {
+  double w1 = sqrt_rd(add_rd(mul_rd(4.0, c.r[0].lo), -1.0));
+  double wrem = add_ru(c.la.hi, -w1);
+  double wrem_sq = mul_ru(wrem, wrem);
+  double h4 = add_rd(mul_rd(4.0, c.r[3].lo), -wrem_sq);
+  if (h4 <= 0.0) return false;
+  h4 = sqrt_rd(h4);
+  double h3 = sqrt_rd(add_rd(mul_rd(4.0, c.r[2].lo), -wrem_sq));
+  double h2 = sqrt_rd(add_rd(mul_rd(4.0, c.r[1].lo), -wrem_sq));
+  double htot = add_rd(h2, add_rd(h3, h4));
+  if (htot >= 1.0) return true;
+  double ub_r5 = c.r[3].hi < R.hi ? c.r[3].hi : R.hi;
+  return can_recurse(K, wrem, add_ru(1.0, -htot), ub_r5, R.lo);
+}
+
+// two_by_two_strategies.cu:53-83
+CCD bool two_by_two_small(const Konst& K, const cell5& c, iv R) {
+  double side = sqrt_rd(mul_rd(2.0, c.r[0].lo));
+  if (side >= 1.0) return two_by_two_r1_large(K, c, R);
+  double wrem = add_ru(c.la.hi, -side);
+  double hrem = add_ru(1.0, -side);
+  double h2 = add_rd(mul_rd(4.0, c.r[1].lo), -mul_ru(wrem, wrem));
+  double w4 = add_rd(mul_rd(4.0, c.r[3].lo), -mul_ru(hrem, hrem));
+  if (h2 <= 0.0 || w4 <= 0.0) return false;
+  h2 = sqrt_rd(h2);
+  w4 = sqrt_rd(w4);
+  if (w4 > side) w4 = side;
+  if (h2 > side) h2 = side;
+  double h = add_ru(1.0, -h2);
+  double w = add_ru(c.la.hi, -w4);
+  return mul_rd(4.0, c.r[2].lo) >= add_ru(mul_ru(h, h), mul_ru(w, w));
+}
+
+// two_by_two_strategies.cu:85-111: best of the three pairings into two
+// columns of two disks
+CCD bool two_by_two_big(const Konst& K, const cell5& c, iv R) {
+  const double q0 = c.r[0].lo, q1 = c.r[1].lo, q2 = c.r[2].lo, q3 = c.r[3].lo;
+  double w = add_rd(two_disk_height(q0, q1, 1.0), two_disk_height(q2, q3, 1.0));
+  double s = add_rd(two_disk_height(q0, q3, 1.0), two_disk_height(q1, q2, 1.0));
+  if (s > w) w = s;
+  s = add_rd(two_disk_height(q0, q2, 1.0), two_disk_height(q1, q3, 1.0));
+  if (s > w) w = s;
+  if (w <= 0.0) return false;
+  double ub_r5 = R.hi < c.r[3].hi ? R.hi : c.r[3].hi;
+  return can_recurse(K, add_ru(c.la.hi, -w), 1.0, ub_r5, R.lo);
+}
+
+CCD bool two_by_two_leaf(const Konst& K, const cell5& c, iv R) {  // :113-115
+  return two_by_two_small(K, c, R) || two_by_two_big(K, c, R);
+}
+
+// can_realize_gap_width: r1_r2_large_r3_r4_gaps_strategy.cu:29-57
+CCD bool gap_width_ok(double gap, double r1, double w1, double r2, double w2, double r4, Trap& tp) {
+  iv y4 = r4 - 0.25 * pt(gap) * gap;
+  if (possibly(y4 <= 0.0)) return false;
+  iv x1 = pt(0.5) * w1;
+  iv x2 = pt(0.5) * w2 + gap + w1;
+  circle c1{{x1, pt(0.5)}, pt(r1)};
+  circle c2{{x2, pt(0.5)}, pt(r2)};
+  crossing x12 = intersect_circles(c1, c2, tp);
+  if (!x12.sure) return false;
+  UB lower = x12.p[0].y <= x12.p[1].y;
+  if (!is_certain(lower)) return false;
+  point p4{pt(0.5) * gap + w1, sqrt(y4, tp)};
+  const point& px = x12.p[lower.lo ? 0 : 1];
+  return definitely(dist2(p4, px) <= r4);
+}
+
+// r1_r2_large_r3_r4_gaps_strategy.cu:59-98
+CCD bool gaps_leaf(const Konst& K, const cell5& c, iv R, Trap& tp) {
+  const double r1 = c.r[0].lo, r2 = c.r[1].lo, r4 = c.r[3].lo;
+  double w1 = add_rd(mul_rd(4.0, r1), -1.0);
+  double w2 = add_rd(mul_rd(4.0, r2), -1.0);
+  if (w1 <= 0.0 || w2 <= 0.0) return false;
+  w1 = sqrt_rd(w1);
+  w2 = sqrt_rd(w2);
+  double lo = 0.0, hi = add_rd(sqrt_rd(r1), sqrt_rd(r2));
+  for (int it = 0; it < kBisectCap; ++it) {
+    double mid = rn_mid(lo, hi);
+    if (mid <= lo || mid >= hi) break;
+    if (gap_width_ok(mid, r1, w1, r2, w2, r4, tp)) lo = mid;
+    else hi = mid;
+  }
+  double wrem = add_ru(c.la.hi, -add_rd(w1, add_rd(lo, w2)));
+  if (wrem <= 0.0) return true;
+  double ub_r5 = R.hi;
+  if (ub_r5 > c.r[3].hi) ub_r5 = c.r[3].hi;
+  return can_recurse(K, wrem, 1.0, ub_r5, R.lo);
+}
+
+// Valid squared-radius ranges: tight_rectangle.cu:72-102 (W = required_weight_for(la)).
+CCD iv r1_range(const Konst& K, iv la) {
+  double f = div_rd(crit_ratio(K, la).lo, kRsbRatio);
+  if (f <= 1.0) f = 1.0;
+  return iv{mul_rd(kDiskWeightMax, f), mul_ru(0.25, add_ru(1.0, mul_ru(la.hi, la.hi)))};
+}
+CCD iv r2_range(iv W, iv r1) {
+  double b = (W - r1).hi;
+  return iv{0.0, r1.hi < b ? r1.hi : b};
+}
+CCD iv r3_range(iv W, iv r1, iv r2) {
+  double b = (W - r1 - r2).hi;
+  return iv{0.0, r2.hi < b ? r2.hi : b};
+}
+CCD iv r4_range(iv W, iv r1, iv r2, iv r3) {
+  double b = (W - r1 - r2 - r3).hi;
+  return iv{0.0, r3.hi < b ? r3.hi : b};
 }
 
 }  // namespace trc

@@ -47,3 +47,17 @@ extern "C" void cc_gen_c2_boxes(uint64_t seed, size_t n, double* soa) {
     }
   }
 }
+
+extern "C" {
+
+// get_subinterval (Prover/subintervals.hpp:32-43) on the host: the n pieces of
+// [lb, ub] as 2n doubles, lb + diff*i fused (one rounding) as on the device.
+void cc_subintervals(double lb, double ub, int n, double* out) {
+  const double diff = (ub - lb) / (double)n;
+  for (int i = 0; i < n; ++i) {
+    out[2 * i] = std::fma(diff, (double)i, lb);
+    out[2 * i + 1] = (i + 1 == n) ? ub : std::fma(diff, (double)(i + 1), lb);
+  }
+}
+
+}  // extern "C"

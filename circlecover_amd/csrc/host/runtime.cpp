@@ -18,6 +18,26 @@ int hip_fail(cc_ctx* ctx, hipError_t e, const char* what) {
   return fail(ctx, CC_E_HIP, std::string(what) + ": " + hipGetErrorString(e));
 }
 
+hipError_t kernel(cc_ctx* ctx, const char* name, hipFunction_t* f) {
+  auto it = ctx->fns.find(name);
+  if (it != ctx->fns.end()) {
+    *f = it->second;
+    return hipSuccess;
+  }
+  hipError_t e = hipModuleGetFunction(f, ctx->mod, name);
+  if (e == hipSuccess) ctx->fns.emplace(name, *f);
+  return e;
+}
+
+Scratch& scratch(cc_ctx* ctx) {
+  if (!ctx->scratch) ctx->scratch = new Scratch();
+  return *ctx->scratch;
+}
+void release_scratch(cc_ctx* ctx) {
+  delete ctx->scratch;
+  ctx->scratch = nullptr;
+}
+
 }  // namespace cchost
 
 using namespace cchost;
@@ -61,6 +81,10 @@ int cc_init(int device, cc_ctx** out) {
   if ((e = hipEventCreate(&ctx->ev0)) != hipSuccess) return bail(e, "hipEventCreate");
   if ((e = hipEventCreate(&ctx->ev1)) != hipSuccess) return bail(e, "hipEventCreate");
   if ((e = hipMalloc(&ctx->d_konst, kKonstBytes)) != hipSuccess) return bail(e, "hipMalloc(konst)");
+  if ((e = hipMalloc(&ctx->d_counts, 64 * sizeof(unsigned long long))) != hipSuccess) return bail(e, "hipMalloc(counts)");
+  if ((e = hipHostMalloc((void**)&ctx->h_counts, 64 * sizeof(unsigned long long), hipHostMallocDefault)) !=
+      hipSuccess)
+    return bail(e, "hipHostMalloc(counts)");
   if ((e = launch(ctx->f_init_konst, dim3(1), dim3(64), 0, ctx->stream, ctx->d_konst)) != hipSuccess)
     return bail(e, "ccrd_init_konst");
   if ((e = hipStreamSynchronize(ctx->stream)) != hipSuccess) return bail(e, "hipStreamSynchronize");
@@ -73,6 +97,9 @@ void cc_destroy(cc_ctx* ctx) {
   if (ctx->device >= 0) (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   if (ctx->d_konst) (void)hipFree(ctx->d_konst);
+  if (ctx->d_counts) (void)hipFree(ctx->d_counts);
+  if (ctx->h_counts) (void)hipHostFree(ctx->h_counts);
+  release_scratch(ctx);
   if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
   if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
   if (ctx->mod) (void)hipModuleUnload(ctx->mod);

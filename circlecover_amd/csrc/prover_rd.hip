@@ -1,7 +1,7 @@
 // Round-down-mode prover kernels (gfx950).  Every kernel in this translation
 // unit is named ccrd_* and is built through the patched-descriptor path of
 // build.py, so its waves start with MODE.FP_ROUND = toward -infinity.
-#include "dev/rsb_chain.hpp"
+#include "dev/rsb_verdict.hpp"
 
 using namespace ccd;
 
@@ -112,41 +112,6 @@ extern "C" __global__ void ccrd_c2_trace_strip(const double* __restrict__ box, c
   const double ub_r4 = r3.hi < rem4 ? r3.hi : rem4;
   for (int k = 0; k < 3 * cap; ++k) trace[k] = 0.0;
   trace[3 * cap] = trc::r123_cover_strip(K, la, W, r1, r2, r3, ub_r4, trace, cap) ? 1.0 : 0.0;
-}
-
-// Full rectangle_size_bound verdict of 7-D cells (14 doubles each: la, r1..r6
-// as lb,ub pairs), one lane per cell, in the reference's order
-// (rectangle_size_bound.cu:162-229).  out[i] = index of the first succeeding
-// step (S1..S5 = 0..4, empty/vacuous = 5, C1..C14 = 6..19) or -1 (unproven);
-// trap[i] = 1 when a reference trap site was reached.
-CCD int rsb_cell_verdict(const Konst& K, const cell7& in, Trap& tp) {
-  if (rsbc::s1_even_split(in.la, in.r[0], in.r[1])) return 0;
-  if (rsbc::s23_uneven_split(in.la, in.r[0], in.r[1])) return 1;
-  if (rsbc::s23_uneven_split(in.la, in.r[0], in.r[2])) return 2;
-  if (rsbc::s4_opposite_corners(in.la, in.r[0], in.r[1], in.r[2])) return 3;
-  if (rsbc::s5_vertical_strip(K, in.la, in.r[0], in.r[1], in.r[2])) return 4;
-  const cell7 c = rsbc::tighten(in);
-  vals7 v;
-  v.R = rsbc::remaining_weight(c);
-  if (rsbc::is_empty(c) || empty(v.R)) return 5;
-  if (rsbc::c1_even_split(c, v)) return 6;
-  if (rsbc::c2_uneven_split(c)) return 7;
-  if (rsbc::c3_multi_disk_strip(c, v, tp)) return 8;
-  if (rsbc::c4_wall_building(K, c, v)) return 9;
-  if (rsbc::c5_corner_explicit(c, v, tp)) return 10;
-  if (rsbc::c6_opposite_corners(c, v, tp)) return 11;
-  if (rsbc::c7_vertical_strip(K, c, v)) return 12;
-  const rsbc::cover22 cov = rsbc::two_by_two(c, tp);
-  v.cover_width = cov.width;
-  v.cover_c4 = cov.c4;
-  if (cov.width > 0 && (cov.width > c.la.hi || rsbc::c8_two_by_two_strip(K, c, v, tp))) return 13;
-  if (rsbc::c9_l_shaped(K, c)) return 14;
-  if (rsbc::c10_corner_wall(K, c, tp)) return 15;
-  if (rsbc::c11_advanced_strip(c)) return 16;
-  if (rsbc::c12_five_disk(K, c, v, tp)) return 17;
-  if (rsbc::c13_six_disk(K, c, v, tp)) return 18;
-  if (rsbc::c14_seven_disk(K, c, v, tp)) return 19;
-  return -1;
 }
 
 extern "C" __global__ void __launch_bounds__(128) ccrd_rsb_verdict(const double* __restrict__ cells, uint64_t n,

@@ -1,0 +1,211 @@
+// tight_rectangle enumeration kernels (gfx950, round-down mode).
+//
+// The reference evaluates one lambda cell per launch as a four-deep loop
+// nest over r1..r4 pieces with early exits (kernel_find_critical_intervals_in,
+// Prover/tight_rectangle/tight_rectangle.cu:169-262).  Here the loop nest is
+// flattened into four level kernels over a batch of cells; each level keeps
+// the (cell, piece) pairs that no criterion of that level proves and compacts
+// them with a wave-aggregated append into the next level's input.  Every
+// pair is visited once, as in the reference, so the multiset of unproven
+// leaves is the same; only their order differs (the reference's is that of
+// its atomicAdd too).
+#include "dev/queue.hpp"
+#include "dev/tr.hpp"
+
+using namespace ccd;
+using namespace ccd::trc;
+
+struct tr_aux {  // per input cell
+  iv W;          // required_weight_for(la)
+  iv r1r;        // r1_range(la) intersected with the cell's r1
+};
+struct tr_s1 {
+  uint32_t cell, pad;
+  iv r1, r2r;
+};
+struct tr_s2 {
+  uint32_t cell, pad;
+  iv r1, r2, r3r;
+};
+struct tr_s3 {
+  uint32_t cell, pad;
+  iv r1, r2, r3, r4r, R4;
+};
+
+CCD cell5 load_cell(const double* __restrict__ p) {
+  cell5 c;
+  c.la = iv{p[0], p[1]};
+#pragma unroll
+  for (int k = 0; k < 4; ++k) c.r[k] = iv{p[2 + 2 * k], p[3 + 2 * k]};
+  return c;
+}
+
+// Level 0, one lane per cell: weight and r1 range (tight_rectangle.cu:175-180).
+extern "C" __global__ void __launch_bounds__(256) ccrd_tr_level0(const double* __restrict__ cells, uint64_t ncells,
+                                                                  const Konst* __restrict__ kp,
+                                                                  tr_aux* __restrict__ aux) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ncells) return;
+  const Konst K = *kp;
+  const cell5 c = load_cell(cells + 10 * i);
+  tr_aux a;
+  a.W = weight_for(K, c.la);
+  a.r1r = intersect(r1_range(K, c.la), c.r[0]);
+  aux[i] = a;
+}
+
+// Level 1, one lane per (cell, r1 piece): tight_rectangle.cu:182-191.
+extern "C" __global__ void __launch_bounds__(256) ccrd_tr_level1(const double* __restrict__ cells, uint64_t ncells,
+                                                                  int n1, const Konst* __restrict__ kp,
+                                                                  const tr_aux* __restrict__ aux,
+                                                                  tr_s1* __restrict__ out,
+                                                                  unsigned long long* __restrict__ count) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ncells * (uint64_t)n1) return;
+  const Konst K = *kp;
+  const uint32_t ci = (uint32_t)(i / (uint64_t)n1);
+  const int i1 = (int)(i % (uint64_t)n1);
+  const tr_aux a = aux[ci];
+  bool keep = false;
+  tr_s1 rec;
+  if (!empty(a.r1r)) {
+    const cell5 c = load_cell(cells + 10 * (uint64_t)ci);
+    rec.cell = ci;
+    rec.pad = 0;
+    rec.r1 = subinterval(a.r1r, i1, n1);
+    rec.r2r = intersect(r2_range(a.W, rec.r1), c.r[1]);
+    keep = !empty(rec.r2r) && !r1_strategies(K, c.la, a.W, rec.r1, rec.r2r.hi);
+  }
+  const uint64_t slot = wave_append(keep, count);
+  if (keep) out[slot] = rec;
+}
+
+// Level 2, one lane per (level-1 record, r2 piece): tight_rectangle.cu:193-206.
+extern "C" __global__ void __launch_bounds__(256) ccrd_tr_level2(const double* __restrict__ cells,
+                                                                  const tr_s1* __restrict__ in, uint64_t nin, int n2,
+                                                                  const Konst* __restrict__ kp,
+                                                                  const tr_aux* __restrict__ aux,
+                                                                  tr_s2* __restrict__ out,
+                                                                  unsigned long long* __restrict__ count,
+                                                                  int* __restrict__ trap) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nin * (uint64_t)n2) return;
+  const Konst K = *kp;
+  const tr_s1 s = in[i / (uint64_t)n2];
+  const int i2 = (int)(i % (uint64_t)n2);
+  const cell5 c = load_cell(cells + 10 * (uint64_t)s.cell);
+  const iv W = aux[s.cell].W;
+  Trap tp{0};
+  tr_s2 rec;
+  rec.cell = s.cell;
+  rec.pad = 0;
+  rec.r1 = s.r1;
+  rec.r2 = subinterval(s.r2r, i2, n2);
+  bool keep = !r1_strategies(K, c.la, W, s.r1, rec.r2.hi);
+  if (keep) {
+    rec.r3r = intersect(r3_range(W, s.r1, rec.r2), c.r[2]);
+    keep = !empty(rec.r3r) && !r1_r2_strategies(K, c.la, W, s.r1, rec.r2, rec.r3r.hi, tp);
+  }
+  if (tp.flag) atomicOr(trap, 1);
+  const uint64_t slot = wave_append(keep, count);
+  if (keep) out[slot] = rec;
+}
+
+// Level 3, one lane per (level-2 record, r3 piece): tight_rectangle.cu:208-227.
+extern "C" __global__ void __launch_bounds__(256) ccrd_tr_level3(const double* __restrict__ cells,
+                                                                  const tr_s2* __restrict__ in, uint64_t nin, int n3,
+                                                                  const Konst* __restrict__ kp,
+                                                                  const tr_aux* __restrict__ aux,
+                                                                  tr_s3* __restrict__ out,
+                                                                  unsigned long long* __restrict__ count,
+                                                                  int* __restrict__ trap) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nin * (uint64_t)n3) return;
+  const Konst K = *kp;
+  const tr_s2 s = in[i / (uint64_t)n3];
+  const int i3 = (int)(i % (uint64_t)n3);
+  const cell5 c = load_cell(cells + 10 * (uint64_t)s.cell);
+  const iv W = aux[s.cell].W;
+  Trap tp{0};
+  tr_s3 rec;
+  rec.cell = s.cell;
+  rec.pad = 0;
+  rec.r1 = s.r1;
+  rec.r2 = s.r2;
+  rec.r3 = subinterval(s.r3r, i3, n3);
+  bool keep = !r1_r2_strategies(K, c.la, W, s.r1, s.r2, rec.r3.hi, tp);
+  if (keep) {
+    rec.r4r = intersect(r4_range(W, s.r1, s.r2, rec.r3), c.r[3]);
+    keep = !empty(rec.r4r) && !r123_strategies(K, c.la, W, s.r1, s.r2, rec.r3, rec.r4r.hi);
+    if (keep) {
+      rec.R4 = W - s.r1 - s.r2 - rec.r3;
+      tighten_lo(rec.R4, 0.0);
+      keep = !empty(rec.R4);
+    }
+  }
+  if (tp.flag) atomicOr(trap, 1);
+  const uint64_t slot = wave_append(keep, count);
+  if (keep) out[slot] = rec;
+}
+
+// Level 4, one lane per (level-3 record, r4 piece): tight_rectangle.cu:229-258.
+// Unproven leaves are appended to `leaves`; the counter counts all of them,
+// records past `cap` are dropped (the caller re-runs with a larger buffer).
+extern "C" __global__ void __launch_bounds__(128) ccrd_tr_level4(const double* __restrict__ cells,
+                                                                  const tr_s3* __restrict__ in, uint64_t nin, int n4,
+                                                                  const Konst* __restrict__ kp,
+                                                                  const tr_aux* __restrict__ aux,
+                                                                  cell5* __restrict__ leaves, uint64_t cap,
+                                                                  unsigned long long* __restrict__ count,
+                                                                  int* __restrict__ trap) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nin * (uint64_t)n4) return;
+  const Konst K = *kp;
+  const tr_s3 s = in[i / (uint64_t)n4];
+  const int i4 = (int)(i % (uint64_t)n4);
+  const iv la = iv{cells[10 * (uint64_t)s.cell], cells[10 * (uint64_t)s.cell + 1]};
+  const iv W = aux[s.cell].W;
+  Trap tp{0};
+  const iv r4 = subinterval(s.r4r, i4, n4);
+  bool keep = !r123_strategies(K, la, W, s.r1, s.r2, s.r3, r4.hi);
+  cell5 v;
+  if (keep) {
+    v.la = la;
+    v.r[0] = s.r1;
+    v.r[1] = s.r2;
+    v.r[2] = s.r3;
+    v.r[3] = r4;
+    tighten_lo(v.r[2], v.r[3].lo);
+    tighten_lo(v.r[1], v.r[2].lo);
+    tighten_lo(v.r[0], v.r[1].lo);
+    iv R = W - v.r[0] - v.r[1] - v.r[2] - v.r[3];
+    tighten_lo(R, 0.0);
+    keep = !empty(R) && !two_by_two_leaf(K, v, R) && !three_disk_leaf(K, v, s.R4, tp) && !gaps_leaf(K, v, R, tp);
+  }
+  if (tp.flag) atomicOr(trap, 1);
+  const uint64_t slot = wave_append(keep, count);
+  if (keep && slot < cap) leaves[slot] = v;
+}
+
+// Bisection of unproven leaves for the next refinement level: the widest of
+// (lambda, r1..r4) by absolute width (round-to-nearest ub - lb, ties to the
+// lowest index) is split into get_subinterval(x, 0, 2) and (x, 1, 2).
+extern "C" __global__ void __launch_bounds__(256) ccrd_tr_bisect(const cell5* __restrict__ in, uint64_t n,
+                                                                  cell5* __restrict__ out) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= 2 * n) return;
+  cell5 c = in[i >> 1];
+  iv* d[5] = {&c.la, &c.r[0], &c.r[1], &c.r[2], &c.r[3]};
+  int best = 0;
+  double bw = rn_span_div(c.la.hi, c.la.lo, 1.0);
+#pragma unroll
+  for (int k = 1; k < 5; ++k) {
+    const double w = rn_span_div(d[k]->hi, d[k]->lo, 1.0);
+    if (w > bw) {
+      bw = w;
+      best = k;
+    }
+  }
+  *d[best] = subinterval(*d[best], (int)(i & 1), 2);
+  out[i] = c;
+}

@@ -43,7 +43,7 @@ def _load() -> ctypes.CDLL:
 
 lib = _load()
 
-_vp, _sz, _i, _u64 = ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int, ctypes.c_uint64
+_vp, _sz, _i, _u64, _d = ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int, ctypes.c_uint64, ctypes.c_double
 _P = ctypes.POINTER
 lib.cc_init.argtypes = [_i, _P(_vp)]
 lib.cc_destroy.argtypes = [_vp]
@@ -66,10 +66,33 @@ lib.cc_debug_trace_strip.argtypes = [_vp, _vp, _vp, _i]
 lib.cc_rsb_eval_cells.argtypes = [_vp, _vp, _sz, _vp, _vp]
 lib.cc_rsb_eval_cells_async.argtypes = [_vp, _vp, _sz, _vp, _vp]
 lib.cc_rsb_eval_masks.argtypes = [_vp, _vp, _sz, _vp]
+lib.cc_subintervals.argtypes = [_d, _d, _i, _vp]
+lib.cc_subintervals.restype = None
+lib.cc_tr_find_critical_in.argtypes = [_vp, _vp, _i, _vp, _sz, _P(_sz)]
+lib.cc_tr_enumerate.argtypes = [_vp, _vp, _sz, _vp, _vp, _sz, _P(_sz)]
+lib.cc_tr_bnb.argtypes = [_vp, _vp, _sz, _vp, _i, _vp, _sz, _P(_sz), _vp]
 
 
 def _ptr(a: np.ndarray) -> int:
     return a.ctypes.data
+
+
+def subintervals(lb: float, ub: float, n: int) -> np.ndarray:
+    """(n, 2) pieces of [lb, ub] as get_subinterval makes them."""
+    out = np.empty((n, 2))
+    lib.cc_subintervals(lb, ub, n, _ptr(out))
+    return out
+
+
+DBL_MAX = float(np.finfo(np.float64).max)
+TR_LAMBDA_MAX = 2.0898841580413818342  # tight_rectangle.cu:462
+
+
+def tr_initial_cells(la_lo: float, la_hi: float, n: int) -> np.ndarray:
+    """(n, 10) tight_rectangle cells: lambda pieces of [la_lo, la_hi], every
+    squared radius in [0, DBL_MAX] (tight_rectangle.cu:460-465)."""
+    la = subintervals(la_lo, la_hi, n)
+    return np.concatenate([la, np.tile([0.0, DBL_MAX], (n, 4))], axis=1)
 
 
 def gen_c2_boxes(seed: int, n: int) -> np.ndarray:
@@ -170,6 +193,40 @@ class Prover:
         trap = np.empty(n, dtype=np.uint8)
         self._check(lib.cc_rsb_eval_cells(self.ctx, _ptr(cells), n, _ptr(verdict), _ptr(trap)))
         return verdict, trap
+
+    # ---- tight_rectangle enumeration
+    def _grow(self, call, cap: int, width: int):
+        """Runs call(out, cap, n_out) and re-runs it with the reported size on
+        CC_E_OVERFLOW (the reference's relaunch protocol)."""
+        while True:
+            out = np.empty((max(cap, 1), width))
+            n = ctypes.c_size_t()
+            rc = call(out, cap, n)
+            if rc == CC_E_OVERFLOW and n.value > cap:
+                cap = n.value
+                continue
+            self._check(rc)
+            return out[: n.value]
+
+    def tr_find_critical_in(self, cell, is_init: bool, cap: int = 1 << 16) -> np.ndarray:
+        cell = np.ascontiguousarray(cell, dtype=np.float64).reshape(10)
+        return self._grow(lambda o, c, n: lib.cc_tr_find_critical_in(self.ctx, _ptr(cell), int(is_init), _ptr(o), c,
+                                                                     ctypes.byref(n)), cap, 10)
+
+    def tr_enumerate(self, cells: np.ndarray, counts, cap: int = 1 << 16) -> np.ndarray:
+        cells = np.ascontiguousarray(cells, dtype=np.float64).reshape(-1, 10)
+        cnt = np.ascontiguousarray(counts, dtype=np.int32)
+        return self._grow(lambda o, c, n: lib.cc_tr_enumerate(self.ctx, _ptr(cells), cells.shape[0], _ptr(cnt),
+                                                              _ptr(o), c, ctypes.byref(n)), cap, 10)
+
+    def tr_bnb(self, cells: np.ndarray, counts, levels: int, cap: int = 1 << 16):
+        """Returns (unproven leaves, unproven count after each level)."""
+        cells = np.ascontiguousarray(cells, dtype=np.float64).reshape(-1, 10)
+        cnt = np.ascontiguousarray(counts, dtype=np.int32)
+        per = np.zeros(levels + 1, dtype=np.uint64)
+        leaves = self._grow(lambda o, c, n: lib.cc_tr_bnb(self.ctx, _ptr(cells), cells.shape[0], _ptr(cnt), levels,
+                                                          _ptr(o), c, ctypes.byref(n), _ptr(per)), cap, 10)
+        return leaves, per
 
     def rsb_eval_masks(self, cells: np.ndarray) -> np.ndarray:
         cells = np.ascontiguousarray(cells, dtype=np.float64).reshape(-1, 14)

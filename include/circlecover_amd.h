@@ -108,10 +108,41 @@ int cc_rsb_eval_cells(cc_ctx* ctx, const cc_rsb_cell* cells, size_t n, int32_t* 
 int cc_rsb_eval_masks(cc_ctx* ctx, const cc_rsb_cell* cells, size_t n, uint64_t* mask);
 int cc_rsb_eval_cells_async(cc_ctx* ctx, const cc_rsb_cell* d_cells, size_t n, int32_t* d_verdict, uint8_t* d_trap);
 
+/* ---- tight_rectangle enumeration (replaces kernel_find_critical_intervals_in
+ *      and its host wrapper, Prover/tight_rectangle/tight_rectangle.cu:169-298) ---- */
+
+/* One launch of the reference: the unproven leaves of `cell` with the
+ * reference's piece counts (is_init: 128 for r1..r4; otherwise 128,128,128,64;
+ * tight_rectangle.cu:45-62).  Leaves are written in unspecified order (the
+ * reference's order is that of its atomicAdd); *n_out = number of leaves.
+ * Returns CC_E_OVERFLOW with *n_out set when cap is too small (the caller
+ * enlarges its buffer and calls again, as tight_rectangle.cu:283-293 does),
+ * CC_E_TRAP if a cell reached a reference trap site. */
+int cc_tr_find_critical_in(cc_ctx* ctx, const cc_tr_cell* cell, int is_init, cc_tr_cell* out, size_t cap,
+                           size_t* n_out);
+/* The same enumeration over a batch of cells with counts[4] pieces for
+ * r1..r4 (each >= 1); the leaves of all cells are returned together. */
+int cc_tr_enumerate(cc_ctx* ctx, const cc_tr_cell* cells, size_t ncells, const int counts[4], cc_tr_cell* out,
+                    size_t cap, size_t* n_out);
+/* Level-synchronous branch-and-bound (no reference counterpart; the small
+ * tight_rectangle workload): enumerate `cells` with counts[4], then `levels`
+ * times split every unproven leaf on its widest dimension (absolute width,
+ * ties to the lowest of lambda, r1..r4; halves get_subinterval(x, k, 2)) and
+ * re-evaluate both halves with piece counts 1.  Returns the final unproven
+ * leaves; per_level (levels+1 entries, may be NULL) receives the number of
+ * unproven leaves after each level. */
+int cc_tr_bnb(cc_ctx* ctx, const cc_tr_cell* cells, size_t ncells, const int counts[4], int levels,
+              cc_tr_cell* out, size_t cap, size_t* n_out, uint64_t* per_level);
+
 /* Synthetic boxes (host): splitmix64(seed); la.lb ~ U[1,2.5]; three radius
  * lower bounds ~ U[0.02,0.7] sorted descending; each width 2^U[-20,-6];
  * ub = lb + width rounded upward.  Writes 8*n doubles in SoA order (ld = n). */
 void cc_gen_c2_boxes(uint64_t seed, size_t n, double* soa);
+
+/* get_subinterval (Prover/subintervals.hpp:32-43) on the host: the n pieces
+ * of [lb, ub] as 2n doubles (lb + diff*i with one rounding, the last piece
+ * ending at ub).  Used to split lambda ranges into initial cells. */
+void cc_subintervals(double lb, double ub, int n, double* out);
 
 /* Directed-rounding primitive self-test: out[8*i+k] for k = add_rd, add_ru,
  * mul_rd, mul_ru, div_rd, div_ru, sqrt_rd(a), sqrt_ru(a). */

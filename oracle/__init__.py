@@ -41,6 +41,19 @@ def _load(name: str) -> ctypes.CDLL:
     L.or_trace_strip.argtypes = [_vp, _vp, _i]
     L.or_rsb_verdicts.argtypes = [_vp, _sz, _vp, _vp, _i]
     L.or_rsb_masks.argtypes = [_vp, _sz, _vp, _i]
+    L.or_tr_find_critical.argtypes = [_vp, _sz, _vp, _vp, _sz, _vp, _vp, _i]
+    L.or_rsb_grid.argtypes = [_vp, _i, _i, _i, _vp]
+    L.or_rsb_grid.restype = _sz
+    L.or_rsb_bnb.argtypes = [_vp, _sz, _i, _vp, _sz, _vp, _vp, _vp, _vp, _i]
+    L.or_rsb_find_critical.argtypes = [_vp, _vp, _vp, _sz, _vp, _vp, _i]
+    L.or_rsb_bisect.argtypes = [_vp, _sz, _vp]
+    L.or_rsb_bisect.restype = None
+    L.or_subintervals.argtypes = [_d, _d, _i, _vp]
+    L.or_subintervals.restype = None
+    L.or_tr_bisect.argtypes = [_vp, _sz, _vp]
+    L.or_tr_bisect.restype = None
+    L.or_tr_merge_adjacent_r1.argtypes = [_vp, _sz, _vp]
+    L.or_tr_merge_adjacent_r1.restype = _sz
     return L
 
 
@@ -106,6 +119,102 @@ def rsb_masks(cells: np.ndarray, threads: int = 0) -> np.ndarray:
     out = np.empty(cells.shape[0], dtype=np.uint64)
     lib.or_rsb_masks(cells.ctypes.data, cells.shape[0], out.ctypes.data, threads)
     return out
+
+
+def tr_find_critical(cells: np.ndarray, counts, threads: int = 0, cap: int = 1 << 22):
+    """Unproven tight_rectangle leaves of a batch of (n, 10) cells, enumerated
+    with `counts` = (n1, n2, n3, n4) pieces.  Returns (leaves (m, 10), trap)."""
+    cells = np.ascontiguousarray(cells, dtype=np.float64).reshape(-1, 10)
+    cnt = np.ascontiguousarray(counts, dtype=np.int32)
+    while True:
+        out = np.empty((cap, 10))
+        n_out = ctypes.c_size_t()
+        trap = ctypes.c_int()
+        lib.or_tr_find_critical(cells.ctypes.data, cells.shape[0], cnt.ctypes.data, out.ctypes.data, cap,
+                                ctypes.addressof(n_out), ctypes.addressof(trap), threads)
+        if n_out.value <= cap:
+            return out[: n_out.value].copy(), bool(trap.value)
+        cap = n_out.value
+
+
+def subintervals(lb: float, ub: float, n: int) -> np.ndarray:
+    """(n, 2) pieces get_subinterval([lb, ub], i, n) (subintervals.hpp:32-43)."""
+    out = np.empty((n, 2))
+    lib.or_subintervals(lb, ub, n, out.ctypes.data)
+    return out
+
+
+def tr_bisect(cells: np.ndarray) -> np.ndarray:
+    cells = np.ascontiguousarray(cells, dtype=np.float64).reshape(-1, 10)
+    out = np.empty((2 * cells.shape[0], 10))
+    lib.or_tr_bisect(cells.ctypes.data, cells.shape[0], out.ctypes.data)
+    return out
+
+
+def tr_merge_adjacent_r1(cells: np.ndarray) -> np.ndarray:
+    cells = np.ascontiguousarray(cells, dtype=np.float64).reshape(-1, 10)
+    out = np.empty_like(cells)
+    m = lib.or_tr_merge_adjacent_r1(cells.ctypes.data, cells.shape[0], out.ctypes.data)
+    return out[:m].copy()
+
+
+def rsb_grid(outer, nla: int, nr1: int, nr2: int) -> np.ndarray:
+    """Nonempty initial cells of the reference's (lambda, r1, r2) host loop
+    over `outer` (14 doubles), in loop order."""
+    outer = np.ascontiguousarray(outer, dtype=np.float64).reshape(14)
+    out = np.empty((nla * nr1 * nr2, 14))
+    m = lib.or_rsb_grid(outer.ctypes.data, nla, nr1, nr2, out.ctypes.data)
+    return out[:m].copy()
+
+
+def rsb_bnb(cells: np.ndarray, max_depth: int, threads: int = 0, cap: int = 1 << 20):
+    """Widest-dimension branch-and-bound; returns (unproven cells at
+    max_depth, cells evaluated per depth, verdict histogram (depth, 21),
+    trap)."""
+    cells = np.ascontiguousarray(cells, dtype=np.float64).reshape(-1, 14)
+    while True:
+        out = np.empty((cap, 14))
+        n_out = ctypes.c_size_t()
+        trap = ctypes.c_int()
+        per = np.zeros(max_depth + 1, dtype=np.uint64)
+        hist = np.zeros((max_depth + 1, 21), dtype=np.uint64)
+        lib.or_rsb_bnb(cells.ctypes.data, cells.shape[0], max_depth, out.ctypes.data, cap, ctypes.addressof(n_out),
+                       per.ctypes.data, hist.ctypes.data, ctypes.addressof(trap), threads)
+        if n_out.value <= cap:
+            return out[: n_out.value].copy(), per, hist, bool(trap.value)
+        cap = n_out.value
+
+
+def rsb_find_critical(cell, counts, threads: int = 0, cap: int = 1 << 20):
+    """Unproven leaves of one rectangle_size_bound kernel launch with
+    `counts` pieces for r3..r6.  Returns (leaves (m, 14), trap)."""
+    cell = np.ascontiguousarray(cell, dtype=np.float64).reshape(14)
+    cnt = np.ascontiguousarray(counts, dtype=np.int32)
+    while True:
+        out = np.empty((cap, 14))
+        n_out = ctypes.c_size_t()
+        trap = ctypes.c_int()
+        lib.or_rsb_find_critical(cell.ctypes.data, cnt.ctypes.data, out.ctypes.data, cap, ctypes.addressof(n_out),
+                                 ctypes.addressof(trap), threads)
+        if n_out.value <= cap:
+            return out[: n_out.value].copy(), bool(trap.value)
+        cap = n_out.value
+
+
+def rsb_bisect(cells: np.ndarray) -> np.ndarray:
+    cells = np.ascontiguousarray(cells, dtype=np.float64).reshape(-1, 14)
+    out = np.empty((2 * cells.shape[0], 14))
+    lib.or_rsb_bisect(cells.ctypes.data, cells.shape[0], out.ctypes.data)
+    return out
+
+
+def canonical(cells: np.ndarray) -> np.ndarray:
+    """Records sorted lexicographically by their doubles (atomic append order
+    on the GPU is not deterministic; the set is)."""
+    cells = np.asarray(cells)
+    if cells.shape[0] == 0:
+        return cells
+    return cells[np.lexsort(cells.T[::-1])]
 
 
 def rn_helpers(a: float, b: float, c: float):

@@ -56,6 +56,8 @@ bool seven_disk_strategies(const Cell& c, const Vals& v);                       
 // Returns the index of the first succeeding step (0..19: S1..S5 = 0..4,
 // empty/vacuous = 5, C1..C14 = 6..19) or -1 when the cell stays unproven.
 int cell_verdict(const Cell& c);
+// cell_verdict without the shortcut steps S1..S5 (same return codes).
+int chain_verdict(const Cell& c);
 // All steps evaluated independently: bit k = step k succeeds, bit 20+k = trap.
 uint64_t cell_mask(const Cell& c);
 

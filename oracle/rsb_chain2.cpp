@@ -942,6 +942,12 @@ int cell_verdict(const Cell& in) {
   if (shortcut_uneven_split_recursion(la, in.r[0], in.r[1], in.r[2])) return 2;
   if (shortcut_r1_r2_opposite_corners_recursion(la, in.r[0], in.r[1], in.r[2])) return 3;
   if (shortcut_r1_r2_vertical_strip(la, in.r[0], in.r[1], in.r[2])) return 4;
+  return chain_verdict(in);
+}
+
+// The leaf part of cell_verdict (rectangle_size_bound.cu:186-229): tighten,
+// Intermediate_values, emptiness, C1..C14.
+int chain_verdict(const Cell& in) {
   Cell c = tighten(in);
   Vals v = make_vals(c);
   if (is_empty(c) || v.R.empty()) return 5;
