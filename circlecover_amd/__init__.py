@@ -14,9 +14,10 @@ from .native import (  # noqa: F401
     gen_c2_boxes,
     lib,
     library_path,
+    rsb_grid,
     subintervals,
     tr_initial_cells,
 )
 
 __all__ = ["CCError", "Prover", "gen_c2_boxes", "lib", "library_path", "C2_CRITERIA", "DBL_MAX", "TR_LAMBDA_MAX",
-           "subintervals", "tr_initial_cells"]
+           "subintervals", "tr_initial_cells", "rsb_grid"]

@@ -68,6 +68,11 @@ lib.cc_rsb_eval_cells_async.argtypes = [_vp, _vp, _sz, _vp, _vp]
 lib.cc_rsb_eval_masks.argtypes = [_vp, _vp, _sz, _vp]
 lib.cc_subintervals.argtypes = [_d, _d, _i, _vp]
 lib.cc_subintervals.restype = None
+lib.cc_rsb_grid.argtypes = [_vp, _i, _i, _i, _vp, _sz]
+lib.cc_rsb_grid.restype = _sz
+lib.cc_rsb_find_critical_in.argtypes = [_vp, _vp, _i, _vp, _sz, _P(_sz)]
+lib.cc_rsb_enumerate.argtypes = [_vp, _vp, _sz, _vp, _vp, _sz, _P(_sz)]
+lib.cc_rsb_bnb.argtypes = [_vp, _vp, _sz, _i, _vp, _sz, _P(_sz), _vp]
 lib.cc_tr_find_critical_in.argtypes = [_vp, _vp, _i, _vp, _sz, _P(_sz)]
 lib.cc_tr_enumerate.argtypes = [_vp, _vp, _sz, _vp, _vp, _sz, _P(_sz)]
 lib.cc_tr_bnb.argtypes = [_vp, _vp, _sz, _vp, _i, _vp, _sz, _P(_sz), _vp]
@@ -93,6 +98,30 @@ def tr_initial_cells(la_lo: float, la_hi: float, n: int) -> np.ndarray:
     squared radius in [0, DBL_MAX] (tight_rectangle.cu:460-465)."""
     la = subintervals(la_lo, la_hi, n)
     return np.concatenate([la, np.tile([0.0, DBL_MAX], (n, 4))], axis=1)
+
+
+RSB_DISK_WEIGHT_MAX = 9.0 / 64.0  # rectangle_size_bound/values.hpp:33
+
+
+def rsb_grid(la_lo: float, la_hi: float, nla: int, nr1: int, nr2: int) -> np.ndarray:
+    """(m, 14) initial rectangle_size_bound cells over lambda in [la_lo, la_hi]
+    and squared radii in [0, 9/64], as the reference's host loop builds them."""
+    outer = np.array([la_lo, la_hi] + [0.0, RSB_DISK_WEIGHT_MAX] * 6)
+    out = np.empty((nla * nr1 * nr2, 14))
+    m = lib.cc_rsb_grid(_ptr(outer), nla, nr1, nr2, _ptr(out), out.shape[0])
+    return out[:m]
+
+
+class BnbStats(ctypes.Structure):
+    _fields_ = [("cells_per_depth", ctypes.c_uint64 * 64), ("verdict_hist", ctypes.c_uint64 * 21),
+                ("trap_cells", ctypes.c_uint64), ("cell_evals", ctypes.c_uint64), ("depth_reached", ctypes.c_int),
+                ("device_ms", ctypes.c_double), ("wall_ms", ctypes.c_double)]
+
+    def as_dict(self) -> dict:
+        d = self.depth_reached
+        return {"cells_per_depth": list(self.cells_per_depth)[: d + 1], "verdict_hist": list(self.verdict_hist),
+                "trap_cells": self.trap_cells, "cell_evals": self.cell_evals, "depth_reached": d,
+                "device_ms": self.device_ms, "wall_ms": self.wall_ms}
 
 
 def gen_c2_boxes(seed: int, n: int) -> np.ndarray:
@@ -195,9 +224,11 @@ class Prover:
         return verdict, trap
 
     # ---- tight_rectangle enumeration
-    def _grow(self, call, cap: int, width: int):
+    def _grow(self, call, cap: int, width: int, with_trap: bool = False):
         """Runs call(out, cap, n_out) and re-runs it with the reported size on
-        CC_E_OVERFLOW (the reference's relaunch protocol)."""
+        CC_E_OVERFLOW (the reference's relaunch protocol).  With with_trap,
+        CC_E_TRAP (output complete, a cell reached a trap site) is returned as
+        a flag instead of raised."""
         while True:
             out = np.empty((max(cap, 1), width))
             n = ctypes.c_size_t()
@@ -205,8 +236,10 @@ class Prover:
             if rc == CC_E_OVERFLOW and n.value > cap:
                 cap = n.value
                 continue
-            self._check(rc)
-            return out[: n.value]
+            trap = with_trap and rc == CC_E_TRAP
+            if not trap:
+                self._check(rc)
+            return (out[: n.value], trap) if with_trap else out[: n.value]
 
     def tr_find_critical_in(self, cell, is_init: bool, cap: int = 1 << 16) -> np.ndarray:
         cell = np.ascontiguousarray(cell, dtype=np.float64).reshape(10)
@@ -227,6 +260,28 @@ class Prover:
         leaves = self._grow(lambda o, c, n: lib.cc_tr_bnb(self.ctx, _ptr(cells), cells.shape[0], _ptr(cnt), levels,
                                                           _ptr(o), c, ctypes.byref(n), _ptr(per)), cap, 10)
         return leaves, per
+
+    # ---- rectangle_size_bound enumeration / branch-and-bound
+    def rsb_find_critical_in(self, cell, is_init: bool, cap: int = 1 << 16) -> np.ndarray:
+        cell = np.ascontiguousarray(cell, dtype=np.float64).reshape(14)
+        return self._grow(lambda o, c, n: lib.cc_rsb_find_critical_in(self.ctx, _ptr(cell), int(is_init), _ptr(o), c,
+                                                                      ctypes.byref(n)), cap, 14)
+
+    def rsb_enumerate(self, cells: np.ndarray, counts, cap: int = 1 << 16, with_trap: bool = False):
+        """Unproven leaves; with with_trap, (leaves, trap flag) instead of
+        raising CC_E_TRAP."""
+        cells = np.ascontiguousarray(cells, dtype=np.float64).reshape(-1, 14)
+        cnt = np.ascontiguousarray(counts, dtype=np.int32)
+        return self._grow(lambda o, c, n: lib.cc_rsb_enumerate(self.ctx, _ptr(cells), cells.shape[0], _ptr(cnt),
+                                                               _ptr(o), c, ctypes.byref(n)), cap, 14, with_trap)
+
+    def rsb_bnb(self, cells: np.ndarray, max_depth: int, cap: int = 1 << 16):
+        """Returns (unproven cells at max_depth, BnbStats)."""
+        cells = np.ascontiguousarray(cells, dtype=np.float64).reshape(-1, 14)
+        st = BnbStats()
+        out = self._grow(lambda o, c, n: lib.cc_rsb_bnb(self.ctx, _ptr(cells), cells.shape[0], max_depth, _ptr(o), c,
+                                                        ctypes.byref(n), ctypes.byref(st)), cap, 14)
+        return out, st
 
     def rsb_eval_masks(self, cells: np.ndarray) -> np.ndarray:
         cells = np.ascontiguousarray(cells, dtype=np.float64).reshape(-1, 14)

@@ -139,6 +139,48 @@ int cc_tr_bnb(cc_ctx* ctx, const cc_tr_cell* cells, size_t ncells, const int cou
  * ub = lb + width rounded upward.  Writes 8*n doubles in SoA order (ld = n). */
 void cc_gen_c2_boxes(uint64_t seed, size_t n, double* soa);
 
+/* ---- rectangle_size_bound enumeration and branch-and-bound ---- */
+
+/* The reference's initial cells (host loop of find_critical_intervals_in,
+ * Prover/rectangle_size_bound/rectangle_size_bound.cu:276-310, with
+ * Variables::from_la_r1_r2, variables.hpp:37-51): lambda, r1 and r2 of
+ * `outer` split into nla, nr1 and nr2 pieces, r2 limited by r1.ub, r3..r6
+ * limited by r2.ub, tightened, empty cells skipped.  Writes up to cap cells in
+ * loop order and returns the number of nonempty cells. */
+size_t cc_rsb_grid(const cc_rsb_cell* outer, int nla, int nr1, int nr2, cc_rsb_cell* out, size_t cap);
+/* One launch of the reference's kernel (rectangle_size_bound.cu:161-236):
+ * shortcuts S1/S2 on the cell, S3..S5 per r3 piece, the leaf chain on every
+ * (r3, r4, r5, r6) piece; is_init selects the piece counts (64,64,64,64 or
+ * 16,16,16,32, :39-57).  Unproven leaves (tightened) in unspecified order;
+ * CC_E_OVERFLOW with *n_out set when cap is too small, as the reference's
+ * relaunch protocol (:256-269) expects. */
+int cc_rsb_find_critical_in(cc_ctx* ctx, const cc_rsb_cell* cell, int is_init, cc_rsb_cell* out, size_t cap,
+                            size_t* n_out);
+/* The same enumeration over a batch of cells with counts[4] pieces for r3..r6. */
+int cc_rsb_enumerate(cc_ctx* ctx, const cc_rsb_cell* cells, size_t ncells, const int counts[4], cc_rsb_cell* out,
+                     size_t cap, size_t* n_out);
+
+#define CC_BNB_MAX_DEPTH 64
+typedef struct cc_bnb_stats {
+  uint64_t cells_per_depth[CC_BNB_MAX_DEPTH]; /* cells evaluated at each depth */
+  uint64_t verdict_hist[21];                  /* [v+1] = cells with verdict v (cc_rsb_eval_cells codes) */
+  uint64_t trap_cells;                        /* cells that reached a reference trap site */
+  uint64_t cell_evals;                        /* criterion evaluations: v+1 per cell proven by step v, 20 per unproven */
+  int depth_reached;
+  double device_ms;                           /* first launch to last kernel end (HIP events) */
+  double wall_ms;                             /* entry to return, host clock */
+} cc_bnb_stats;
+
+/* Widest-dimension branch-and-bound over 7-D cells (no reference
+ * counterpart; the reference subdivides with fixed piece counts instead).
+ * Every cell gets the cc_rsb_eval_cells verdict; an unproven cell below
+ * max_depth is split on its widest dimension (absolute width, ties to the
+ * lowest of lambda, r1..r6; halves get_subinterval(x, k, 2)), an unproven
+ * cell at max_depth is returned.  Level-synchronous: one launch per depth
+ * with wave-aggregated compaction of the children. */
+int cc_rsb_bnb(cc_ctx* ctx, const cc_rsb_cell* cells, size_t ncells, int max_depth, cc_rsb_cell* out, size_t cap,
+               size_t* n_out, cc_bnb_stats* stats);
+
 /* get_subinterval (Prover/subintervals.hpp:32-43) on the host: the n pieces
  * of [lb, ub] as 2n doubles (lb + diff*i with one rounding, the last piece
  * ending at ub).  Used to split lambda ranges into initial cells. */

@@ -170,7 +170,7 @@ def rsb_grid(outer, nla: int, nr1: int, nr2: int) -> np.ndarray:
 def rsb_bnb(cells: np.ndarray, max_depth: int, threads: int = 0, cap: int = 1 << 20):
     """Widest-dimension branch-and-bound; returns (unproven cells at
     max_depth, cells evaluated per depth, verdict histogram (depth, 21),
-    trap)."""
+    number of cells that reached a trap site)."""
     cells = np.ascontiguousarray(cells, dtype=np.float64).reshape(-1, 14)
     while True:
         out = np.empty((cap, 14))
@@ -181,7 +181,7 @@ def rsb_bnb(cells: np.ndarray, max_depth: int, threads: int = 0, cap: int = 1 <<
         lib.or_rsb_bnb(cells.ctypes.data, cells.shape[0], max_depth, out.ctypes.data, cap, ctypes.addressof(n_out),
                        per.ctypes.data, hist.ctypes.data, ctypes.addressof(trap), threads)
         if n_out.value <= cap:
-            return out[: n_out.value].copy(), per, hist, bool(trap.value)
+            return out[: n_out.value].copy(), per, hist, int(trap.value)
         cap = n_out.value
 
 
@@ -206,6 +206,18 @@ def rsb_bisect(cells: np.ndarray) -> np.ndarray:
     out = np.empty((2 * cells.shape[0], 14))
     lib.or_rsb_bisect(cells.ctypes.data, cells.shape[0], out.ctypes.data)
     return out
+
+
+def rsb_count_ops(cells: np.ndarray):
+    """(directed ops per cell, verdict per cell) from the counting build."""
+    cells = np.ascontiguousarray(cells, dtype=np.float64).reshape(-1, 14)
+    ops = np.zeros(cells.shape[0], dtype=np.uint64)
+    v = np.zeros(cells.shape[0], dtype=np.int32)
+    L = count_lib()
+    L.or_rsb_count_ops.argtypes = [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p, ctypes.c_void_p]
+    if L.or_rsb_count_ops(cells.ctypes.data, cells.shape[0], ops.ctypes.data, v.ctypes.data) != 0:
+        raise RuntimeError("oracle built without op counting")
+    return ops, v
 
 
 def canonical(cells: np.ndarray) -> np.ndarray:

@@ -121,6 +121,51 @@ int or_c2_count_ops(const double* soa, size_t n, size_t ld, uint64_t* ops) {
 #endif
 }
 
+// Directed-op count of the rectangle_size_bound verdict of each cell
+// (counting build only; -1 otherwise).  ops[i] = operations of cell i,
+// verdict[i] = its verdict.
+int or_rsb_count_ops(const double* cells, size_t n, uint64_t* ops, int32_t* verdict) {
+#ifdef CC_ORACLE_COUNT
+  for (size_t i = 0; i < n; ++i) {
+    const double* p = cells + 14 * i;
+    rsb::Cell c;
+    c.la = IV(p[0], p[1]);
+    for (int k = 0; k < 6; ++k) c.r[k] = IV(p[2 + 2 * k], p[3 + 2 * k]);
+    uint64_t before = g_op_count;
+    verdict[i] = rsb::cell_verdict(c);
+    ops[i] = g_op_count - before;
+  }
+  return 0;
+#else
+  (void)cells;
+  (void)n;
+  (void)ops;
+  (void)verdict;
+  return -1;
+#endif
+}
+
+// Per-step directed-op counts of every step evaluated on its own (counting
+// build only): ops[20*i + k] for cell i, step k; mask[i] as or_rsb_masks.
+int or_rsb_count_steps(const double* cells, size_t n, uint64_t* ops, uint64_t* mask) {
+#ifdef CC_ORACLE_COUNT
+  for (size_t i = 0; i < n; ++i) {
+    const double* p = cells + 14 * i;
+    rsb::Cell c;
+    c.la = IV(p[0], p[1]);
+    for (int k = 0; k < 6; ++k) c.r[k] = IV(p[2 + 2 * k], p[3 + 2 * k]);
+    mask[i] = rsb::cell_mask(c, ops + 20 * i);
+  }
+  return 0;
+#else
+  (void)cells;
+  (void)n;
+  (void)ops;
+  (void)mask;
+  return -1;
+#endif
+}
+
 // Primitive access for the golden-vector tests: op 0..5 = add, mul, div
 // (binary) and sqrt (unary, b ignored), in round-down (dir=0) or round-up
 // (dir=1).

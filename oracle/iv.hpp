@@ -29,8 +29,10 @@ namespace ccor {
 #ifdef CC_ORACLE_COUNT
 extern thread_local uint64_t g_op_count;
 #define CCOR_COUNT() (++::ccor::g_op_count)
+#define CCOR_OPS() (::ccor::g_op_count)
 #else
 #define CCOR_COUNT() ((void)0)
+#define CCOR_OPS() (uint64_t{0})
 #endif
 
 // MXCSR images: all exceptions masked, FTZ/DAZ off; bits 13-14 select rounding.

@@ -59,7 +59,9 @@ int cell_verdict(const Cell& c);
 // cell_verdict without the shortcut steps S1..S5 (same return codes).
 int chain_verdict(const Cell& c);
 // All steps evaluated independently: bit k = step k succeeds, bit 20+k = trap.
-uint64_t cell_mask(const Cell& c);
+// step_ops (20 entries, may be null) receives the directed-op count of each
+// step in the counting build (the tighten/vals set-up is charged to step 5).
+uint64_t cell_mask(const Cell& c, uint64_t* step_ops = nullptr);
 
 }  // namespace rsb
 }  // namespace ccor

@@ -974,12 +974,15 @@ int chain_verdict(const Cell& in) {
 // Every step of cell_verdict evaluated independently (no short-circuit):
 // bit k set when step k succeeds (steps numbered as in cell_verdict); bit
 // 20+k when step k reached a trap site.
-uint64_t cell_mask(const Cell& in) {
+uint64_t cell_mask(const Cell& in, uint64_t* step_ops) {
   uint64_t m = 0;
+  uint64_t last = CCOR_OPS();
   auto put = [&](int k, bool ok) {
     if (ok) m |= 1ull << k;
     if (g_trap) m |= 1ull << (20 + k);
     g_trap = 0;
+    if (step_ops) step_ops[k] = CCOR_OPS() - last;
+    last = CCOR_OPS();
   };
   g_trap = 0;
   const IV& la = in.la;

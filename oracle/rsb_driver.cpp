@@ -108,7 +108,8 @@ size_t or_rsb_grid(const double* outer, int nla, int nr1, int nr2, double* out) 
 // verdict; unproven cells below max_depth are split on their widest
 // dimension, unproven cells at max_depth are reported in `out` (up to cap;
 // *n_out counts all).  per_level[d] = number of cells evaluated at depth d
-// (max_depth+1 entries); verdict_hist[d*21 + v+1] = histogram of verdicts.
+// (max_depth+1 entries); verdict_hist[d*21 + v+1] = histogram of verdicts;
+// *trap = number of cells that reached a trap site.
 int or_rsb_bnb(const double* cells, size_t n, int max_depth, double* out, size_t cap, size_t* n_out,
                uint64_t* per_level, uint64_t* verdict_hist, int* trap, int threads) {
   if (threads <= 0) threads = (int)std::thread::hardware_concurrency();
@@ -139,7 +140,7 @@ int or_rsb_bnb(const double* cells, size_t n, int max_depth, double* out, size_t
     for (auto& t : pool) t.join();
     next.clear();
     for (size_t i = 0; i < cur.size(); ++i) {
-      any_trap |= tr[i];
+      any_trap += tr[i];
       verdict_hist[d * 21 + verdict[i] + 1]++;
       if (verdict[i] >= 0) continue;
       if (d == max_depth) {
