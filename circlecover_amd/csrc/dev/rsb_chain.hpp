@@ -11,6 +11,12 @@
 //   * the chain is split into separately launched stages (see
 //     prover_rd.hip) so each kernel's register budget covers only its stage.
 #pragma once
+// CC_EXACT_SHORTCUTS (0 by default) enables early exits that provably return
+// what the full loops return (see three_height); the branch-and-bound's
+// batched kernels are built with it.
+#ifndef CC_EXACT_SHORTCUTS
+#define CC_EXACT_SHORTCUTS 0
+#endif
 #include "rsb.hpp"
 
 namespace ccd {
@@ -150,6 +156,19 @@ CCD bool three_can_cover(double r1, double r2, double r3, double w, double h) {
 static __device__ __noinline__ double three_height(double r1, double r2, double r3, double ub_w) {
   const double q1 = mul_rd(4.0, r1), q2 = mul_rd(4.0, r2), q3 = mul_rd(4.0, r3);
   double lo = 0.0, hi = mul_rd(2.0, sqrt_rd(r3));
+#if CC_EXACT_SHORTCUTS
+  // The covered width sum(chord(q_i, RU(h*h))) is non-increasing in h >= 0
+  // (every operation is monotone), every midpoint the loop below visits is a
+  // positive double, and lo stays 0 as long as the test fails.  So when the
+  // test already fails at the smallest positive double, the loop would halve
+  // hi down to zero (up to ~1075 iterations) and return 0: return it now.
+  // hi > 0 excludes NaN bounds, for which the loop behaves differently.
+  if (hi > 0.0) {
+    const double tiny = 0x1p-1074;
+    const double hsq = mul_ru(tiny, tiny);
+    if (add_rd(chord(q1, hsq), add_rd(chord(q2, hsq), chord(q3, hsq))) < ub_w) return 0.0;
+  }
+#endif
   for (int it = 0; it < kBisectCap; ++it) {
     double mid = rn_mid(lo, hi);
     if (mid <= lo || mid >= hi) return lo;
@@ -183,6 +202,13 @@ static __device__ __noinline__ double nd_maximize(double ub_w, const double* d, 
   if (n == 3) return three_height(d[0], d[1], d[2], ub_w);
   double hi = mul_rd(2.0, sqrt_rd(d[n - 1]));
   double lo = 0.0;
+#if CC_EXACT_SHORTCUTS
+  // nd_can_cover_no_small(ub_w, h) = (sum of chords at ub_w >= h) or
+  // (sum of chords at h >= ub_w): both parts are non-increasing in h, so a
+  // test that fails at the smallest positive double fails at every midpoint
+  // and the loop returns 0 (as in three_height).
+  if (hi > 0.0 && !nd_can_cover_no_small(ub_w, 0x1p-1074, d, n)) return 0.0;
+#endif
   for (int it = 0; it < kBisectCap; ++it) {
     double h = rn_mid(lo, hi);
     if (h <= lo || h >= hi) return lo;

@@ -35,6 +35,46 @@ CCD int rsb_chain_verdict(const Konst& K, const cell7& in, Trap& tp) {
   return -1;
 }
 
+// Steps 0..18 of rsb_cell_verdict (everything before C14): the verdict if
+// one of them proves the cell, -2 if only C14 is left to try.
+CCD int rsb_verdict_before_c14(const Konst& K, const cell7& in, Trap& tp) {
+  if (rsbc::s1_even_split(in.la, in.r[0], in.r[1])) return 0;
+  if (rsbc::s23_uneven_split(in.la, in.r[0], in.r[1])) return 1;
+  if (rsbc::s23_uneven_split(in.la, in.r[0], in.r[2])) return 2;
+  if (rsbc::s4_opposite_corners(in.la, in.r[0], in.r[1], in.r[2])) return 3;
+  if (rsbc::s5_vertical_strip(K, in.la, in.r[0], in.r[1], in.r[2])) return 4;
+  const cell7 c = rsbc::tighten(in);
+  vals7 v;
+  v.R = rsbc::remaining_weight(c);
+  if (rsbc::is_empty(c) || empty(v.R)) return 5;
+  if (rsbc::c1_even_split(c, v)) return 6;
+  if (rsbc::c2_uneven_split(c)) return 7;
+  if (rsbc::c3_multi_disk_strip(c, v, tp)) return 8;
+  if (rsbc::c4_wall_building(K, c, v)) return 9;
+  if (rsbc::c5_corner_explicit(c, v, tp)) return 10;
+  if (rsbc::c6_opposite_corners(c, v, tp)) return 11;
+  if (rsbc::c7_vertical_strip(K, c, v)) return 12;
+  const rsbc::cover22 cov = rsbc::two_by_two(c, tp);
+  v.cover_width = cov.width;
+  v.cover_c4 = cov.c4;
+  if (cov.width > 0 && (cov.width > c.la.hi || rsbc::c8_two_by_two_strip(K, c, v, tp))) return 13;
+  if (rsbc::c9_l_shaped(K, c)) return 14;
+  if (rsbc::c10_corner_wall(K, c, tp)) return 15;
+  if (rsbc::c11_advanced_strip(c)) return 16;
+  if (rsbc::c12_five_disk(K, c, v, tp)) return 17;
+  if (rsbc::c13_six_disk(K, c, v, tp)) return 18;
+  return -2;
+}
+
+// C14 alone on a cell that reached it: 19 if it proves the cell, else -1.
+CCD int rsb_verdict_c14(const Konst& K, const cell7& in, Trap& tp) {
+  const cell7 c = rsbc::tighten(in);
+  vals7 v;
+  v.R = rsbc::remaining_weight(c);
+  v.cover_width = 0.0;
+  return rsbc::c14_seven_disk(K, c, v, tp) ? 19 : -1;
+}
+
 // The cell-level shortcuts S1..S5 (rectangle_size_bound.cu:162 and :174-176)
 // on the untightened cell, then the leaf part.
 CCD int rsb_cell_verdict(const Konst& K, const cell7& in, Trap& tp) {

@@ -102,6 +102,11 @@ hipError_t launch(hipFunction_t f, dim3 grid, dim3 block, size_t shmem, hipStrea
 // Level buffers shared by the enumeration and branch-and-bound drivers.
 struct Scratch {
   DevBuf cells, aux, s1, s2, s3, leaves, next, verdict;
+  void* fast = nullptr;                 // batched branch-and-bound state (rsb_fast.cpp)
+  void (*fast_free)(void*) = nullptr;
+  ~Scratch() {
+    if (fast_free) fast_free(fast);
+  }
 };
 Scratch& scratch(cc_ctx* ctx);
 

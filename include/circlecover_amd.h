@@ -181,6 +181,31 @@ typedef struct cc_bnb_stats {
 int cc_rsb_bnb(cc_ctx* ctx, const cc_rsb_cell* cells, size_t ncells, int max_depth, cc_rsb_cell* out, size_t cap,
                size_t* n_out, cc_bnb_stats* stats);
 
+/* Launch shape of the batched branch-and-bound. */
+typedef struct cc_fast_params {
+  int cpw_a;         /* cells a wave takes per work-queue fetch, stage A (multiple of 64) */
+  int cpw_b;         /* the same for stage B (C14) */
+  int blocks_per_cu; /* persistent workgroups (256 threads) per compute unit */
+  int use_graph;     /* capture the depth loop as a hipGraph */
+} cc_fast_params;
+
+/* The branch-and-bound of cc_rsb_bnb, batched (no reference counterpart):
+ * persistent stage kernels pull cells from device work queues (stage A:
+ * every step before C14; stage B: C14 on the cells stage A leaves), all
+ * counts stay in device memory, the loop over depths is one hipGraph, and
+ * bisections whose test fails at their smallest argument exit at once (an
+ * exact shortcut: they return what the full loop returns).  Same unproven
+ * cells, per-depth counts and verdict histogram as cc_rsb_bnb; if a level
+ * outgrows the queues the call falls back to cc_rsb_bnb.  params == NULL
+ * uses the autotuned shape (or a default). */
+int cc_rsb_bnb_batched(cc_ctx* ctx, const cc_rsb_cell* cells, size_t ncells, int max_depth,
+                       const cc_fast_params* params, cc_rsb_cell* out, size_t cap, size_t* n_out,
+                       cc_bnb_stats* stats);
+/* Times cc_rsb_bnb_batched over a small grid of launch shapes on `cells`
+ * and keeps the fastest as the context's default. */
+int cc_rsb_autotune(cc_ctx* ctx, const cc_rsb_cell* cells, size_t ncells, int max_depth, cc_fast_params* best,
+                    double* best_ms);
+
 /* get_subinterval (Prover/subintervals.hpp:32-43) on the host: the n pieces
  * of [lb, ub] as 2n doubles (lb + diff*i with one rounding, the last piece
  * ending at ub).  Used to split lambda ranges into initial cells. */
