@@ -97,11 +97,9 @@ extern "C" __global__ void ccrd_fast_reset(fast_ctrl* __restrict__ ctrl, unsigne
 // Stage A of depth d: every step before C14 on each cell of `in`.  Proven
 // cells are counted; cells that only C14 can still prove go to `listb` as
 // (index | trap << 31).
-extern "C" __global__ void __launch_bounds__(256) ccrd_fast_stage_a(const rec112* __restrict__ in, uint64_t cap_in,
-                                                                     int d, unsigned cpw,
-                                                                     const Konst* __restrict__ kp,
-                                                                     uint32_t* __restrict__ listb, uint64_t cap,
-                                                                     fast_ctrl* __restrict__ ctrl) {
+__device__ __forceinline__ void stage_a(const rec112* __restrict__ in, uint64_t cap_in, int d, unsigned cpw,
+                                        const Konst* __restrict__ kp, uint32_t* __restrict__ listb, uint64_t cap,
+                                        fast_ctrl* __restrict__ ctrl) {
   __shared__ unsigned int bins[21];
   __shared__ unsigned int traps;
   if (threadIdx.x < 21) bins[threadIdx.x] = 0;
@@ -142,14 +140,11 @@ extern "C" __global__ void __launch_bounds__(256) ccrd_fast_stage_a(const rec112
 // Stage B of depth d: C14 on the listed cells.  A proven cell is counted;
 // an unproven cell is split into `next` (depth d+1) or, at the depth limit,
 // appended to `unproven`.
-extern "C" __global__ void __launch_bounds__(256) ccrd_fast_stage_b(const rec112* __restrict__ in,
-                                                                     const uint32_t* __restrict__ listb,
-                                                                     uint64_t cap_list, int d,
-                                                                     int at_limit, unsigned cpw,
-                                                                     const Konst* __restrict__ kp,
-                                                                     rec112* __restrict__ next, uint64_t cap_next,
-                                                                     rec112* __restrict__ unproven, uint64_t cap_u,
-                                                                     fast_ctrl* __restrict__ ctrl) {
+__device__ __forceinline__ void stage_b(const rec112* __restrict__ in, const uint32_t* __restrict__ listb,
+                                        uint64_t cap_list, int d, int at_limit, unsigned cpw,
+                                        const Konst* __restrict__ kp, rec112* __restrict__ next, uint64_t cap_next,
+                                        rec112* __restrict__ unproven, uint64_t cap_u,
+                                        fast_ctrl* __restrict__ ctrl) {
   __shared__ unsigned int bins[2];
   __shared__ unsigned int traps;
   if (threadIdx.x < 2) bins[threadIdx.x] = 0;
@@ -200,3 +195,24 @@ extern "C" __global__ void __launch_bounds__(256) ccrd_fast_stage_b(const rec112
     if (traps) atomicAdd(&ctrl->trap_cells, (unsigned long long)traps);
   }
 }
+
+// Kernel variants per occupancy target: waves per SIMD W caps the register
+// budget at 512/W VGPRs (spills go to scratch); the autotuner picks W.
+#define CC_STAGE_KERNELS(W)                                                                                    \
+  extern "C" __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, W)))                 \
+  ccrd_fast_stage_a_w##W(const rec112* __restrict__ in, uint64_t cap_in, int d, unsigned cpw,                  \
+                         const Konst* __restrict__ kp, uint32_t* __restrict__ listb, uint64_t cap,            \
+                         fast_ctrl* __restrict__ ctrl) {                                                      \
+    stage_a(in, cap_in, d, cpw, kp, listb, cap, ctrl);                                                         \
+  }                                                                                                           \
+  extern "C" __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, W)))                 \
+  ccrd_fast_stage_b_w##W(const rec112* __restrict__ in, const uint32_t* __restrict__ listb, uint64_t cap_list, \
+                         int d, int at_limit, unsigned cpw, const Konst* __restrict__ kp,                      \
+                         rec112* __restrict__ next, uint64_t cap_next, rec112* __restrict__ unproven,         \
+                         uint64_t cap_u, fast_ctrl* __restrict__ ctrl) {                                      \
+    stage_b(in, listb, cap_list, d, at_limit, cpw, kp, next, cap_next, unproven, cap_u, ctrl);                \
+  }
+
+CC_STAGE_KERNELS(1)
+CC_STAGE_KERNELS(2)
+CC_STAGE_KERNELS(4)

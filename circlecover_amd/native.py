@@ -75,7 +75,7 @@ lib.cc_rsb_enumerate.argtypes = [_vp, _vp, _sz, _vp, _vp, _sz, _P(_sz)]
 lib.cc_rsb_bnb.argtypes = [_vp, _vp, _sz, _i, _vp, _sz, _P(_sz), _vp]
 class FastParams(ctypes.Structure):
     _fields_ = [("cpw_a", ctypes.c_int), ("cpw_b", ctypes.c_int), ("blocks_per_cu", ctypes.c_int),
-                ("use_graph", ctypes.c_int)]
+                ("use_graph", ctypes.c_int), ("waves_per_simd", ctypes.c_int)]
 
 
 lib.cc_rsb_bnb_batched.argtypes = [_vp, _vp, _sz, _i, _vp, _vp, _sz, _P(_sz), _vp]
@@ -292,7 +292,8 @@ class Prover:
 
     def rsb_bnb_batched(self, cells: np.ndarray, max_depth: int, params=None, cap: int = 1 << 16):
         """Batched branch-and-bound; params = (cpw_a, cpw_b, blocks_per_cu,
-        use_graph) or None for the tuned default.  Returns (unproven, BnbStats)."""
+        use_graph, waves_per_simd) or None for the tuned default.  Returns
+        (unproven, BnbStats)."""
         cells = np.ascontiguousarray(cells, dtype=np.float64).reshape(-1, 14)
         st = BnbStats()
         pp = ctypes.byref(FastParams(*params)) if params is not None else None
@@ -307,7 +308,7 @@ class Prover:
         ms = ctypes.c_double()
         self._check(lib.cc_rsb_autotune(self.ctx, _ptr(cells), cells.shape[0], max_depth, ctypes.byref(best),
                                         ctypes.byref(ms)))
-        return (best.cpw_a, best.cpw_b, best.blocks_per_cu, best.use_graph), ms.value
+        return (best.cpw_a, best.cpw_b, best.blocks_per_cu, best.use_graph, best.waves_per_simd), ms.value
 
     def rsb_eval_masks(self, cells: np.ndarray) -> np.ndarray:
         cells = np.ascontiguousarray(cells, dtype=np.float64).reshape(-1, 14)

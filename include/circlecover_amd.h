@@ -187,6 +187,7 @@ typedef struct cc_fast_params {
   int cpw_b;         /* the same for stage B (C14) */
   int blocks_per_cu; /* persistent workgroups (256 threads) per compute unit */
   int use_graph;     /* capture the depth loop as a hipGraph */
+  int waves_per_simd; /* occupancy target of the stage kernels: 1, 2 or 4 (register budget 512/W) */
 } cc_fast_params;
 
 /* The branch-and-bound of cc_rsb_bnb, batched (no reference counterpart):

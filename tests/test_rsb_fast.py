@@ -20,9 +20,10 @@ def _same_stats(a, b):
 @pytest.mark.gpu
 @pytest.mark.parametrize("n,depth,params", [
     (16384, C3_DEPTH, None),
-    (16384, C3_DEPTH, (64, 64, 1, 0)),
-    (4096, 3, (1024, 128, 4, 1)),
-    (1024, 0, (128, 64, 2, 1)),
+    (16384, C3_DEPTH, (64, 64, 1, 0, 1)),
+    (16384, C3_DEPTH, (256, 128, 8, 1, 4)),
+    (4096, 3, (1024, 128, 4, 1, 2)),
+    (1024, 0, (128, 64, 2, 1, 1)),
 ])
 def test_gpu_batched_matches_oracle(prover, n, depth, params):
     cells = c3_cells(n, seed=n)
