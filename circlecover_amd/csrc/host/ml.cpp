@@ -1,0 +1,328 @@
+// Learned criterion ranker (ml.hip) and first-success evaluation
+// (c2_first.hip): training loop, gradient access, batched ranking and the
+// ranked-versus-plain evaluation of the ten-criterion box set.
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "runtime.hpp"
+
+using namespace cchost;
+
+namespace {
+
+constexpr int kIn = 8, kH = 64, kOut = 10, kRec = kIn + 4 * kH + kOut;
+constexpr int kW1 = 0, kB1 = kIn * kH, kW2 = kB1 + kH, kB2 = kW2 + kH * kH, kW3 = kB2 + kH, kB3 = kW3 + kH * kOut;
+static_assert(kB3 + kOut == CC_RANKER_PARAMS, "parameter count");
+
+struct MlState {
+  DevBuf soa, label, perm, rec, loss, params, m, v, grad, norm, wb, bias, top1, idx, counts, first, evals, trap, bstart;
+};
+
+MlState& ml(cc_ctx* ctx) {
+  Scratch& S = scratch(ctx);
+  if (!S.ml) {
+    S.ml = new MlState();
+    S.ml_free = [](void* p) { delete static_cast<MlState*>(p); };
+  }
+  return *static_cast<MlState*>(S.ml);
+}
+
+struct Rng {  // splitmix64
+  uint64_t s;
+  uint64_t next() {
+    uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+  }
+  double unit() { return (double)(next() >> 11) * 0x1.0p-53; }
+};
+
+float feature_host(const double* soa, size_t ld, size_t i, int k) {
+  const double lo = soa[(2 * (k >> 1)) * ld + i], hi = soa[(2 * (k >> 1) + 1) * ld + i];
+  if (k & 1) {
+    const double w = hi - lo;
+    return (float)(w > 0x1p-60 ? std::log2(w) : -60.0);
+  }
+  return (float)(0.5 * (lo + hi));
+}
+
+uint16_t bf16_bits(float x) {
+  uint32_t u;
+  std::memcpy(&u, &x, 4);
+  if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)(u >> 16);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+
+int upload(cc_ctx* ctx, DevBuf& b, const void* src, size_t bytes) {
+  CC_HIP(ctx, b.ensure(bytes));
+  if (bytes) CC_HIP(ctx, hipMemcpyAsync(b.p, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+  return CC_OK;
+}
+
+// Uploads the SoA boxes as a dense (8, n) block.
+int upload_soa(cc_ctx* ctx, DevBuf& b, const double* soa, size_t n, size_t ld) {
+  CC_HIP(ctx, b.ensure(8 * n * sizeof(double) + 8));
+  for (int r = 0; r < 8; ++r)
+    if (n)
+      CC_HIP(ctx, hipMemcpyAsync(static_cast<double*>(b.p) + r * n, soa + r * ld, n * sizeof(double),
+                                 hipMemcpyHostToDevice, ctx->stream));
+  return CC_OK;
+}
+
+// BF16 weight block [W1 | W2 | W3] and FP32 biases [b1 | b2 | b3] of a model.
+int upload_inference(cc_ctx* ctx, MlState& M, const cc_ranker* model) {
+  std::vector<uint16_t> wb(kIn * kH + kH * kH + kH * kOut);
+  std::vector<float> bias(2 * kH + kOut);
+  size_t o = 0;
+  for (int p = kW1; p < kB1; ++p) wb[o++] = bf16_bits(model->params[p]);
+  for (int p = kW2; p < kB2; ++p) wb[o++] = bf16_bits(model->params[p]);
+  for (int p = kW3; p < kB3; ++p) wb[o++] = bf16_bits(model->params[p]);
+  for (int j = 0; j < kH; ++j) bias[j] = model->params[kB1 + j];
+  for (int j = 0; j < kH; ++j) bias[kH + j] = model->params[kB2 + j];
+  for (int c = 0; c < kOut; ++c) bias[2 * kH + c] = model->params[kB3 + c];
+  int rc = upload(ctx, M.wb, wb.data(), wb.size() * sizeof(uint16_t));
+  if (!rc) rc = upload(ctx, M.bias, bias.data(), bias.size() * sizeof(float));
+  if (!rc) rc = upload(ctx, M.norm, model->norm, sizeof model->norm);
+  return rc;
+}
+
+int rank_dev(cc_ctx* ctx, MlState& M, const void* d_soa, size_t n, float* d_logits) {
+  hipFunction_t f;
+  CC_HIP(ctx, kernel(ctx, "ccml_rank", &f));
+  CC_HIP(ctx, M.top1.ensure(n + 1));
+  const uint64_t n64 = n, ld = n;
+  const uint64_t tiles = (n + 31) / 32;
+  unsigned blocks = (unsigned)((tiles + 3) / 4);
+  if (blocks > 4096) blocks = 4096;
+  if (blocks == 0) blocks = 1;
+  CC_HIP(ctx, launch(f, dim3(blocks), dim3(256), 0, ctx->stream, d_soa, n64, ld, M.norm.p, M.wb.p, M.bias.p, M.top1.p,
+                     d_logits));
+  return CC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int cc_ranker_train(cc_ctx* ctx, const double* soa, size_t n, size_t ld, const uint8_t* label, int epochs,
+                    int batch, float lr, uint64_t seed, cc_ranker* model, float* epoch_loss) {
+  if (!ctx || !soa || !label || !model || ld < n || epochs < 0 || batch < 1 || batch > (1 << 20)) return CC_E_ARG;
+  CC_HIP(ctx, hipSetDevice(ctx->device));
+  MlState& M = ml(ctx);
+  std::vector<uint32_t> lab;  // boxes some criterion proves
+  for (size_t i = 0; i < n; ++i)
+    if (label[i] < kOut) lab.push_back((uint32_t)i);
+  if (lab.empty()) return fail(ctx, CC_E_ARG, "no proven boxes to train on");
+  // feature normalisation over the training boxes
+  for (int k = 0; k < kIn; ++k) {
+    double s = 0, s2 = 0;
+    for (uint32_t i : lab) {
+      const double f = feature_host(soa, ld, i, k);
+      s += f;
+      s2 += f * f;
+    }
+    const double mean = s / lab.size(), var = s2 / lab.size() - mean * mean;
+    model->norm[k] = (float)mean;
+    model->norm[kIn + k] = (float)(var > 1e-12 ? std::sqrt(var) : 1.0);
+  }
+  // He-uniform initial weights, zero biases
+  Rng rng{seed};
+  std::memset(model->params, 0, sizeof model->params);
+  auto init = [&](int off, int fan_in, int count) {
+    const double a = std::sqrt(6.0 / fan_in);
+    for (int p = 0; p < count; ++p) model->params[off + p] = (float)((2.0 * rng.unit() - 1.0) * a);
+  };
+  init(kW1, kIn, kIn * kH);
+  init(kW2, kH, kH * kH);
+  init(kW3, kH, kH * kOut);
+  int rc = upload_soa(ctx, M.soa, soa, n, ld);
+  if (!rc) rc = upload(ctx, M.label, label, n);
+  if (!rc) rc = upload(ctx, M.params, model->params, sizeof model->params);
+  if (!rc) rc = upload(ctx, M.norm, model->norm, sizeof model->norm);
+  if (rc) return rc;
+  CC_HIP(ctx, M.m.ensure(sizeof model->params));
+  CC_HIP(ctx, M.v.ensure(sizeof model->params));
+  CC_HIP(ctx, hipMemsetAsync(M.m.p, 0, sizeof model->params, ctx->stream));
+  CC_HIP(ctx, hipMemsetAsync(M.v.p, 0, sizeof model->params, ctx->stream));
+  CC_HIP(ctx, M.rec.ensure((size_t)batch * kRec * sizeof(float)));
+  CC_HIP(ctx, M.loss.ensure(lab.size() * sizeof(float)));
+  CC_HIP(ctx, M.perm.ensure(lab.size() * sizeof(uint32_t)));
+  hipFunction_t ffb, fga;
+  CC_HIP(ctx, kernel(ctx, "ccml_fwd_bwd", &ffb));
+  CC_HIP(ctx, kernel(ctx, "ccml_grad_adam", &fga));
+  const uint64_t ld_dev = n;
+  const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+  int t = 0;
+  std::vector<uint32_t> perm = lab;
+  std::vector<float> losses(lab.size());
+  for (int e = 0; e < epochs; ++e) {
+    if (ctx->interrupt) return fail(ctx, CC_E_INTERRUPTED, "interrupted");
+    for (size_t i = perm.size() - 1; i > 0; --i) {  // Fisher-Yates
+      const size_t j = (size_t)(rng.next() % (i + 1));
+      const uint32_t tmp = perm[i];
+      perm[i] = perm[j];
+      perm[j] = tmp;
+    }
+    CC_HIP(ctx, hipMemcpyAsync(M.perm.p, perm.data(), perm.size() * sizeof(uint32_t), hipMemcpyHostToDevice,
+                               ctx->stream));
+    for (size_t off = 0; off < perm.size(); off += (size_t)batch) {
+      const int B = (int)(perm.size() - off < (size_t)batch ? perm.size() - off : (size_t)batch);
+      ++t;
+      const uint32_t* sample = static_cast<const uint32_t*>(M.perm.p) + off;
+      float* loss = static_cast<float*>(M.loss.p) + off;
+      CC_HIP(ctx, launch(ffb, dim3((unsigned)((B + 255) / 256)), dim3(256), 0, ctx->stream, M.soa.p, ld_dev, sample,
+                         M.label.p, B, M.norm.p, M.params.p, M.rec.p, loss));
+      void* no_grad = nullptr;
+      const int update = 1;
+      CC_HIP(ctx, launch(fga, dim3((CC_RANKER_PARAMS + 255) / 256), dim3(256), 0, ctx->stream, M.rec.p, B,
+                         M.params.p, M.m.p, M.v.p, t, lr, b1, b2, eps, update, no_grad));
+    }
+    if (epoch_loss) {
+      CC_HIP(ctx, hipMemcpyAsync(losses.data(), M.loss.p, losses.size() * sizeof(float), hipMemcpyDeviceToHost,
+                                 ctx->stream));
+      CC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+      double s = 0;
+      for (float x : losses) s += x;
+      epoch_loss[e] = (float)(s / losses.size());
+    }
+  }
+  CC_HIP(ctx, hipMemcpyAsync(model->params, M.params.p, sizeof model->params, hipMemcpyDeviceToHost, ctx->stream));
+  CC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return CC_OK;
+}
+
+int cc_ranker_gradient(cc_ctx* ctx, const double* soa, size_t n, size_t ld, const uint8_t* label,
+                       const uint32_t* sample, int B, const cc_ranker* model, float* grad, float* loss) {
+  if (!ctx || !soa || !label || !sample || !model || !grad || B < 1 || ld < n) return CC_E_ARG;
+  for (int s = 0; s < B; ++s)
+    if (sample[s] >= n || label[sample[s]] >= kOut) return fail(ctx, CC_E_ARG, "sample without a label");
+  CC_HIP(ctx, hipSetDevice(ctx->device));
+  MlState& M = ml(ctx);
+  int rc = upload_soa(ctx, M.soa, soa, n, ld);
+  if (!rc) rc = upload(ctx, M.label, label, n);
+  if (!rc) rc = upload(ctx, M.params, model->params, sizeof model->params);
+  if (!rc) rc = upload(ctx, M.norm, model->norm, sizeof model->norm);
+  if (!rc) rc = upload(ctx, M.perm, sample, (size_t)B * sizeof(uint32_t));
+  if (rc) return rc;
+  CC_HIP(ctx, M.rec.ensure((size_t)B * kRec * sizeof(float)));
+  CC_HIP(ctx, M.loss.ensure((size_t)B * sizeof(float)));
+  CC_HIP(ctx, M.grad.ensure(sizeof model->params));
+  hipFunction_t ffb, fga;
+  CC_HIP(ctx, kernel(ctx, "ccml_fwd_bwd", &ffb));
+  CC_HIP(ctx, kernel(ctx, "ccml_grad_adam", &fga));
+  const uint64_t ld_dev = n;
+  CC_HIP(ctx, launch(ffb, dim3((unsigned)((B + 255) / 256)), dim3(256), 0, ctx->stream, M.soa.p, ld_dev, M.perm.p,
+                     M.label.p, B, M.norm.p, M.params.p, M.rec.p, M.loss.p));
+  void* none = nullptr;
+  const int t = 1, update = 0;
+  const float z = 0.f;
+  CC_HIP(ctx, launch(fga, dim3((CC_RANKER_PARAMS + 255) / 256), dim3(256), 0, ctx->stream, M.rec.p, B, none, none,
+                     none, t, z, z, z, z, update, M.grad.p));
+  CC_HIP(ctx, hipMemcpyAsync(grad, M.grad.p, sizeof model->params, hipMemcpyDeviceToHost, ctx->stream));
+  if (loss) CC_HIP(ctx, hipMemcpyAsync(loss, M.loss.p, (size_t)B * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+  CC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return CC_OK;
+}
+
+int cc_ranker_rank(cc_ctx* ctx, const double* soa, size_t n, size_t ld, const cc_ranker* model, uint8_t* top1,
+                   float* logits) {
+  if (!ctx || !soa || !model || !top1 || ld < n) return CC_E_ARG;
+  if (n == 0) return CC_OK;
+  CC_HIP(ctx, hipSetDevice(ctx->device));
+  MlState& M = ml(ctx);
+  int rc = upload_soa(ctx, M.soa, soa, n, ld);
+  if (!rc) rc = upload_inference(ctx, M, model);
+  if (rc) return rc;
+  float* d_logits = nullptr;
+  if (logits) {
+    CC_HIP(ctx, M.grad.ensure(n * kOut * sizeof(float)));
+    d_logits = static_cast<float*>(M.grad.p);
+  }
+  if ((rc = rank_dev(ctx, M, M.soa.p, n, d_logits))) return rc;
+  CC_HIP(ctx, hipMemcpyAsync(top1, M.top1.p, n, hipMemcpyDeviceToHost, ctx->stream));
+  if (logits)
+    CC_HIP(ctx, hipMemcpyAsync(logits, d_logits, n * kOut * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+  CC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return CC_OK;
+}
+
+int cc_c2_eval_first(cc_ctx* ctx, const double* soa, size_t n, size_t ld, const cc_ranker* model, uint8_t* first,
+                     uint8_t* evals, uint8_t* trap, cc_first_stats* stats) {
+  if (!ctx || !soa || !first || !evals || !trap || ld < n) return CC_E_ARG;
+  if (n > 0xffffffffull) return fail(ctx, CC_E_ARG, "too many boxes");
+  CC_HIP(ctx, hipSetDevice(ctx->device));
+  MlState& M = ml(ctx);
+  int rc = upload_soa(ctx, M.soa, soa, n, ld);
+  if (!rc && model) rc = upload_inference(ctx, M, model);
+  if (rc) return rc;
+  CC_HIP(ctx, M.idx.ensure(n * sizeof(uint32_t) + 4));
+  CC_HIP(ctx, M.first.ensure(n + 1));
+  CC_HIP(ctx, M.evals.ensure(n + 1));
+  CC_HIP(ctx, M.trap.ensure(n + 1));
+  CC_HIP(ctx, M.counts.ensure(2 * 16 * sizeof(unsigned long long)));
+  CC_HIP(ctx, M.bstart.ensure(16 * sizeof(uint64_t)));
+  hipEvent_t e[3];
+  for (auto& x : e) CC_HIP(ctx, hipEventCreate(&x));
+  struct Cleanup {
+    hipEvent_t* e;
+    ~Cleanup() {
+      for (int k = 0; k < 3; ++k) (void)hipEventDestroy(e[k]);
+    }
+  } cleanup{e};
+  uint64_t bstart[16] = {0};
+  CC_HIP(ctx, hipEventRecord(e[0], ctx->stream));
+  if (model && n) {
+    if ((rc = rank_dev(ctx, M, M.soa.p, n, nullptr))) return rc;
+    hipFunction_t fc, fp, fs;
+    CC_HIP(ctx, kernel(ctx, "ccml_bucket_count", &fc));
+    CC_HIP(ctx, kernel(ctx, "ccml_bucket_prefix", &fp));
+    CC_HIP(ctx, kernel(ctx, "ccml_bucket_scatter", &fs));
+    unsigned long long* cnt = static_cast<unsigned long long*>(M.counts.p);
+    CC_HIP(ctx, hipMemsetAsync(cnt, 0, 16 * sizeof(unsigned long long), ctx->stream));
+    const uint64_t n64 = n;
+    CC_HIP(ctx, launch(fc, dim3(1024), dim3(256), 0, ctx->stream, M.top1.p, n64, cnt));
+    CC_HIP(ctx, launch(fp, dim3(1), dim3(64), 0, ctx->stream, cnt, cnt + 16, M.bstart.p));
+    CC_HIP(ctx, launch(fs, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, M.top1.p, n64, cnt + 16,
+                       M.idx.p));
+  } else {
+    std::vector<uint32_t> id(n);
+    for (size_t i = 0; i < n; ++i) id[i] = (uint32_t)i;
+    for (int b = 1; b <= kOut; ++b) bstart[b] = n;  // everything in bucket 0: plain order
+    if (n) CC_HIP(ctx, hipMemcpyAsync(M.idx.p, id.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice, ctx->stream));
+    CC_HIP(ctx, hipMemcpyAsync(M.bstart.p, bstart, sizeof bstart, hipMemcpyHostToDevice, ctx->stream));
+  }
+  CC_HIP(ctx, hipEventRecord(e[1], ctx->stream));
+  if (n) {
+    hipFunction_t f;
+    CC_HIP(ctx, kernel(ctx, "ccrd_c2_first", &f));
+    const uint64_t n64 = n, ld_dev = n;
+    CC_HIP(ctx, launch(f, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, M.soa.p, ld_dev, M.idx.p, n64,
+                       M.bstart.p, ctx->d_konst, M.first.p, M.evals.p, M.trap.p));
+  }
+  CC_HIP(ctx, hipEventRecord(e[2], ctx->stream));
+  CC_HIP(ctx, hipMemcpyAsync(first, M.first.p, n, hipMemcpyDeviceToHost, ctx->stream));
+  CC_HIP(ctx, hipMemcpyAsync(evals, M.evals.p, n, hipMemcpyDeviceToHost, ctx->stream));
+  CC_HIP(ctx, hipMemcpyAsync(trap, M.trap.p, n, hipMemcpyDeviceToHost, ctx->stream));
+  CC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  if (stats) {
+    float a = 0, b = 0;
+    CC_HIP(ctx, hipEventElapsedTime(&a, e[0], e[1]));
+    CC_HIP(ctx, hipEventElapsedTime(&b, e[1], e[2]));
+    std::memset(stats, 0, sizeof *stats);
+    stats->rank_ms = model ? a : 0.0;
+    stats->eval_ms = b;
+    for (size_t i = 0; i < n; ++i) {
+      stats->criterion_evals += evals[i];
+      if (first[i] < kOut) {
+        stats->proven += 1;
+        stats->evals_proven += evals[i];
+      }
+    }
+  }
+  return CC_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,446 @@
+// The reference's two prover entry points on top of the C ABI:
+//   circlecover::tight_rectangle::find_critical_intervals()
+//     (Prover/tight_rectangle/tight_rectangle.cu:458-537)
+//   circlecover::rectangle_size_bound::find_critical_intervals()
+//     (Prover/rectangle_size_bound/rectangle_size_bound.cu:312-382)
+// with the same generation schedules, per-launch merges, report format
+// (Prover/output_case.hpp:35-54, the Interval and Variables operator<<)
+// and SIGINT behaviour (Prover/interrupt.cpp:30-46).  Launches of one
+// generation are batched into few device calls; merges are applied per
+// launch exactly as the reference does, so the records are the same.
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <csignal>
+#include <cstdio>
+#include <cstring>
+#include <iomanip>
+#include <iostream>
+#include <sstream>
+#include <stdexcept>
+#include <system_error>
+#include <vector>
+
+#include "runtime.hpp"
+
+using namespace cchost;
+
+namespace cchost {
+int tr_enumerate_host(cc_ctx* ctx, const cc_tr_cell* cells, size_t ncells, const int counts[4],
+                      std::vector<cc_tr_cell>& leaves, std::vector<uint32_t>& leaf_cell, int* trap);
+}
+
+namespace {
+
+// ------------------------------------------------------------------ output
+constexpr size_t kOutputWidth = 120;
+
+void begin_case(std::ostream& o, const char* message) {  // output_case.hpp:35-49
+  const size_t mlen = std::strlen(message);
+  if (mlen >= kOutputWidth - 2) {
+    o << message << std::endl;
+    return;
+  }
+  const size_t pre = (kOutputWidth - mlen - 2) / 2, post = kOutputWidth - mlen - 2 - pre;
+  o << std::string(pre, '-') << ' ' << message << ' ' << std::string(post, '-') << std::endl;
+}
+void end_case(std::ostream& o) { o << std::string(kOutputWidth, '-') << std::endl; }  // :51-54
+
+// Interval operator<< (algcuda/interval.hpp:294-303): right-aligned, width
+// 26, precision 19.
+void put_iv(std::ostream& o, const cc_iv& x) {
+  std::ios_base::fmtflags f = o.flags(std::ios::right);
+  std::streamsize p = o.precision(19);
+  o << '[' << std::setw(26) << x.lb << ", " << std::setw(26) << x.ub << ']';
+  o.flags(f);
+  o.precision(p);
+}
+void put_cell(std::ostream& o, const cc_tr_cell& v) {  // tight_rectangle.cu:539-545
+  o << "λ = ";
+  put_iv(o, v.la);
+  for (int i = 0; i < 4; ++i) {
+    o << ", r²_" << (i + 1) << " = ";
+    put_iv(o, v.r[i]);
+  }
+}
+void put_cell(std::ostream& o, const cc_rsb_cell& v) {  // variables.hpp:112-118
+  o << "λ = ";
+  put_iv(o, v.la);
+  for (int i = 0; i < 6; ++i) {
+    o << ", r²_" << (i + 1) << " = ";
+    put_iv(o, v.r[i]);
+  }
+}
+
+// get_subinterval (subintervals.hpp:32-43), lb + diff*i fused.
+cc_iv piece(cc_iv x, int i, int n) {
+  const double diff = (x.ub - x.lb) / (double)n;
+  return cc_iv{std::fma(diff, (double)i, x.lb), i + 1 == n ? x.ub : std::fma(diff, (double)(i + 1), x.lb)};
+}
+void do_join(cc_iv& a, const cc_iv& b) {
+  if (b.lb < a.lb) a.lb = b.lb;
+  if (b.ub > a.ub) a.ub = b.ub;
+}
+bool iv_less(const cc_iv& a, const cc_iv& b) { return a.lb < b.lb || (a.lb == b.lb && a.ub < b.ub); }
+bool iv_eq(const cc_iv& a, const cc_iv& b) { return a.lb == b.lb && a.ub == b.ub; }
+
+// merge_adjacent_r1 (tight_rectangle.cu:121-159).  Records with equal r1
+// keys are all joined into one running record whatever their order, so the
+// result does not depend on the sort's treatment of ties.
+std::vector<cc_tr_cell> merge_adjacent_r1(std::vector<cc_tr_cell> v) {
+  std::vector<cc_tr_cell> res;
+  if (v.empty()) return res;
+  std::sort(v.begin(), v.end(), [](const cc_tr_cell& a, const cc_tr_cell& b) { return iv_less(a.r[0], b.r[0]); });
+  res.push_back(v.front());
+  for (const cc_tr_cell& c : v) {
+    cc_tr_cell& p = res.back();
+    const bool touch = p.r[0].lb <= c.r[0].lb ? p.r[0].ub >= c.r[0].lb : c.r[0].ub >= p.r[0].lb;
+    if (touch) {
+      for (int k = 0; k < 4; ++k) do_join(p.r[k], c.r[k]);
+      continue;
+    }
+    res.push_back(c);
+  }
+  return res;
+}
+
+// merge_critical_intervals<N> (rectangle_size_bound.cu:103-130): sort by
+// (lambda, radii) lexicographically, then join radii[N..6) of consecutive
+// records whose lambda and radii[0..N) are equal.
+std::vector<cc_rsb_cell> merge_critical(std::vector<cc_rsb_cell> v, int N) {
+  std::vector<cc_rsb_cell> res;
+  if (v.empty()) return res;
+  std::sort(v.begin(), v.end(), [](const cc_rsb_cell& a, const cc_rsb_cell& b) {
+    if (iv_less(a.la, b.la)) return true;
+    if (iv_less(b.la, a.la)) return false;
+    for (int i = 0; i < 6; ++i) {
+      if (iv_less(a.r[i], b.r[i])) return true;
+      if (iv_less(b.r[i], a.r[i])) return false;
+    }
+    return false;
+  });
+  res.push_back(v.front());
+  for (const cc_rsb_cell& c : v) {
+    cc_rsb_cell& p = res.back();
+    bool same = iv_eq(c.la, p.la);
+    for (int i = 0; same && i < N; ++i) same = iv_eq(p.r[i], c.r[i]);
+    if (same) {
+      for (int i = N; i < 6; ++i) do_join(p.r[i], c.r[i]);
+    } else {
+      res.push_back(c);
+    }
+  }
+  return res;
+}
+
+// -------------------------------------------------------------- interrupt
+volatile std::sig_atomic_t g_interrupted = 0;
+void (*g_prev_handler)(int) = nullptr;
+void on_sigint(int) {
+  if (g_interrupted) std::_Exit(SIGINT);
+  g_interrupted = 1;
+}
+struct Interruptible {  // start/stop_interruptible_computation (interrupt.cpp:38-46)
+  Interruptible() {
+    g_interrupted = 0;
+    g_prev_handler = std::signal(SIGINT, &on_sigint);
+  }
+  ~Interruptible() {
+    std::signal(SIGINT, g_prev_handler);
+    g_interrupted = 0;
+  }
+};
+
+// Host mirror of the device tr_maxdist block (tr_enum.hip).
+struct MaxDist {
+  int have_2d, have_3d;
+  cc_iv la_2d, la_3d;
+  cc_iv d_r1_2d;
+  cc_tr_cell d_r1_2d_crit[2];
+  cc_iv d_r2_2d;
+  cc_tr_cell d_r2_2d_crit[2];
+  double lb_h1_2d, ub_la_by_S1, ub_R3_2d;
+  cc_iv d_r1_3d;
+  cc_tr_cell d_r1_3d_crit[2];
+  cc_iv d_r2_3d;
+  cc_tr_cell d_r2_3d_crit[2];
+  cc_iv d_r3_3d;
+  cc_tr_cell d_r3_3d_crit[2];
+  double ub_R4_3d;
+  cc_iv S1, h2, S3;
+  double pocket_3d;
+  int trap;
+};
+
+int max_distances(cc_ctx* ctx, const std::vector<cc_tr_cell>& crit, MaxDist* out) {
+  hipFunction_t f;
+  CC_HIP(ctx, kernel(ctx, "ccrd_tr_max_distances", &f));
+  DevBuf d_cells, d_out;
+  CC_HIP(ctx, d_cells.ensure(crit.size() * sizeof(cc_tr_cell) + 8));
+  CC_HIP(ctx, d_out.ensure(sizeof(MaxDist)));
+  if (!crit.empty())
+    CC_HIP(ctx, hipMemcpyAsync(d_cells.p, crit.data(), crit.size() * sizeof(cc_tr_cell), hipMemcpyHostToDevice,
+                               ctx->stream));
+  const uint64_t n = crit.size();
+  CC_HIP(ctx, launch(f, dim3(1), dim3(64), 0, ctx->stream, d_cells.p, n, ctx->d_konst, d_out.p));
+  CC_HIP(ctx, hipMemcpyAsync(out, d_out.p, sizeof(MaxDist), hipMemcpyDeviceToHost, ctx->stream));
+  CC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return CC_OK;
+}
+
+// Batches of at most this many launches share one enumeration call.
+constexpr size_t kTrBatch = 16;
+
+// find_critical_intervals_in (tight_rectangle.cu:300-320) for the cells
+// `launches` (one reference launch each), appending each launch's merged
+// leaves in launch order.
+int tr_launches(cc_ctx* ctx, const std::vector<cc_tr_cell>& launches, bool init, std::vector<cc_tr_cell>& append_to,
+                std::ostream& err) {
+  static const int kInit[4] = {128, 128, 128, 128}, kSub[4] = {128, 128, 128, 64};
+  for (size_t b = 0; b < launches.size(); b += kTrBatch) {
+    if (g_interrupted || ctx->interrupt) return CC_E_INTERRUPTED;
+    const size_t m = std::min(kTrBatch, launches.size() - b);
+    std::vector<cc_tr_cell> leaves;
+    std::vector<uint32_t> from;
+    int trap = 0;
+    int rc = tr_enumerate_host(ctx, launches.data() + b, m, init ? kInit : kSub, leaves, from, &trap);
+    if (rc) return rc;
+    if (trap) return fail(ctx, CC_E_TRAP, "a cell reached a sqrt of a possibly negative interval");
+    std::vector<std::vector<cc_tr_cell>> per(m);
+    for (size_t i = 0; i < leaves.size(); ++i) per[from[i]].push_back(leaves[i]);
+    for (size_t c = 0; c < m; ++c) {
+      std::vector<cc_tr_cell> merged = merge_adjacent_r1(per[c]);
+      append_to.insert(append_to.end(), merged.begin(), merged.end());
+    }
+    err << "launch " << (b + m) << " of " << launches.size() << " done (" << append_to.size()
+        << " critical intervals found)                \r" << std::flush;
+  }
+  return CC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int cc_interrupt(cc_ctx* ctx) {
+  if (!ctx) return CC_E_ARG;
+  ctx->interrupt = 1;
+  return CC_OK;
+}
+
+int cc_tr_prove(cc_ctx* ctx, int la_init, int generations, FILE* out) {
+  if (!ctx) return CC_E_ARG;
+  static const int kSubdivSchedule[11] = {1, 2, 2, 2, 2, 1, 2, 1, 2, 1, 2};  // tight_rectangle.cu:50-53
+  if (la_init <= 0) la_init = 256;
+  if (generations < 0 || generations > 11) generations = 11;
+  std::ostringstream o;
+  std::ostream& err = std::cerr;
+  begin_case(o, "Critical intervals for rectangles (tight)");
+  const cc_tr_cell initial{{1.0, 2.0898841580413818342}, {{0.0, DBL_MAX}, {0.0, DBL_MAX}, {0.0, DBL_MAX}, {0.0, DBL_MAX}}};
+  std::vector<cc_tr_cell> current, next, launches;
+  err << "Initializing..." << std::endl;
+  for (int a = 0; a < la_init; ++a) {
+    cc_tr_cell s = initial;
+    s.la = piece(initial.la, a, la_init);
+    launches.push_back(s);
+  }
+  int rc = tr_launches(ctx, launches, true, next, err);
+  if (rc) return rc;
+  current.swap(next);
+  next.clear();
+  err << "Starting refinement - press Ctrl-C to interrupt and use best result achieved so far..." << std::endl;
+  {
+    Interruptible guard;
+    for (int g = 1; g <= generations; ++g) {
+      err << "Running generation " << g << " on " << current.size() << " intervals..." << std::endl;
+      launches.clear();
+      const int nla = kSubdivSchedule[g - 1];
+      for (const cc_tr_cell& s : current)
+        for (int a = 0; a < nla; ++a) {
+          cc_tr_cell c = s;
+          c.la = piece(s.la, a, nla);
+          launches.push_back(c);
+        }
+      rc = tr_launches(ctx, launches, false, next, err);
+      if (rc == CC_E_INTERRUPTED) break;  // keep the best complete generation
+      if (rc) return rc;
+      current.swap(next);
+      next.clear();
+      if (current.empty()) break;
+    }
+  }
+  if (!current.empty()) {
+    MaxDist r;
+    if ((rc = max_distances(ctx, current, &r))) return rc;
+    auto crit_line = [&](const cc_tr_cell& c) {
+      o << "\t\t";
+      put_cell(o, c);
+      o << std::endl;
+    };
+    if (r.have_2d) {  // tight_rectangle.cu:508-519
+      o << "Two disks: Range for lambda that contains 2-disk critical intervals: ";
+      put_iv(o, r.la_2d);
+      o << "; maximal distances from theoretical values: " << std::endl;
+      o << "\tr_1^2: ";
+      put_iv(o, r.d_r1_2d);
+      o << ", r_2^2: ";
+      put_iv(o, r.d_r2_2d);
+      o << ", R_3 <= " << r.ub_R3_2d << ", h_1 >= " << r.lb_h1_2d << ", lambda / S_1 <= " << r.ub_la_by_S1
+        << std::endl;
+      o << "\tDominating criticals for r_1: " << std::endl;
+      crit_line(r.d_r1_2d_crit[0]);
+      crit_line(r.d_r1_2d_crit[1]);
+      o << "\tDominating criticals for r_2: " << std::endl;
+      crit_line(r.d_r2_2d_crit[0]);
+      crit_line(r.d_r2_2d_crit[1]);
+    }
+    if (r.have_3d) {  // :521-533
+      o << "Three disks: Range for lambda that contains 3-disk critical intervals: ";
+      put_iv(o, r.la_3d);
+      o << "; maximal distances from theoretical values: " << std::endl;
+      o << "\tr_1^2: ";
+      put_iv(o, r.d_r1_3d);
+      o << ", r_2^2: ";
+      put_iv(o, r.d_r2_3d);
+      o << ", r_3^2: ";
+      put_iv(o, r.d_r3_3d);
+      o << ", R_4 <= " << r.ub_R4_3d << std::endl;
+      o << "\tS_1: ";
+      put_iv(o, r.S1);
+      o << ", h_2: ";
+      put_iv(o, r.h2);
+      o << ", S_3: ";
+      put_iv(o, r.S3);
+      o << ", pocket width <= " << r.pocket_3d << std::endl;
+      o << "\tDominating criticals for r_1: " << std::endl;
+      crit_line(r.d_r1_3d_crit[0]);
+      crit_line(r.d_r1_3d_crit[1]);
+      o << "\tDominating criticals for r_2: " << std::endl;
+      crit_line(r.d_r2_3d_crit[0]);
+      crit_line(r.d_r2_3d_crit[1]);
+      o << "\tDominating criticals for r_3: " << std::endl;
+      crit_line(r.d_r3_3d_crit[0]);
+      crit_line(r.d_r3_3d_crit[1]);
+    }
+  }
+  end_case(o);
+  const std::string s = o.str();
+  std::fwrite(s.data(), 1, s.size(), out ? out : stdout);
+  std::fflush(out ? out : stdout);
+  return CC_OK;
+}
+
+int cc_rsb_prove(cc_ctx* ctx, int la_init, int r1_init, int r2_init, int generations, FILE* out) {
+  if (!ctx) return CC_E_ARG;
+  // rectangle_size_bound.cu:36-62 and values.hpp:28-35
+  if (la_init <= 0) la_init = 256;
+  if (r1_init <= 0) r1_init = 128;
+  if (r2_init <= 0) r2_init = 128;
+  if (generations < 0) generations = 1;
+  static const int kInit[4] = {64, 64, 64, 64}, kSub[4] = {16, 16, 16, 32};
+  const double w = 9.0 / 64.0;
+  std::ostringstream o;
+  std::ostream& err = std::cerr;
+  begin_case(o, "Critical intervals for rectangles with small disks");
+  err << "Size bound: r <= " << 3.0 / 8.0 << ", critical ratio: " << 0.61 << std::endl;
+  const cc_rsb_cell initial{{1.0, 2.5}, {{0.0, w}, {0.0, w}, {0.0, w}, {0.0, w}, {0.0, w}, {0.0, w}}};
+  auto run = [&](const cc_rsb_cell& outer, int nla, int nr1, int nr2, const int* counts,
+                 std::vector<cc_rsb_cell>& append_to) -> int {
+    std::vector<cc_rsb_cell> cells((size_t)nla * nr1 * nr2);
+    const size_t m = cc_rsb_grid(&outer, nla, nr1, nr2, cells.data(), cells.size());
+    cells.resize(m);
+    const size_t chunk = 4096;
+    for (size_t b = 0; b < m; b += chunk) {
+      if (g_interrupted || ctx->interrupt) return CC_E_INTERRUPTED;
+      const size_t k = std::min(chunk, m - b);
+      std::vector<cc_rsb_cell> leaves(1 << 16);
+      size_t n = 0;
+      int rc = cc_rsb_enumerate(ctx, cells.data() + b, k, counts, leaves.data(), leaves.size(), &n);
+      if (rc == CC_E_OVERFLOW) {
+        leaves.resize(n);
+        rc = cc_rsb_enumerate(ctx, cells.data() + b, k, counts, leaves.data(), leaves.size(), &n);
+      }
+      if (rc) return rc;
+      append_to.insert(append_to.end(), leaves.begin(), leaves.begin() + n);
+      err << "cells " << (b + k) << " of " << m << " done (" << append_to.size()
+          << " critical intervals found)                \r" << std::flush;
+    }
+    return CC_OK;
+  };
+  std::vector<cc_rsb_cell> current, next;
+  err << "Initializing..." << std::endl;
+  int rc = run(initial, la_init, r1_init, r2_init, kInit, next);
+  if (rc) return rc;
+  current = merge_critical(next, 0);
+  next.clear();
+  if (current.empty()) {
+    err << "No critical regions after initialization - done!" << std::endl;
+    end_case(o);
+    const std::string s = o.str();
+    std::fwrite(s.data(), 1, s.size(), out ? out : stdout);
+    return CC_OK;
+  }
+  {
+    Interruptible guard;
+    for (int g = 1; g <= generations; ++g) {
+      err << "Running generation " << g << " on " << current.size() << " intervals..." << std::endl;
+      bool stop = false;
+      for (const cc_rsb_cell& s : current) {
+        rc = run(s, 64, 32, 32, kSub, next);  // rectangle_size_bound.cu:50-52
+        if (rc == CC_E_INTERRUPTED) {
+          stop = true;
+          break;
+        }
+        if (rc) return rc;
+      }
+      if (stop) break;
+      current = merge_critical(next, g % 3 == 0 ? 3 : g % 3 == 1 ? 2 : 1);
+      next.clear();
+      if (current.empty()) break;
+    }
+  }
+  for (const cc_rsb_cell& c : current) {  // rectangle_size_bound.cu:374-379
+    o << "Critical: ";
+    put_cell(o, c);
+    o << std::endl;
+  }
+  end_case(o);
+  const std::string s = o.str();
+  std::fwrite(s.data(), 1, s.size(), out ? out : stdout);
+  std::fflush(out ? out : stdout);
+  return CC_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------ C++ entry points
+namespace {
+struct Ctx {
+  cc_ctx* p = nullptr;
+  Ctx() {
+    const int rc = cc_init(0, &p);
+    if (rc != CC_OK) throw std::system_error(rc, std::generic_category(), "no usable gfx950 device");
+  }
+  ~Ctx() { cc_destroy(p); }
+};
+void check(cc_ctx* ctx, int rc) {
+  if (rc == CC_OK || rc == CC_E_INTERRUPTED) return;
+  if (rc == CC_E_OVERFLOW) throw std::logic_error(cc_last_error(ctx));
+  throw std::system_error(rc, std::generic_category(), cc_last_error(ctx));
+}
+}  // namespace
+
+namespace circlecover {
+namespace tight_rectangle {
+__attribute__((visibility("default"))) void find_critical_intervals() {
+  Ctx c;
+  check(c.p, cc_tr_prove(c.p, 0, -1, stdout));
+}
+}  // namespace tight_rectangle
+namespace rectangle_size_bound {
+__attribute__((visibility("default"))) void find_critical_intervals() {
+  Ctx c;
+  check(c.p, cc_rsb_prove(c.p, 0, 0, 0, -1, stdout));
+}
+}  // namespace rectangle_size_bound
+}  // namespace circlecover

@@ -104,8 +104,11 @@ struct Scratch {
   DevBuf cells, aux, s1, s2, s3, leaves, next, verdict;
   void* fast = nullptr;                 // batched branch-and-bound state (rsb_fast.cpp)
   void (*fast_free)(void*) = nullptr;
+  void* ml = nullptr;                   // ranker buffers (ml.cpp)
+  void (*ml_free)(void*) = nullptr;
   ~Scratch() {
     if (fast_free) fast_free(fast);
+    if (ml_free) ml_free(ml);
   }
 };
 Scratch& scratch(cc_ctx* ctx);

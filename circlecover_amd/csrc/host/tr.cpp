@@ -3,6 +3,7 @@
 // tight_rectangle.cu:264-298) generalised to a batch of cells, and the small
 // level-synchronous branch-and-bound on top of it.
 #include <cstring>
+#include <vector>
 
 #include "runtime.hpp"
 
@@ -80,8 +81,9 @@ int enumerate_dev(cc_ctx* ctx, const void* d_cells, size_t ncells, const int n[4
     for (int pass = 0; pass < 2; ++pass) {
       const uint64_t cap = S.leaves.cap / kCell;
       CC_HIP(ctx, hipMemsetAsync(cnt + 3, 0, sizeof(unsigned long long), ctx->stream));
+      CC_HIP(ctx, S.verdict.ensure((S.leaves.cap / kCell) * sizeof(uint32_t)));
       CC_HIP(ctx, launch(f4, dim3(blocks_for(t4, 128)), dim3(128), 0, ctx->stream, d_cells, S.s3.p, c3, n[3],
-                         ctx->d_konst, S.aux.p, S.leaves.p, cap, cnt + 3, d_trap));
+                         ctx->d_konst, S.aux.p, S.leaves.p, cap, S.verdict.p, cnt + 3, d_trap));
       if ((rc = read_counts(ctx, 5))) return rc;
       c4 = ctx->h_counts[3];
       if (c4 <= cap) break;
@@ -115,6 +117,30 @@ int upload_cells(cc_ctx* ctx, const cc_tr_cell* cells, size_t ncells) {
 }
 
 }  // namespace
+
+namespace cchost {
+
+// Leaves of a batch of cells together with the index of the cell each came
+// from (for per-launch merging of batched launches).
+int tr_enumerate_host(cc_ctx* ctx, const cc_tr_cell* cells, size_t ncells, const int counts[4],
+                      std::vector<cc_tr_cell>& leaves, std::vector<uint32_t>& leaf_cell, int* trap) {
+  CC_HIP(ctx, hipSetDevice(ctx->device));
+  int rc = upload_cells(ctx, cells, ncells);
+  if (rc) return rc;
+  size_t n = 0;
+  if ((rc = enumerate_dev(ctx, scratch(ctx).cells.p, ncells, counts, &n, trap))) return rc;
+  leaves.resize(n);
+  leaf_cell.resize(n);
+  if (n) {
+    CC_HIP(ctx, hipMemcpyAsync(leaves.data(), scratch(ctx).leaves.p, n * kCell, hipMemcpyDeviceToHost, ctx->stream));
+    CC_HIP(ctx, hipMemcpyAsync(leaf_cell.data(), scratch(ctx).verdict.p, n * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                               ctx->stream));
+    CC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  }
+  return CC_OK;
+}
+
+}  // namespace cchost
 
 extern "C" {
 

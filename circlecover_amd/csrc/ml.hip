@@ -1,0 +1,324 @@
+// Learned criterion ranker: an 8 -> 64 -> 64 -> 10 MLP (ReLU) that predicts,
+// from a 4-D box, which criterion of the ten-criterion set proves it first.
+// Weights are kept as FP32 masters and used as BF16 (round to nearest even)
+// in every forward pass; accumulation is FP32.  Kernels here run in the
+// default round-to-nearest mode (no ccrd_ prefix).
+//
+//   ccml_fwd_bwd    one lane per training sample: forward, softmax
+//                   cross-entropy, backward; stores what the weight
+//                   gradients need
+//   ccml_grad_adam  one lane per parameter: the batch gradient in a fixed
+//                   summation order (deterministic), then an Adam step
+//   ccml_rank       batched inference on MFMA (v_mfma_f32_32x32x8bf16_1k):
+//                   each wave takes 32 boxes per tile; top-1 criterion out
+//   ccml_bucket_*   groups boxes by predicted criterion (counting sort)
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace {
+
+constexpr int kIn = 8, kH = 64, kOut = 10;
+// packed parameter offsets (floats)
+constexpr int oW1 = 0, oB1 = oW1 + kIn * kH, oW2 = oB1 + kH, oB2 = oW2 + kH * kH, oW3 = oB2 + kH,
+              oB3 = oW3 + kH * kOut, kParams = oB3 + kOut;
+// per-sample record of ccml_fwd_bwd: x, h1, h2, dz1, dz2, dz3
+constexpr int rX = 0, rH1 = rX + kIn, rH2 = rH1 + kH, rD1 = rH2 + kH, rD2 = rD1 + kH, rD3 = rD2 + kH,
+              kRec = rD3 + kOut;
+
+__device__ __forceinline__ float bf16_round(float x) {
+  uint32_t u = __float_as_uint(x);
+  if ((u & 0x7f800000u) == 0x7f800000u) return x;  // inf / nan unchanged
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return __uint_as_float(u & 0xffff0000u);
+}
+__device__ __forceinline__ uint16_t bf16_bits(float x) { return (uint16_t)(__float_as_uint(bf16_round(x)) >> 16); }
+
+// Features of box i: centre and log2 width of lambda, r1, r2, r3, each
+// normalised by (value - mean) / std.  A zero width counts as 2^-60.
+__device__ __forceinline__ float feature(const double* __restrict__ soa, uint64_t ld, uint64_t i, int k,
+                                         const float* __restrict__ norm) {
+  const double lo = soa[(2 * (k >> 1)) * ld + i], hi = soa[(2 * (k >> 1) + 1) * ld + i];
+  double v;
+  if (k & 1) {
+    const double w = hi - lo;
+    v = w > 0x1p-60 ? log2(w) : -60.0;
+  } else {
+    v = 0.5 * (lo + hi);
+  }
+  return ((float)v - norm[k]) / norm[kIn + k];
+}
+
+}  // namespace
+
+extern "C" __global__ void __launch_bounds__(256) ccml_fwd_bwd(const double* __restrict__ soa, uint64_t ld,
+                                                                const uint32_t* __restrict__ sample,
+                                                                const uint8_t* __restrict__ label, int B,
+                                                                const float* __restrict__ norm,
+                                                                const float* __restrict__ params,
+                                                                float* __restrict__ rec, float* __restrict__ loss) {
+  __shared__ float w[kParams];
+  for (int p = threadIdx.x; p < kParams; p += blockDim.x) {
+    const bool bias = (p >= oB1 && p < oW2) || (p >= oB2 && p < oW3) || p >= oB3;
+    w[p] = bias ? params[p] : bf16_round(params[p]);
+  }
+  __syncthreads();
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= B) return;
+  const uint64_t i = sample[s];
+  float* r = rec + (uint64_t)s * kRec;
+  float x[kIn];
+#pragma unroll
+  for (int k = 0; k < kIn; ++k) {
+    x[k] = bf16_round(feature(soa, ld, i, k, norm));
+    r[rX + k] = x[k];
+  }
+  float h1[kH], h2[kH], z3[kOut];
+  for (int j = 0; j < kH; ++j) {
+    float a = w[oB1 + j];
+#pragma unroll
+    for (int k = 0; k < kIn; ++k) a += x[k] * w[oW1 + k * kH + j];
+    h1[j] = a > 0.f ? a : 0.f;
+    r[rH1 + j] = h1[j];
+  }
+  for (int j = 0; j < kH; ++j) {
+    float a = w[oB2 + j];
+    for (int k = 0; k < kH; ++k) a += h1[k] * w[oW2 + k * kH + j];
+    h2[j] = a > 0.f ? a : 0.f;
+    r[rH2 + j] = h2[j];
+  }
+  float zmax = -3.0e38f;
+  for (int c = 0; c < kOut; ++c) {
+    float a = w[oB3 + c];
+    for (int k = 0; k < kH; ++k) a += h2[k] * w[oW3 + k * kOut + c];
+    z3[c] = a;
+    zmax = a > zmax ? a : zmax;
+  }
+  float se = 0.f;
+  for (int c = 0; c < kOut; ++c) se += expf(z3[c] - zmax);
+  const int y = label[i];
+  loss[s] = logf(se) - (z3[y] - zmax);
+  const float inv_b = 1.0f / (float)B;
+  float d3[kOut];
+  for (int c = 0; c < kOut; ++c) {
+    d3[c] = (expf(z3[c] - zmax) / se - (c == y ? 1.f : 0.f)) * inv_b;
+    r[rD3 + c] = d3[c];
+  }
+  // dz2 = (W3 dz3) * [h2 > 0]
+  for (int k = 0; k < kH; ++k) {
+    float a = 0.f;
+    for (int c = 0; c < kOut; ++c) a += d3[c] * w[oW3 + k * kOut + c];
+    const float d = h2[k] > 0.f ? a : 0.f;
+    h2[k] = d;  // h2 no longer needed: reuse as dz2
+    r[rD2 + k] = d;
+  }
+  // dz1 = (W2 dz2) * [h1 > 0]
+  for (int k = 0; k < kH; ++k) {
+    float a = 0.f;
+    for (int j = 0; j < kH; ++j) a += h2[j] * w[oW2 + k * kH + j];
+    r[rD1 + k] = h1[k] > 0.f ? a : 0.f;
+  }
+}
+
+// One lane per parameter: gradient over the B samples (sample order), then
+// Adam (lr, beta1, beta2, eps; step t >= 1) on the FP32 master unless
+// update == 0.  grad (may be null) receives the raw gradient.
+extern "C" __global__ void __launch_bounds__(256) ccml_grad_adam(const float* __restrict__ rec, int B,
+                                                                  float* __restrict__ params, float* __restrict__ m,
+                                                                  float* __restrict__ v, int t, float lr,
+                                                                  float beta1, float beta2, float eps, int update,
+                                                                  float* __restrict__ grad) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= kParams) return;
+  int a_off, b_off;  // gradient = sum_s a[s] * b[s] (a_off < 0: a = 1)
+  if (p < oB1) {
+    a_off = rX + p / kH;
+    b_off = rD1 + p % kH;
+  } else if (p < oW2) {
+    a_off = -1;
+    b_off = rD1 + (p - oB1);
+  } else if (p < oB2) {
+    a_off = rH1 + (p - oW2) / kH;
+    b_off = rD2 + (p - oW2) % kH;
+  } else if (p < oW3) {
+    a_off = -1;
+    b_off = rD2 + (p - oB2);
+  } else if (p < oB3) {
+    a_off = rH2 + (p - oW3) / kOut;
+    b_off = rD3 + (p - oW3) % kOut;
+  } else {
+    a_off = -1;
+    b_off = rD3 + (p - oB3);
+  }
+  float g = 0.f;
+  for (int s = 0; s < B; ++s) {
+    const float* r = rec + (uint64_t)s * kRec;
+    g += (a_off < 0 ? 1.f : r[a_off]) * r[b_off];
+  }
+  if (grad) grad[p] = g;
+  if (!update) return;
+  const float mm = beta1 * m[p] + (1.f - beta1) * g;
+  const float vv = beta2 * v[p] + (1.f - beta2) * g * g;
+  m[p] = mm;
+  v[p] = vv;
+  const float mh = mm / (1.f - powf(beta1, (float)t));
+  const float vh = vv / (1.f - powf(beta2, (float)t));
+  params[p] -= lr * mh / (sqrtf(vh) + eps);
+}
+
+// ---------------------------------------------------------------- inference
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+// Batched forward on MFMA.  Operand layouts of v_mfma_f32_32x32x8bf16_1k for
+// lane l (g = l / 32): A[m][k] with m = l % 32, k = 4g + e (e = element
+// 0..3); B[k][n] with n = l % 32, k = 4g + e; D[m][n] with n = l % 32,
+// m = 8 (r / 4) + 4g + (r % 4) for accumulator r = 0..15.
+// wb: BF16 weights as [W1 (8x64) | W2 (64x64) | W3 (64x10)] row-major (k, n);
+// bias: FP32 [b1 | b2 | b3].  top1[i] = argmax logit (lowest index on
+// ties); logits (may be null) receives the 10 logits of each box.
+extern "C" __global__ void __launch_bounds__(256) ccml_rank(const double* __restrict__ soa, uint64_t n, uint64_t ld,
+                                                             const float* __restrict__ norm,
+                                                             const uint16_t* __restrict__ wb,
+                                                             const float* __restrict__ bias,
+                                                             uint8_t* __restrict__ top1,
+                                                             float* __restrict__ logits) {
+  __shared__ uint16_t act[4][32][kH];  // per wave: one activation tile, bf16
+  __shared__ float out[4][32][32];     // per wave: logits tile
+  const int wv = threadIdx.x >> 6, l = threadIdx.x & 63, g = l >> 5, col = l & 31;
+  const uint16_t* W1 = wb;
+  const uint16_t* W2 = wb + kIn * kH;
+  const uint16_t* W3 = W2 + kH * kH;
+  const uint64_t tiles = (n + 31) / 32;
+  for (uint64_t tile = (uint64_t)blockIdx.x * 4 + wv; tile < tiles; tile += (uint64_t)gridDim.x * 4) {
+    const uint64_t i = tile * 32 + col;
+    const bool live = i < n;
+    // layer 1: A = features (k = 4g..4g+3 of box i)
+    bf16x4 a;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) a[e] = (short)(live ? bf16_bits(feature(soa, ld, i, 4 * g + e, norm)) : 0);
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+      bf16x4 b;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) b[e] = (short)W1[(4 * g + e) * kH + 32 * t + col];
+      f32x16 acc = {};
+      acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc, 0, 0, 0);
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int m = 8 * (r >> 2) + 4 * g + (r & 3);
+        const float z = acc[r] + bias[32 * t + col];
+        act[wv][m][32 * t + col] = bf16_bits(z > 0.f ? z : 0.f);
+      }
+    }
+    __builtin_amdgcn_s_waitcnt(0);
+    __builtin_amdgcn_wave_barrier();
+    // layer 2: K = 64 in 8 steps of 8
+    f32x16 acc2[2] = {{}, {}};
+#pragma unroll
+    for (int s = 0; s < 8; ++s) {
+      bf16x4 a2;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) a2[e] = (short)act[wv][col][8 * s + 4 * g + e];
+#pragma unroll
+      for (int t = 0; t < 2; ++t) {
+        bf16x4 b;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) b[e] = (short)W2[(8 * s + 4 * g + e) * kH + 32 * t + col];
+        acc2[t] = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a2, b, acc2[t], 0, 0, 0);
+      }
+    }
+    __builtin_amdgcn_s_waitcnt(0);
+    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+    for (int t = 0; t < 2; ++t)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int m = 8 * (r >> 2) + 4 * g + (r & 3);
+        const float z = acc2[t][r] + bias[kH + 32 * t + col];
+        act[wv][m][32 * t + col] = bf16_bits(z > 0.f ? z : 0.f);
+      }
+    __builtin_amdgcn_s_waitcnt(0);
+    __builtin_amdgcn_wave_barrier();
+    // layer 3: N = 10 padded to 32
+    f32x16 acc3 = {};
+#pragma unroll
+    for (int s = 0; s < 8; ++s) {
+      bf16x4 a3, b;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        a3[e] = (short)act[wv][col][8 * s + 4 * g + e];
+        b[e] = (short)(col < kOut ? W3[(8 * s + 4 * g + e) * kOut + col] : 0);
+      }
+      acc3 = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a3, b, acc3, 0, 0, 0);
+    }
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int m = 8 * (r >> 2) + 4 * g + (r & 3);
+      out[wv][m][col] = acc3[r] + (col < kOut ? bias[2 * kH + col] : 0.f);
+    }
+    __builtin_amdgcn_s_waitcnt(0);
+    __builtin_amdgcn_wave_barrier();
+    if (g == 0 && live) {
+      int best = 0;
+      float bv = out[wv][col][0];
+      for (int c = 1; c < kOut; ++c)
+        if (out[wv][col][c] > bv) {
+          bv = out[wv][col][c];
+          best = c;
+        }
+      top1[i] = (uint8_t)best;
+      if (logits)
+        for (int c = 0; c < kOut; ++c) logits[i * kOut + c] = out[wv][col][c];
+    }
+    __builtin_amdgcn_wave_barrier();
+  }
+}
+
+// Counting sort of box indices by predicted criterion: counts, then a
+// scatter with per-bucket cursors (order inside a bucket unspecified).
+extern "C" __global__ void ccml_bucket_count(const uint8_t* __restrict__ top1, uint64_t n,
+                                             unsigned long long* __restrict__ counts) {
+  __shared__ unsigned int c[kOut];
+  if (threadIdx.x < kOut) c[threadIdx.x] = 0;
+  __syncthreads();
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    atomicAdd(&c[top1[i] < kOut ? top1[i] : 0], 1u);
+  __syncthreads();
+  if (threadIdx.x < kOut && c[threadIdx.x]) atomicAdd(&counts[threadIdx.x], (unsigned long long)c[threadIdx.x]);
+}
+
+// Exclusive prefix of the bucket counts: cursor[b] = bstart[b] = first
+// position of bucket b, bstart[10] = total.  One lane.
+extern "C" __global__ void ccml_bucket_prefix(const unsigned long long* __restrict__ counts,
+                                              unsigned long long* __restrict__ cursor,
+                                              uint64_t* __restrict__ bstart) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  unsigned long long acc = 0;
+  for (int b = 0; b < kOut; ++b) {
+    cursor[b] = acc;
+    bstart[b] = acc;
+    acc += counts[b];
+  }
+  bstart[kOut] = acc;
+}
+
+// Scatter of box indices into their buckets: per block, ranks inside the
+// block from LDS counters and one global atomic per bucket and block (order
+// inside a bucket unspecified).
+extern "C" __global__ void __launch_bounds__(256) ccml_bucket_scatter(const uint8_t* __restrict__ top1, uint64_t n,
+                                                                       unsigned long long* __restrict__ cursor,
+                                                                       uint32_t* __restrict__ idx) {
+  __shared__ unsigned int cnt[kOut];
+  __shared__ unsigned long long base[kOut];
+  if (threadIdx.x < kOut) cnt[threadIdx.x] = 0;
+  __syncthreads();
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool live = i < n;
+  const int b = live ? (top1[i] < kOut ? top1[i] : 0) : 0;
+  const unsigned rank = live ? atomicAdd(&cnt[b], 1u) : 0u;
+  __syncthreads();
+  if (threadIdx.x < kOut && cnt[threadIdx.x])
+    base[threadIdx.x] = atomicAdd(&cursor[threadIdx.x], (unsigned long long)cnt[threadIdx.x]);
+  __syncthreads();
+  if (live) idx[base[b] + rank] = (uint32_t)i;
+}

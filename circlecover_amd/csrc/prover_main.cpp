@@ -1,0 +1,61 @@
+// circlecover_prover: the reference's executable (Prover/prover.cpp:33-37)
+// over libcirclecover_amd.  Without arguments it runs the tight_rectangle
+// proof and then the rectangle_size_bound proof with the reference's
+// constants.  Optional flags shorten a run of the same procedure:
+//   --only tr|rsb        run one of the two proofs
+//   --tr-la N            initial lambda launches of tight_rectangle (256)
+//   --tr-generations G   refinement generations of tight_rectangle (11)
+//   --rsb-grid A,B,C     initial lambda x r1 x r2 grid (256,128,128)
+//   --rsb-generations G  refinement generations of rectangle_size_bound (1)
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <iostream>
+#include <stdexcept>
+
+#include "circlecover_amd.h"
+
+int main(int argc, const char* argv[]) {
+  bool tr = true, rsb = true, custom = false;
+  int tr_la = 0, tr_gen = -1, rsb_la = 0, rsb_r1 = 0, rsb_r2 = 0, rsb_gen = -1;
+  for (int i = 1; i < argc; ++i) {
+    const char* a = argv[i];
+    const char* v = i + 1 < argc ? argv[i + 1] : nullptr;
+    if (!std::strcmp(a, "--only") && v) {
+      tr = !std::strcmp(v, "tr");
+      rsb = !std::strcmp(v, "rsb");
+      ++i;
+    } else if (!std::strcmp(a, "--tr-la") && v) {
+      tr_la = std::atoi(v), custom = true, ++i;
+    } else if (!std::strcmp(a, "--tr-generations") && v) {
+      tr_gen = std::atoi(v), custom = true, ++i;
+    } else if (!std::strcmp(a, "--rsb-grid") && v) {
+      if (std::sscanf(v, "%d,%d,%d", &rsb_la, &rsb_r1, &rsb_r2) != 3) {
+        std::cerr << "--rsb-grid expects A,B,C" << std::endl;
+        return 2;
+      }
+      custom = true, ++i;
+    } else if (!std::strcmp(a, "--rsb-generations") && v) {
+      rsb_gen = std::atoi(v), custom = true, ++i;
+    } else {
+      std::cerr << "unknown argument " << a << std::endl;
+      return 2;
+    }
+  }
+  if (!custom) {  // the reference's main
+    if (tr) circlecover::tight_rectangle::find_critical_intervals();
+    if (rsb) circlecover::rectangle_size_bound::find_critical_intervals();
+    return 0;
+  }
+  cc_ctx* ctx = nullptr;
+  if (cc_init(0, &ctx) != CC_OK) {
+    std::cerr << "no usable gfx950 device" << std::endl;
+    return 1;
+  }
+  int rc = CC_OK;
+  if (tr) rc = cc_tr_prove(ctx, tr_la, tr_gen, stdout);
+  if (rc == CC_OK && rsb) rc = cc_rsb_prove(ctx, rsb_la, rsb_r1, rsb_r2, rsb_gen, stdout);
+  if (rc != CC_OK && rc != CC_E_INTERRUPTED) std::cerr << "error " << rc << ": " << cc_last_error(ctx) << std::endl;
+  cc_destroy(ctx);
+  return rc == CC_OK || rc == CC_E_INTERRUPTED ? 0 : 1;
+}

@@ -156,6 +156,7 @@ extern "C" __global__ void __launch_bounds__(128) ccrd_tr_level4(const double* _
                                                                   const Konst* __restrict__ kp,
                                                                   const tr_aux* __restrict__ aux,
                                                                   cell5* __restrict__ leaves, uint64_t cap,
+                                                                  uint32_t* __restrict__ leaf_cell,
                                                                   unsigned long long* __restrict__ count,
                                                                   int* __restrict__ trap) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -184,7 +185,10 @@ extern "C" __global__ void __launch_bounds__(128) ccrd_tr_level4(const double* _
   }
   if (tp.flag) atomicOr(trap, 1);
   const uint64_t slot = wave_append(keep, count);
-  if (keep && slot < cap) leaves[slot] = v;
+  if (keep && slot < cap) {
+    leaves[slot] = v;
+    leaf_cell[slot] = s.cell;
+  }
 }
 
 // Bisection of unproven leaves for the next refinement level: the widest of
@@ -208,4 +212,112 @@ extern "C" __global__ void __launch_bounds__(256) ccrd_tr_bisect(const cell5* __
   }
   *d[best] = subinterval(*d[best], (int)(i & 1), 2);
   out[i] = c;
+}
+
+// Distances of the final critical cells from the theoretical worst cases
+// (kernel_compute_distance_from_theoretical_worst_cases,
+// tight_rectangle.cu:343-456): one lane walks the cells in order, as the
+// reference's <<<1,1>>> launch does, so the "dominating critical" records
+// are the same.  Output layout: struct tr_maxdist below (host mirror in
+// host/prover_api.cpp).
+struct tr_maxdist {
+  int have_2d, have_3d;
+  iv la_2d, la_3d;
+  iv d_r1_2d;
+  cell5 d_r1_2d_crit[2];
+  iv d_r2_2d;
+  cell5 d_r2_2d_crit[2];
+  double lb_h1_2d, ub_la_by_S1, ub_R3_2d;
+  iv d_r1_3d;
+  cell5 d_r1_3d_crit[2];
+  iv d_r2_3d;
+  cell5 d_r2_3d_crit[2];
+  iv d_r3_3d;
+  cell5 d_r3_3d_crit[2];
+  double ub_R4_3d;
+  iv S1, h2, S3;
+  double pocket_3d;
+  int trap;
+};
+
+extern "C" __global__ void ccrd_tr_max_distances(const cell5* __restrict__ crit, uint64_t n,
+                                                 const Konst* __restrict__ kp, tr_maxdist* __restrict__ o) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const Konst K = *kp;
+  Trap tp{0};
+  tr_maxdist r;
+  r.have_2d = r.have_3d = 0;
+  r.la_3d = pt(1.0);
+  r.la_2d = pt(2.0);
+  r.d_r1_2d = r.d_r2_2d = pt(0.0);
+  r.ub_R3_2d = r.ub_R4_3d = 0.0;
+  r.d_r1_3d = r.d_r2_3d = r.d_r3_3d = pt(0.0);
+  r.pocket_3d = -1.0;
+  r.lb_h1_2d = 1.0;
+  r.ub_la_by_S1 = 1.0;
+  const cell5 zero{pt(0.0), {pt(0.0), pt(0.0), pt(0.0), pt(0.0)}};
+  for (int k = 0; k < 2; ++k)
+    r.d_r1_2d_crit[k] = r.d_r2_2d_crit[k] = r.d_r1_3d_crit[k] = r.d_r2_3d_crit[k] = r.d_r3_3d_crit[k] = zero;
+  r.S1 = r.h2 = r.S3 = pt(0.0);
+  for (uint64_t i = 0; i < n; ++i) {
+    const cell5 v = crit[i];
+    if (v.r[2].hi < 0.1) {  // two-disk critical: r1 the circumcircle, r2^2 = 1/4
+      r.have_2d = 1;
+      join_into(r.la_2d, v.la);
+      const iv r1t = 0.25 * (square(v.la) + 1.0);
+      const iv d1 = v.r[0] - r1t;
+      if (d1.lo < r.d_r1_2d.lo) r.d_r1_2d_crit[0] = v;
+      if (d1.hi > r.d_r1_2d.hi) r.d_r1_2d_crit[1] = v;
+      join_into(r.d_r1_2d, d1);
+      const iv d2 = v.r[1] - pt(0.25);
+      if (d2.lo < r.d_r2_2d.lo) r.d_r2_2d_crit[0] = v;
+      if (d2.hi > r.d_r2_2d.hi) r.d_r2_2d_crit[1] = v;
+      join_into(r.d_r2_2d, d2);
+      const iv R3 = weight_for(K, v.la) - v.r[0] - v.r[1];
+      if (R3.hi > r.ub_R3_2d) r.ub_R3_2d = R3.hi;
+      const double S1 = mul_rd(2.0, sqrt_rd(add_rd(v.r[0].lo, -0.25)));
+      const double q = div_ru(v.la.hi, S1);
+      if (q > r.ub_la_by_S1) r.ub_la_by_S1 = q;
+      const double half = add_ru(v.la.hi, -mul_rd(0.5, S1));
+      const double h1 = mul_rd(2.0, sqrt_rd(add_rd(v.r[0].lo, -mul_ru(half, half))));
+      if (h1 < r.lb_h1_2d) r.lb_h1_2d = h1;
+    } else {  // three-disk critical
+      r.have_3d = 1;
+      join_into(r.la_3d, v.la);
+      const iv la2 = square(v.la);
+      const iv la4 = square(la2);
+      const iv th = (16.0 * la4 + 40.0 * la2 + 9.0) / (256.0 * la2);
+      const iv d1 = v.r[0] - th;
+      if (d1.lo < r.d_r1_3d.lo) r.d_r1_3d_crit[0] = v;
+      if (d1.hi > r.d_r1_3d.hi) r.d_r1_3d_crit[1] = v;
+      const iv d2 = v.r[1] - th;
+      if (d2.lo < r.d_r2_3d.lo) r.d_r2_3d_crit[0] = v;
+      if (d2.hi > r.d_r2_3d.hi) r.d_r2_3d_crit[1] = v;
+      const iv d3 = v.r[2] - th;
+      if (d3.lo < r.d_r3_3d.lo) r.d_r3_3d_crit[0] = v;
+      if (d3.hi > r.d_r3_3d.hi) r.d_r3_3d_crit[1] = v;
+      join_into(r.d_r1_3d, d1);
+      join_into(r.d_r2_3d, d2);
+      join_into(r.d_r3_3d, d3);
+      const iv R4 = weight_for(K, v.la) - v.r[0] - v.r[1] - v.r[2];
+      if (R4.hi > r.ub_R4_3d) r.ub_R4_3d = R4.hi;
+      const iv S1 = sqrt(4.0 * v.r[0] - 1.0, tp);
+      const iv h2 = sqrt(4.0 * v.r[1] - square(v.la - S1), tp);
+      const iv S3 = sqrt(4.0 * v.r[2] - square(1.0 - h2), tp);
+      const double pw = (v.la - S1 - S3).hi;
+      if (pw > r.pocket_3d) {
+        if (r.pocket_3d < 0.0) {
+          r.S1 = S1;
+          r.h2 = h2;
+          r.S3 = S3;
+        }
+        r.pocket_3d = pw;
+      }
+      join_into(r.S1, S1);
+      join_into(r.h2, h2);
+      join_into(r.S3, S3);
+    }
+  }
+  r.trap = tp.flag;
+  *o = r;
 }

@@ -80,6 +80,25 @@ class FastParams(ctypes.Structure):
 
 lib.cc_rsb_bnb_batched.argtypes = [_vp, _vp, _sz, _i, _vp, _vp, _sz, _P(_sz), _vp]
 lib.cc_rsb_autotune.argtypes = [_vp, _vp, _sz, _i, _vp, _P(ctypes.c_double)]
+RANKER_PARAMS = 5386
+
+
+class Ranker(ctypes.Structure):
+    _fields_ = [("params", ctypes.c_float * RANKER_PARAMS), ("norm", ctypes.c_float * 16)]
+
+    def params_array(self) -> np.ndarray:
+        return np.ctypeslib.as_array(self.params).copy()
+
+
+class FirstStats(ctypes.Structure):
+    _fields_ = [("proven", ctypes.c_uint64), ("criterion_evals", ctypes.c_uint64), ("evals_proven", ctypes.c_uint64),
+                ("rank_ms", ctypes.c_double), ("eval_ms", ctypes.c_double)]
+
+
+lib.cc_ranker_train.argtypes = [_vp, _vp, _sz, _sz, _vp, _i, _i, ctypes.c_float, _u64, _vp, _vp]
+lib.cc_ranker_gradient.argtypes = [_vp, _vp, _sz, _sz, _vp, _vp, _i, _vp, _vp, _vp]
+lib.cc_ranker_rank.argtypes = [_vp, _vp, _sz, _sz, _vp, _vp, _vp]
+lib.cc_c2_eval_first.argtypes = [_vp, _vp, _sz, _sz, _vp, _vp, _vp, _vp, _vp]
 lib.cc_tr_find_critical_in.argtypes = [_vp, _vp, _i, _vp, _sz, _P(_sz)]
 lib.cc_tr_enumerate.argtypes = [_vp, _vp, _sz, _vp, _vp, _sz, _P(_sz)]
 lib.cc_tr_bnb.argtypes = [_vp, _vp, _sz, _vp, _i, _vp, _sz, _P(_sz), _vp]
@@ -309,6 +328,49 @@ class Prover:
         self._check(lib.cc_rsb_autotune(self.ctx, _ptr(cells), cells.shape[0], max_depth, ctypes.byref(best),
                                         ctypes.byref(ms)))
         return (best.cpw_a, best.cpw_b, best.blocks_per_cu, best.use_graph, best.waves_per_simd), ms.value
+
+    # ---- learned criterion ranker
+    def ranker_train(self, soa: np.ndarray, label: np.ndarray, epochs: int = 20, batch: int = 4096,
+                     lr: float = 1e-3, seed: int = 1):
+        soa = np.ascontiguousarray(soa, dtype=np.float64)
+        label = np.ascontiguousarray(label, dtype=np.uint8)
+        model = Ranker()
+        losses = np.zeros(max(epochs, 1), dtype=np.float32)
+        self._check(lib.cc_ranker_train(self.ctx, _ptr(soa), soa.shape[1], soa.shape[1], _ptr(label), epochs, batch,
+                                        lr, seed, ctypes.byref(model), _ptr(losses)))
+        return model, losses[:epochs]
+
+    def ranker_gradient(self, soa: np.ndarray, label: np.ndarray, sample: np.ndarray, model: "Ranker"):
+        soa = np.ascontiguousarray(soa, dtype=np.float64)
+        label = np.ascontiguousarray(label, dtype=np.uint8)
+        sample = np.ascontiguousarray(sample, dtype=np.uint32)
+        grad = np.empty(RANKER_PARAMS, dtype=np.float32)
+        loss = np.empty(sample.shape[0], dtype=np.float32)
+        self._check(lib.cc_ranker_gradient(self.ctx, _ptr(soa), soa.shape[1], soa.shape[1], _ptr(label), _ptr(sample),
+                                           sample.shape[0], ctypes.byref(model), _ptr(grad), _ptr(loss)))
+        return grad, loss
+
+    def ranker_rank(self, soa: np.ndarray, model: "Ranker", with_logits: bool = False):
+        soa = np.ascontiguousarray(soa, dtype=np.float64)
+        n = soa.shape[1]
+        top1 = np.empty(n, dtype=np.uint8)
+        lg = np.empty((n, 10), dtype=np.float32) if with_logits else None
+        self._check(lib.cc_ranker_rank(self.ctx, _ptr(soa), n, n, ctypes.byref(model), _ptr(top1),
+                                       _ptr(lg) if with_logits else None))
+        return (top1, lg) if with_logits else top1
+
+    def eval_first(self, soa: np.ndarray, model: "Ranker" = None):
+        """First-success evaluation (plain order if model is None): returns
+        (first, evals, trap, FirstStats)."""
+        soa = np.ascontiguousarray(soa, dtype=np.float64)
+        n = soa.shape[1]
+        first = np.empty(n, dtype=np.uint8)
+        evals = np.empty(n, dtype=np.uint8)
+        trap = np.empty(n, dtype=np.uint8)
+        st = FirstStats()
+        self._check(lib.cc_c2_eval_first(self.ctx, _ptr(soa), n, n, ctypes.byref(model) if model is not None else None,
+                                         _ptr(first), _ptr(evals), _ptr(trap), ctypes.byref(st)))
+        return first, evals, trap, st
 
     def rsb_eval_masks(self, cells: np.ndarray) -> np.ndarray:
         cells = np.ascontiguousarray(cells, dtype=np.float64).reshape(-1, 14)

@@ -15,6 +15,7 @@
 
 #include <stddef.h>
 #include <stdint.h>
+#include <stdio.h>
 
 #ifdef __cplusplus
 extern "C" {
@@ -207,6 +208,73 @@ int cc_rsb_bnb_batched(cc_ctx* ctx, const cc_rsb_cell* cells, size_t ncells, int
 int cc_rsb_autotune(cc_ctx* ctx, const cc_rsb_cell* cells, size_t ncells, int max_depth, cc_fast_params* best,
                     double* best_ms);
 
+/* ---- learned criterion ranker (no reference counterpart) ----
+ * An 8 -> 64 -> 64 -> 10 MLP (ReLU; BF16 weights, FP32 accumulate) that
+ * predicts from a box (centre and log2 width of lambda, r1, r2, r3) which of
+ * the ten cc_eval_batch criteria proves it first.  It only reorders work:
+ * every verdict is still decided by the interval criteria. */
+#define CC_RANKER_PARAMS 5386 /* W1 8x64, b1 64, W2 64x64, b2 64, W3 64x10, b3 10 (row-major k, n) */
+typedef struct cc_ranker {
+  float params[CC_RANKER_PARAMS]; /* FP32 master weights */
+  float norm[16];                 /* feature mean[8], std[8] */
+} cc_ranker;
+
+/* Trains on the boxes with label[i] < 10 (the first proving criterion):
+ * softmax cross-entropy, Adam (beta 0.9/0.999, eps 1e-8), `batch` boxes per
+ * step, He-uniform initial weights from `seed`.  epoch_loss (epochs entries,
+ * may be NULL) receives the mean training loss of each epoch. */
+int cc_ranker_train(cc_ctx* ctx, const double* soa, size_t n, size_t ld, const uint8_t* label, int epochs,
+                    int batch, float lr, uint64_t seed, cc_ranker* model, float* epoch_loss);
+/* Gradient of the mean loss over the B boxes sample[0..B) with respect to
+ * every parameter (one forward and backward pass, no update). */
+int cc_ranker_gradient(cc_ctx* ctx, const double* soa, size_t n, size_t ld, const uint8_t* label,
+                       const uint32_t* sample, int B, const cc_ranker* model, float* grad, float* loss);
+/* Batched MFMA inference: top1[i] = predicted criterion of box i; logits
+ * (n*10, may be NULL) receives the raw outputs. */
+int cc_ranker_rank(cc_ctx* ctx, const double* soa, size_t n, size_t ld, const cc_ranker* model, uint8_t* top1,
+                   float* logits);
+
+typedef struct cc_first_stats {
+  uint64_t proven;          /* boxes some criterion proves */
+  uint64_t criterion_evals; /* criteria evaluated over all boxes */
+  uint64_t evals_proven;    /* criteria evaluated over the proven boxes */
+  double rank_ms;           /* inference and grouping (HIP events) */
+  double eval_ms;           /* criterion evaluation (HIP events) */
+} cc_first_stats;
+
+/* First-success evaluation of the ten-criterion set: criteria are tried one
+ * at a time until one proves the box.  model == NULL: plain order 0..9.
+ * Otherwise boxes are grouped by the ranker's predicted criterion, which is
+ * tried first, then the others in plain order.  Whether a box is proven
+ * does not depend on the order; first[i] = the criterion that proved it
+ * (255: none), evals[i] = criteria evaluated, trap[i] = 1 if an evaluated
+ * criterion reached a trap site. */
+int cc_c2_eval_first(cc_ctx* ctx, const double* soa, size_t n, size_t ld, const cc_ranker* model, uint8_t* first,
+                     uint8_t* evals, uint8_t* trap, cc_first_stats* stats);
+
+/* ---- whole proofs (replace find_critical_intervals(), see also the C++
+ *      declarations at the end of this header) ---- */
+
+/* The tight_rectangle proof of Prover/tight_rectangle/tight_rectangle.cu:
+ * 458-537: la_init initial lambda launches over [1, 2.0898841580413818342]
+ * (<= 0: the reference's 256), then `generations` refinement generations of
+ * the reference schedule {1,2,2,2,2,1,2,1,2,1,2} (< 0: all 11), each launch's
+ * leaves merged by merge_adjacent_r1, and the Max_distances report.  Writes
+ * the reference's stdout report to `out` (NULL: stdout), progress to
+ * stderr.  SIGINT during the generations ends the run with the last
+ * complete generation, as the reference does. */
+int cc_tr_prove(cc_ctx* ctx, int la_init, int generations, FILE* out);
+/* The rectangle_size_bound proof of rectangle_size_bound.cu:312-382:
+ * initial grid la_init x r1_init x r2_init (<= 0: 256, 128, 128), merge<0>,
+ * `generations` refinement generations (< 0: the reference's 1) with the
+ * 64x32x32 host grid per critical cell and merge<2>/<1>/<3>, then one
+ * "Critical: " line per remaining cell. */
+int cc_rsb_prove(cc_ctx* ctx, int la_init, int r1_init, int r2_init, int generations, FILE* out);
+/* Asks a running proof or branch-and-bound to stop at its next level
+ * boundary (returns CC_E_INTERRUPTED there; reference: was_interrupted,
+ * Prover/interrupt.cpp:30-46). */
+int cc_interrupt(cc_ctx* ctx);
+
 /* get_subinterval (Prover/subintervals.hpp:32-43) on the host: the n pieces
  * of [lb, ub] as 2n doubles (lb + diff*i with one rounding, the last piece
  * ending at ub).  Used to split lambda ranges into initial cells. */
@@ -224,4 +292,19 @@ int cc_debug_trace_strip(cc_ctx* ctx, const double* box, double* trace, int cap)
 #ifdef __cplusplus
 }
 #endif
+#ifdef __cplusplus
+/* The reference's C++ entry points (Prover/prover.cpp:23-31), exported by the
+ * library: each creates a context on device 0, runs cc_tr_prove /
+ * cc_rsb_prove with the reference's constants and throws std::system_error
+ * (HIP errors) or std::logic_error (capacity) as the reference does. */
+namespace circlecover {
+namespace tight_rectangle {
+void find_critical_intervals();
+}
+namespace rectangle_size_bound {
+void find_critical_intervals();
+}
+}  // namespace circlecover
+#endif
+
 #endif
