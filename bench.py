@@ -12,6 +12,17 @@ FP64 operations (reference op sequence, counted by the CPU oracle's counting
 build on a seeded sample and scaled to the batch) per kernel launch, divided
 by the launch duration measured with HIP events on the launch stream.
 `cpu_baseline` times the CPU oracle on the host cores on the same boxes.
+
+Unless --core-only is given, the line also carries `workloads`: the other
+configurations measured in the same run (not part of `value`):
+  rsb_bnb      branch-and-bound over the 256x64x64 rectangle_size_bound grid,
+               lambda in [1, 8], depth 24: level-synchronous vs batched wall
+               time, identical unproven set required;
+  tr_small     tight_rectangle: 64 lambda cells of [1, 1.5], r1..r4 in
+               64/32/32/32 pieces, three bisection levels;
+  fp32_prefilter  the FP32 pre-filter on the 2^22 boxes (first-success order);
+  ranker       the learned criterion ranker trained on 2^20 boxes of another
+               seed, ranked vs plain first-success evaluation of the 2^22 boxes.
 """
 from __future__ import annotations
 
@@ -52,6 +63,70 @@ def ops_per_box(soa: np.ndarray) -> float:
     return float(ops.sum()) / soa.shape[1]
 
 
+def extended(p, soa: np.ndarray, mask: np.ndarray) -> dict:
+    """The other configurations, each timed once after a warm-up run."""
+    import circlecover_amd as cc
+
+    out = {}
+    # rectangle_size_bound branch-and-bound: plain vs batched
+    cells = cc.rsb_grid(1.0, 8.0, 256, 64, 64)
+    p.rsb_bnb_batched(cells, 24)  # warm-up, graph capture
+    t = time.perf_counter()
+    ub, sb = p.rsb_bnb_batched(cells, 24)
+    t_b = time.perf_counter() - t
+    t = time.perf_counter()
+    up, sp = p.rsb_bnb(cells, 24)
+    t_p = time.perf_counter() - t
+    same = (sorted(map(tuple, up)) == sorted(map(tuple, ub)) and list(sp.verdict_hist) == list(sb.verdict_hist))
+    out["rsb_bnb"] = {"cells": int(cells.shape[0]), "max_depth": 24, "plain_s": t_p, "batched_s": t_b,
+                      "speedup": t_p / t_b, "unproven": int(ub.shape[0]), "identical": bool(same),
+                      "cells_evaluated": int(sum(sb.cells_per_depth[: sb.depth_reached + 1])),
+                      "cell_evals": int(sb.cell_evals), "cell_evals_per_s_batched": sb.cell_evals / t_b,
+                      "cell_evals_per_s_plain": sp.cell_evals / t_p, "trap_cells": int(sb.trap_cells)}
+    # tight_rectangle small branch-and-bound
+    tc = cc.tr_initial_cells(1.0, 1.5, 64)
+    p.tr_bnb(tc, (64, 32, 32, 32), 3)
+    t = time.perf_counter()
+    lv, per = p.tr_bnb(tc, (64, 32, 32, 32), 3)
+    out["tr_small"] = {"wall_s": time.perf_counter() - t, "unproven": int(lv.shape[0]),
+                       "unproven_per_level": [int(x) for x in per]}
+    # FP32 pre-filter vs plain FP64 first-success
+    p.eval_first(soa)
+    f64_first, _, _, st64 = p.eval_first(soa)
+    first, evals, trap, dec, w, st32 = p.eval_first_f32(soa, with_widths=True)
+    bad = int(((dec == 1) & ((mask & 1) == 0)).sum())
+    r_cr = w[:, 0] / np.maximum(w[:, 1], 1e-300)
+    r_rem = w[:, 2] / np.maximum(w[:, 3], 1e-300)
+    out["fp32_prefilter"] = {"fraction_decided_fp32": float(dec.mean()), "fp32_accept_fp64_reject": bad,
+                             "identical_first": bool((first == f64_first).all()),
+                             "eval_ms_fp32_then_fp64": st32.eval_ms, "eval_ms_fp64": st64.eval_ms,
+                             "width_inflation": {
+                                 "r1_strategies.critical_ratio": [float(np.median(r_cr)),
+                                                                  float(np.percentile(r_cr, 99))],
+                                 "r1_strategies.remaining_weight": [float(np.median(r_rem)),
+                                                                    float(np.percentile(r_rem, 99))]}}
+    # learned ranker
+    train = cc.gen_c2_boxes(SEED + 1, 1 << 20)
+    tf, _, _, _ = p.eval_first(train)  # labels: first proving criterion in plain order
+    t = time.perf_counter()
+    model, losses = p.ranker_train(train, tf, epochs=20, batch=4096, seed=1)
+    t_train = time.perf_counter() - t
+    p.eval_first(soa, model)
+    t = time.perf_counter()
+    rf, re, _, sr = p.eval_first(soa, model)
+    t_r = time.perf_counter() - t
+    t = time.perf_counter()
+    pf, pe, _, sp1 = p.eval_first(soa)
+    t_pl = time.perf_counter() - t
+    out["ranker"] = {"train_s": t_train, "final_loss": float(losses[-1]), "train_boxes": int((tf < 10).sum()),
+                     "evals_per_proven_plain": sp1.evals_proven / sp1.proven,
+                     "evals_per_proven_ranked": sr.evals_proven / sr.proven,
+                     "device_ms_plain": sp1.eval_ms, "device_ms_ranked": sr.rank_ms + sr.eval_ms,
+                     "rank_ms": sr.rank_ms, "wall_s_ranked": t_r, "wall_s_plain": t_pl,
+                     "same_proven_set": bool(((rf < 10) == (pf < 10)).all())}
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -59,10 +134,11 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--boxes", type=int, default=N_BOXES)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-oracle baseline leg")
+    ap.add_argument("--core-only", action="store_true", help="skip the other configurations (workloads)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
-              "(the workload shards per GPU as described in DESIGN.md)", file=sys.stderr)
+              "(cells are independent: each GPU would take a slice of the boxes or initial cells)", file=sys.stderr)
         return 2
 
     from circlecover_amd import Prover, gen_c2_boxes
@@ -87,6 +163,7 @@ def main() -> int:
         mask = d_mask.download(np.empty(n, dtype=np.uint32))
         for b in (d_soa, d_mask, d_first):
             b.free()
+        workloads = None if args.core_only else extended(p, soa, mask)
 
     ms_per_step = 1e3 * wall / args.steps
     value = n * N_CRIT / (wall / args.steps)
@@ -117,6 +194,8 @@ def main() -> int:
                      "ops_per_box": opb, "ops_per_launch": ops_launch, "launch_ms": launch_ms,
                      "kernel": "ccrd_c2_eval"},
     }
+    if workloads is not None:
+        out["workloads"] = workloads
     if not args.no_cpu:
         threads = len(os.sched_getaffinity(0))
         threads = max(1, min(threads, 16))

@@ -5,6 +5,7 @@
 // a wave's boxes come from one bucket (except at the few bucket borders),
 // the wave runs one criterion at a time instead of a different one per lane.
 // With every box in bucket 0 this is the plain order.
+#include "dev/f32.hpp"
 #include "dev/rsb.hpp"
 
 using namespace ccd;
@@ -61,5 +62,66 @@ extern "C" __global__ void __launch_bounds__(256) ccrd_c2_first(const double* __
   }
   first[i] = (uint8_t)found;
   evals[i] = (uint8_t)count;
+  trap[i] = (uint8_t)tp.flag;
+}
+
+// FP32 pre-filter (plain order): each box is converted outward to FP32 and
+// criterion 0 (r1_strategies, which proves most boxes first) is tried in
+// FP32 intervals; a box it proves is decided there (first = 0, f32[i] = 1).
+// Every other box is evaluated in FP64 in plain order as ccrd_c2_first
+// does.  widths (may be null) receives, per box, the widths of
+// critical_ratio(lambda) and of required_weight_for(lambda) - r1 in FP32 and
+// in FP64 (4 floats: cr32, cr64, rem32, rem64) for the inflation report.
+extern "C" __global__ void __launch_bounds__(256) ccrd_c2_first_f32(const double* __restrict__ soa, uint64_t ld,
+                                                                     uint64_t n, const Konst* __restrict__ kp,
+                                                                     uint8_t* __restrict__ first,
+                                                                     uint8_t* __restrict__ evals,
+                                                                     uint8_t* __restrict__ trap,
+                                                                     uint8_t* __restrict__ f32_decided,
+                                                                     float* __restrict__ widths) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const Konst K = *kp;
+  const iv la{soa[0 * ld + i], soa[1 * ld + i]};
+  const iv r1{soa[2 * ld + i], soa[3 * ld + i]};
+  const iv r2{soa[4 * ld + i], soa[5 * ld + i]};
+  const iv r3{soa[6 * ld + i], soa[7 * ld + i]};
+  const f32::KonstF KF = f32::konst(K);
+  const f32::ivf laf = f32::out(la), r1f = f32::out(r1);
+  const f32::ivf Wf = f32::weight_for(KF, laf);
+  const float ub_r2f = f32::from_ru(r2.hi);
+  if (widths) {
+    const f32::ivf crf = f32::crit_ratio(KF, laf);
+    const iv cr = trc::crit_ratio(K, la);
+    const iv W = trc::weight_for(K, la);
+    const f32::ivf remf = Wf - r1f;
+    const iv rem = W - r1;
+    widths[4 * i + 0] = f32::add_ru(crf.hi, -crf.lo);
+    widths[4 * i + 1] = (float)(cr.hi - cr.lo);
+    widths[4 * i + 2] = f32::add_ru(remf.hi, -remf.lo);
+    widths[4 * i + 3] = (float)(rem.hi - rem.lo);
+  }
+  if (f32::r1_strategies(KF, laf, Wf, r1f, ub_r2f)) {
+    first[i] = 0;
+    evals[i] = 1;
+    trap[i] = 0;
+    f32_decided[i] = 1;
+    return;
+  }
+  f32_decided[i] = 0;
+  const iv W = trc::weight_for(K, la);
+  const double rem4 = (W - r1 - r2 - r3).hi;
+  const double ub_r4 = r3.hi < rem4 ? r3.hi : rem4;
+  Trap tp{0};
+  int found = 255, count = 1;  // the FP32 attempt counts as one evaluation
+  for (int k = 0; k < 10; ++k) {
+    ++count;
+    if (c2_criterion(k, K, la, W, r1, r2, r3, ub_r4, tp)) {
+      found = k;
+      break;
+    }
+  }
+  first[i] = (uint8_t)found;
+  evals[i] = (uint8_t)(count > 255 ? 255 : count);
   trap[i] = (uint8_t)tp.flag;
 }

@@ -325,4 +325,51 @@ int cc_c2_eval_first(cc_ctx* ctx, const double* soa, size_t n, size_t ld, const 
   return CC_OK;
 }
 
+int cc_c2_eval_first_f32(cc_ctx* ctx, const double* soa, size_t n, size_t ld, uint8_t* first, uint8_t* evals,
+                         uint8_t* trap, uint8_t* f32_decided, float* widths, cc_first_stats* stats) {
+  if (!ctx || !soa || !first || !evals || !trap || !f32_decided || ld < n) return CC_E_ARG;
+  CC_HIP(ctx, hipSetDevice(ctx->device));
+  MlState& M = ml(ctx);
+  int rc = upload_soa(ctx, M.soa, soa, n, ld);
+  if (rc) return rc;
+  CC_HIP(ctx, M.first.ensure(n + 1));
+  CC_HIP(ctx, M.evals.ensure(n + 1));
+  CC_HIP(ctx, M.trap.ensure(n + 1));
+  CC_HIP(ctx, M.top1.ensure(n + 1));
+  float* d_w = nullptr;
+  if (widths) {
+    CC_HIP(ctx, M.grad.ensure(4 * n * sizeof(float) + 16));
+    d_w = static_cast<float*>(M.grad.p);
+  }
+  hipFunction_t f;
+  CC_HIP(ctx, kernel(ctx, "ccrd_c2_first_f32", &f));
+  CC_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+  if (n) {
+    const uint64_t n64 = n, ld_dev = n;
+    CC_HIP(ctx, launch(f, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, M.soa.p, ld_dev, n64,
+                       ctx->d_konst, M.first.p, M.evals.p, M.trap.p, M.top1.p, d_w));
+  }
+  CC_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+  CC_HIP(ctx, hipMemcpyAsync(first, M.first.p, n, hipMemcpyDeviceToHost, ctx->stream));
+  CC_HIP(ctx, hipMemcpyAsync(evals, M.evals.p, n, hipMemcpyDeviceToHost, ctx->stream));
+  CC_HIP(ctx, hipMemcpyAsync(trap, M.trap.p, n, hipMemcpyDeviceToHost, ctx->stream));
+  CC_HIP(ctx, hipMemcpyAsync(f32_decided, M.top1.p, n, hipMemcpyDeviceToHost, ctx->stream));
+  if (widths) CC_HIP(ctx, hipMemcpyAsync(widths, d_w, 4 * n * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+  CC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  if (stats) {
+    float ms = 0;
+    CC_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    std::memset(stats, 0, sizeof *stats);
+    stats->eval_ms = ms;
+    for (size_t i = 0; i < n; ++i) {
+      stats->criterion_evals += evals[i];
+      if (first[i] < kOut) {
+        stats->proven += 1;
+        stats->evals_proven += evals[i];
+      }
+    }
+  }
+  return CC_OK;
+}
+
 }  // extern "C"

@@ -228,10 +228,12 @@ int cc_interrupt(cc_ctx* ctx) {
   return CC_OK;
 }
 
-int cc_tr_prove(cc_ctx* ctx, int la_init, int generations, FILE* out) {
+int cc_tr_prove(cc_ctx* ctx, int la_init, int la_first, int la_count, int generations, FILE* out) {
   if (!ctx) return CC_E_ARG;
   static const int kSubdivSchedule[11] = {1, 2, 2, 2, 2, 1, 2, 1, 2, 1, 2};  // tight_rectangle.cu:50-53
   if (la_init <= 0) la_init = 256;
+  if (la_first < 0 || la_first >= la_init) la_first = 0;
+  if (la_count <= 0 || la_first + la_count > la_init) la_count = la_init - la_first;
   if (generations < 0 || generations > 11) generations = 11;
   std::ostringstream o;
   std::ostream& err = std::cerr;
@@ -239,7 +241,7 @@ int cc_tr_prove(cc_ctx* ctx, int la_init, int generations, FILE* out) {
   const cc_tr_cell initial{{1.0, 2.0898841580413818342}, {{0.0, DBL_MAX}, {0.0, DBL_MAX}, {0.0, DBL_MAX}, {0.0, DBL_MAX}}};
   std::vector<cc_tr_cell> current, next, launches;
   err << "Initializing..." << std::endl;
-  for (int a = 0; a < la_init; ++a) {
+  for (int a = la_first; a < la_first + la_count; ++a) {
     cc_tr_cell s = initial;
     s.la = piece(initial.la, a, la_init);
     launches.push_back(s);
@@ -330,10 +332,13 @@ int cc_tr_prove(cc_ctx* ctx, int la_init, int generations, FILE* out) {
   return CC_OK;
 }
 
-int cc_rsb_prove(cc_ctx* ctx, int la_init, int r1_init, int r2_init, int generations, FILE* out) {
+int cc_rsb_prove(cc_ctx* ctx, int la_init, int la_first, int la_count, int r1_init, int r2_init, int generations,
+                 FILE* out) {
   if (!ctx) return CC_E_ARG;
   // rectangle_size_bound.cu:36-62 and values.hpp:28-35
   if (la_init <= 0) la_init = 256;
+  if (la_first < 0 || la_first >= la_init) la_first = 0;
+  if (la_count <= 0 || la_first + la_count > la_init) la_count = la_init - la_first;
   if (r1_init <= 0) r1_init = 128;
   if (r2_init <= 0) r2_init = 128;
   if (generations < 0) generations = 1;
@@ -344,11 +349,21 @@ int cc_rsb_prove(cc_ctx* ctx, int la_init, int r1_init, int r2_init, int generat
   begin_case(o, "Critical intervals for rectangles with small disks");
   err << "Size bound: r <= " << 3.0 / 8.0 << ", critical ratio: " << 0.61 << std::endl;
   const cc_rsb_cell initial{{1.0, 2.5}, {{0.0, w}, {0.0, w}, {0.0, w}, {0.0, w}, {0.0, w}, {0.0, w}}};
-  auto run = [&](const cc_rsb_cell& outer, int nla, int nr1, int nr2, const int* counts,
+  // find_critical_intervals_in (rectangle_size_bound.cu:276-310) over the
+  // lambda pieces [first, first + count) of `outer`
+  auto run = [&](const cc_rsb_cell& outer, int nla, int first, int count, int nr1, int nr2, const int* counts,
                  std::vector<cc_rsb_cell>& append_to) -> int {
-    std::vector<cc_rsb_cell> cells((size_t)nla * nr1 * nr2);
-    const size_t m = cc_rsb_grid(&outer, nla, nr1, nr2, cells.data(), cells.size());
-    cells.resize(m);
+    std::vector<cc_rsb_cell> cells;
+    for (int a = first; a < first + count; ++a) {
+      cc_rsb_cell sub = outer;
+      sub.la = piece(outer.la, a, nla);
+      std::vector<cc_rsb_cell> part((size_t)nr1 * nr2);
+      // one lambda piece: the grid's lambda loop with a single piece equal
+      // to `sub.la` (from_la_r1_r2 intersects it with itself)
+      const size_t m = cc_rsb_grid(&sub, 1, nr1, nr2, part.data(), part.size());
+      cells.insert(cells.end(), part.begin(), part.begin() + m);
+    }
+    const size_t m = cells.size();
     const size_t chunk = 4096;
     for (size_t b = 0; b < m; b += chunk) {
       if (g_interrupted || ctx->interrupt) return CC_E_INTERRUPTED;
@@ -369,7 +384,7 @@ int cc_rsb_prove(cc_ctx* ctx, int la_init, int r1_init, int r2_init, int generat
   };
   std::vector<cc_rsb_cell> current, next;
   err << "Initializing..." << std::endl;
-  int rc = run(initial, la_init, r1_init, r2_init, kInit, next);
+  int rc = run(initial, la_init, la_first, la_count, r1_init, r2_init, kInit, next);
   if (rc) return rc;
   current = merge_critical(next, 0);
   next.clear();
@@ -386,7 +401,7 @@ int cc_rsb_prove(cc_ctx* ctx, int la_init, int r1_init, int r2_init, int generat
       err << "Running generation " << g << " on " << current.size() << " intervals..." << std::endl;
       bool stop = false;
       for (const cc_rsb_cell& s : current) {
-        rc = run(s, 64, 32, 32, kSub, next);  // rectangle_size_bound.cu:50-52
+        rc = run(s, 64, 0, 64, 32, 32, kSub, next);  // rectangle_size_bound.cu:50-52
         if (rc == CC_E_INTERRUPTED) {
           stop = true;
           break;
@@ -434,13 +449,13 @@ namespace circlecover {
 namespace tight_rectangle {
 __attribute__((visibility("default"))) void find_critical_intervals() {
   Ctx c;
-  check(c.p, cc_tr_prove(c.p, 0, -1, stdout));
+  check(c.p, cc_tr_prove(c.p, 0, 0, 0, -1, stdout));
 }
 }  // namespace tight_rectangle
 namespace rectangle_size_bound {
 __attribute__((visibility("default"))) void find_critical_intervals() {
   Ctx c;
-  check(c.p, cc_rsb_prove(c.p, 0, 0, 0, -1, stdout));
+  check(c.p, cc_rsb_prove(c.p, 0, 0, 0, 0, 0, -1, stdout));
 }
 }  // namespace rectangle_size_bound
 }  // namespace circlecover

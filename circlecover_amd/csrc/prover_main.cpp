@@ -4,6 +4,8 @@
 // constants.  Optional flags shorten a run of the same procedure:
 //   --only tr|rsb        run one of the two proofs
 //   --tr-la N            initial lambda launches of tight_rectangle (256)
+//   --tr-la-range F,C    run only the lambda pieces F..F+C-1
+//   --rsb-la-range F,C   the same for rectangle_size_bound
 //   --tr-generations G   refinement generations of tight_rectangle (11)
 //   --rsb-grid A,B,C     initial lambda x r1 x r2 grid (256,128,128)
 //   --rsb-generations G  refinement generations of rectangle_size_bound (1)
@@ -18,6 +20,7 @@
 int main(int argc, const char* argv[]) {
   bool tr = true, rsb = true, custom = false;
   int tr_la = 0, tr_gen = -1, rsb_la = 0, rsb_r1 = 0, rsb_r2 = 0, rsb_gen = -1;
+  int tr_first = 0, tr_count = 0, rsb_first = 0, rsb_count = 0;
   for (int i = 1; i < argc; ++i) {
     const char* a = argv[i];
     const char* v = i + 1 < argc ? argv[i + 1] : nullptr;
@@ -27,6 +30,12 @@ int main(int argc, const char* argv[]) {
       ++i;
     } else if (!std::strcmp(a, "--tr-la") && v) {
       tr_la = std::atoi(v), custom = true, ++i;
+    } else if (!std::strcmp(a, "--tr-la-range") && v) {
+      if (std::sscanf(v, "%d,%d", &tr_first, &tr_count) != 2) return 2;
+      custom = true, ++i;
+    } else if (!std::strcmp(a, "--rsb-la-range") && v) {
+      if (std::sscanf(v, "%d,%d", &rsb_first, &rsb_count) != 2) return 2;
+      custom = true, ++i;
     } else if (!std::strcmp(a, "--tr-generations") && v) {
       tr_gen = std::atoi(v), custom = true, ++i;
     } else if (!std::strcmp(a, "--rsb-grid") && v) {
@@ -53,8 +62,8 @@ int main(int argc, const char* argv[]) {
     return 1;
   }
   int rc = CC_OK;
-  if (tr) rc = cc_tr_prove(ctx, tr_la, tr_gen, stdout);
-  if (rc == CC_OK && rsb) rc = cc_rsb_prove(ctx, rsb_la, rsb_r1, rsb_r2, rsb_gen, stdout);
+  if (tr) rc = cc_tr_prove(ctx, tr_la, tr_first, tr_count, tr_gen, stdout);
+  if (rc == CC_OK && rsb) rc = cc_rsb_prove(ctx, rsb_la, rsb_first, rsb_count, rsb_r1, rsb_r2, rsb_gen, stdout);
   if (rc != CC_OK && rc != CC_E_INTERRUPTED) std::cerr << "error " << rc << ": " << cc_last_error(ctx) << std::endl;
   cc_destroy(ctx);
   return rc == CC_OK || rc == CC_E_INTERRUPTED ? 0 : 1;

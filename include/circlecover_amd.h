@@ -257,23 +257,38 @@ int cc_c2_eval_first(cc_ctx* ctx, const double* soa, size_t n, size_t ld, const 
 
 /* The tight_rectangle proof of Prover/tight_rectangle/tight_rectangle.cu:
  * 458-537: la_init initial lambda launches over [1, 2.0898841580413818342]
- * (<= 0: the reference's 256), then `generations` refinement generations of
+ * (<= 0: the reference's 256), of which the pieces [la_first, la_first +
+ * la_count) are run (la_count <= 0: all), then `generations` refinement generations of
  * the reference schedule {1,2,2,2,2,1,2,1,2,1,2} (< 0: all 11), each launch's
  * leaves merged by merge_adjacent_r1, and the Max_distances report.  Writes
  * the reference's stdout report to `out` (NULL: stdout), progress to
  * stderr.  SIGINT during the generations ends the run with the last
  * complete generation, as the reference does. */
-int cc_tr_prove(cc_ctx* ctx, int la_init, int generations, FILE* out);
+int cc_tr_prove(cc_ctx* ctx, int la_init, int la_first, int la_count, int generations, FILE* out);
 /* The rectangle_size_bound proof of rectangle_size_bound.cu:312-382:
- * initial grid la_init x r1_init x r2_init (<= 0: 256, 128, 128), merge<0>,
+ * initial grid la_init x r1_init x r2_init (<= 0: 256, 128, 128) over the
+ * lambda pieces [la_first, la_first + la_count) (the reference's
+ * start_from_lambda_offset, :276; la_count <= 0: to the end), merge<0>,
  * `generations` refinement generations (< 0: the reference's 1) with the
  * 64x32x32 host grid per critical cell and merge<2>/<1>/<3>, then one
  * "Critical: " line per remaining cell. */
-int cc_rsb_prove(cc_ctx* ctx, int la_init, int r1_init, int r2_init, int generations, FILE* out);
+int cc_rsb_prove(cc_ctx* ctx, int la_init, int la_first, int la_count, int r1_init, int r2_init, int generations,
+                 FILE* out);
 /* Asks a running proof or branch-and-bound to stop at its next level
  * boundary (returns CC_E_INTERRUPTED there; reference: was_interrupted,
  * Prover/interrupt.cpp:30-46). */
 int cc_interrupt(cc_ctx* ctx);
+
+/* FP32 pre-filter in plain first-success order: each box is converted
+ * outward to FP32 and criterion 0 (r1_strategies, the usual first proof) is
+ * tried in outward-rounded FP32 intervals; a box it proves is decided there
+ * (first = 0, f32_decided = 1), every other box is evaluated in FP64 as
+ * cc_c2_eval_first does (the FP32 attempt counts as one evaluation).
+ * widths (4*n floats, may be NULL): per box the widths of
+ * critical_ratio(lambda) and of required_weight_for(lambda) - r1 in FP32 and
+ * in FP64 (cr32, cr64, rem32, rem64). */
+int cc_c2_eval_first_f32(cc_ctx* ctx, const double* soa, size_t n, size_t ld, uint8_t* first, uint8_t* evals,
+                         uint8_t* trap, uint8_t* f32_decided, float* widths, cc_first_stats* stats);
 
 /* get_subinterval (Prover/subintervals.hpp:32-43) on the host: the n pieces
  * of [lb, ub] as 2n doubles (lb + diff*i with one rounding, the last piece

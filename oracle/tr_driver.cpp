@@ -179,3 +179,109 @@ size_t or_tr_merge_adjacent_r1(const double* cells, size_t n, double* out) {
 }
 
 }  // extern "C"
+
+// kernel_compute_distance_from_theoretical_worst_cases
+// (tight_rectangle.cu:343-456), sequential over the cells in order.  Output
+// record (same layout as the library's report block): ints have_2d, have_3d;
+// then IV la_2d, la_3d, d_r1_2d, 2 cells, d_r2_2d, 2 cells, doubles lb_h1_2d,
+// ub_la_by_S1, ub_R3_2d, IV d_r1_3d, 2 cells, d_r2_3d, 2 cells, d_r3_3d,
+// 2 cells, double ub_R4_3d, IV S1, h2, S3, double pocket_3d, int trap.
+namespace {
+struct MaxDist {
+  int have_2d, have_3d;
+  ccor::IV la_2d, la_3d;
+  ccor::IV d_r1_2d;
+  ccor::tr::Cell d_r1_2d_crit[2];
+  ccor::IV d_r2_2d;
+  ccor::tr::Cell d_r2_2d_crit[2];
+  double lb_h1_2d, ub_la_by_S1, ub_R3_2d;
+  ccor::IV d_r1_3d;
+  ccor::tr::Cell d_r1_3d_crit[2];
+  ccor::IV d_r2_3d;
+  ccor::tr::Cell d_r2_3d_crit[2];
+  ccor::IV d_r3_3d;
+  ccor::tr::Cell d_r3_3d_crit[2];
+  double ub_R4_3d;
+  ccor::IV S1, h2, S3;
+  double pocket_3d;
+  int trap;
+};
+}  // namespace
+
+extern "C" size_t or_tr_max_distances_size() { return sizeof(MaxDist); }
+
+extern "C" void or_tr_max_distances(const double* cells, size_t n, void* out) {
+  using namespace ccor;
+  MaxDist r;
+  std::memset(&r, 0, sizeof r);
+  g_trap = 0;
+  r.la_3d = IV::pt(1.0);
+  r.la_2d = IV::pt(2.0);
+  r.d_r1_2d = r.d_r2_2d = IV::pt(0.0);
+  r.d_r1_3d = r.d_r2_3d = r.d_r3_3d = IV::pt(0.0);
+  r.pocket_3d = -1.0;
+  r.lb_h1_2d = 1.0;
+  r.ub_la_by_S1 = 1.0;
+  for (size_t i = 0; i < n; ++i) {
+    tr::Cell v;
+    std::memcpy(&v, cells + 10 * i, sizeof v);
+    if (v.r[2].ub < 0.1) {
+      r.have_2d = 1;
+      r.la_2d.do_join(v.la);
+      IV r1t = 0.25 * (iv_square(v.la) + 1.0);
+      IV d1 = v.r[0] - r1t;
+      if (d1.lb < r.d_r1_2d.lb) r.d_r1_2d_crit[0] = v;
+      if (d1.ub > r.d_r1_2d.ub) r.d_r1_2d_crit[1] = v;
+      r.d_r1_2d.do_join(d1);
+      IV d2 = v.r[1] - IV::pt(0.25);
+      if (d2.lb < r.d_r2_2d.lb) r.d_r2_2d_crit[0] = v;
+      if (d2.ub > r.d_r2_2d.ub) r.d_r2_2d_crit[1] = v;
+      r.d_r2_2d.do_join(d2);
+      IV R3 = tr::required_weight_for(v.la) - v.r[0] - v.r[1];
+      if (R3.ub > r.ub_R3_2d) r.ub_R3_2d = R3.ub;
+      double S1 = mul_rd(2.0, sqrt_rd(add_rd(v.r[0].lb, -0.25)));
+      double q = div_ru(v.la.ub, S1);
+      if (q > r.ub_la_by_S1) r.ub_la_by_S1 = q;
+      double half = add_ru(v.la.ub, -mul_rd(0.5, S1));
+      double h1 = mul_rd(2.0, sqrt_rd(add_rd(v.r[0].lb, -mul_ru(half, half))));
+      if (h1 < r.lb_h1_2d) r.lb_h1_2d = h1;
+    } else {
+      r.have_3d = 1;
+      r.la_3d.do_join(v.la);
+      IV la2 = iv_square(v.la);
+      IV la4 = iv_square(la2);
+      IV th = (16.0 * la4 + 40.0 * la2 + 9.0) / (256.0 * la2);
+      IV d1 = v.r[0] - th;
+      if (d1.lb < r.d_r1_3d.lb) r.d_r1_3d_crit[0] = v;
+      if (d1.ub > r.d_r1_3d.ub) r.d_r1_3d_crit[1] = v;
+      IV d2 = v.r[1] - th;
+      if (d2.lb < r.d_r2_3d.lb) r.d_r2_3d_crit[0] = v;
+      if (d2.ub > r.d_r2_3d.ub) r.d_r2_3d_crit[1] = v;
+      IV d3 = v.r[2] - th;
+      if (d3.lb < r.d_r3_3d.lb) r.d_r3_3d_crit[0] = v;
+      if (d3.ub > r.d_r3_3d.ub) r.d_r3_3d_crit[1] = v;
+      r.d_r1_3d.do_join(d1);
+      r.d_r2_3d.do_join(d2);
+      r.d_r3_3d.do_join(d3);
+      IV R4 = tr::required_weight_for(v.la) - v.r[0] - v.r[1] - v.r[2];
+      if (R4.ub > r.ub_R4_3d) r.ub_R4_3d = R4.ub;
+      IV S1 = iv_sqrt(4.0 * v.r[0] - 1.0);
+      IV h2 = iv_sqrt(4.0 * v.r[1] - iv_square(v.la - S1));
+      IV S3 = iv_sqrt(4.0 * v.r[2] - iv_square(1.0 - h2));
+      double pw = (v.la - S1 - S3).ub;
+      if (pw > r.pocket_3d) {
+        if (r.pocket_3d < 0.0) {
+          r.S1 = S1;
+          r.h2 = h2;
+          r.S3 = S3;
+        }
+        r.pocket_3d = pw;
+      }
+      r.S1.do_join(S1);
+      r.h2.do_join(h2);
+      r.S3.do_join(S3);
+    }
+  }
+  r.trap = g_trap;
+  std::memcpy(out, &r, sizeof r);
+}
