@@ -44,6 +44,16 @@ FP64_PEAK_TFLOPS = 78.6
 METRIC = "full-proof wall time & interval cell-evals/s: learned+batched vs plain GPU"
 
 
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(soa: np.ndarray, threads: int) -> dict:
     import oracle
 
@@ -51,9 +61,16 @@ def cpu_baseline(soa: np.ndarray, threads: int) -> dict:
     t0 = time.perf_counter()
     oracle.c2_eval(soa, threads=threads)
     dt = time.perf_counter() - t0
+    one = soa[:, : 1 << 16].copy()
+    t0 = time.perf_counter()
+    oracle.c2_eval(one, threads=1)
+    dt1 = time.perf_counter() - t0
     return {"value": n * N_CRIT / dt, "unit": "cell-evals/s", "cores": threads, "kind": "port",
             "sample": f"first {n} of the {N_BOXES} seeded boxes, all ten criteria, CPU oracle "
-                      f"(x86 MXCSR directed rounding), {threads} threads, {dt:.2f} s"}
+                      f"(x86 MXCSR directed rounding), {threads} threads, {dt:.2f} s",
+            "single_thread_value": one.shape[1] * N_CRIT / dt1,
+            "single_thread_sample": f"first {one.shape[1]} boxes, 1 thread, {dt1:.2f} s",
+            "cpu_model": cpu_model()}
 
 
 def ops_per_box(soa: np.ndarray) -> float:
@@ -61,6 +78,69 @@ def ops_per_box(soa: np.ndarray) -> float:
 
     ops = oracle.c2_count_ops(soa)
     return float(ops.sum()) / soa.shape[1]
+
+
+COPY_BYTES = 256 << 20  # calibration copy: 256 MiB read + 256 MiB written
+
+
+def traffic_probe() -> int:
+    """One calibration copy and one batch launch, for the HBM counter runs."""
+    from circlecover_amd import Prover, gen_c2_boxes
+
+    n = N_BOXES
+    soa = gen_c2_boxes(SEED, n)
+    with Prover(0) as p:
+        a, b = p.alloc(COPY_BYTES), p.alloc(COPY_BYTES)
+        d_soa, d_mask, d_first = p.alloc(soa.nbytes), p.alloc(4 * n), p.alloc(n)
+        d_soa.upload(soa)
+        p.copy_async(b, a, COPY_BYTES)
+        p.eval_batch_async(d_soa, n, n, d_mask, d_first)
+        p.synchronize()
+        for x in (a, b, d_soa, d_mask, d_first):
+            x.free()
+    return 0
+
+
+def measured_traffic() -> dict | None:
+    """HBM bytes of one batch launch from two rocprofv3 counter runs
+    (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch), each scaled by the
+    calibration copy's reading (expected COPY_BYTES read / written).  None
+    when rocprofv3 is unavailable or a run fails."""
+    import csv
+    import shutil
+    import subprocess
+    import tempfile
+
+    exe = shutil.which("rocprofv3")
+    if exe is None:
+        return None
+    got = {}
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        with tempfile.TemporaryDirectory(dir="/tmp") as d:
+            cmd = [exe, "--pmc", counter, "--output-format", "csv", "-d", d, "-o", "t", "--",
+                   sys.executable, os.path.abspath(__file__), "--traffic-probe"]
+            try:
+                subprocess.run(cmd, cwd="/tmp", capture_output=True, timeout=120, check=True,
+                               env=dict(os.environ, TMPDIR="/tmp"))
+            except (subprocess.SubprocessError, OSError):
+                return None
+            rows = []
+            for root, _, files in os.walk(d):
+                for f in files:
+                    if f.endswith("counter_collection.csv"):
+                        rows += list(csv.DictReader(open(os.path.join(root, f))))
+            vals = {}
+            for r in rows:
+                name = r.get("Kernel_Name", "")
+                key = "copy" if "cc_copy_u4" in name else "c2" if "ccrd_c2_eval" in name else None
+                if key and r.get("Counter_Name", "") == counter:
+                    vals[key] = vals.get(key, 0.0) + float(r["Counter_Value"])
+            if "copy" not in vals or "c2" not in vals or vals["copy"] <= 0:
+                return None
+            scale = COPY_BYTES / vals["copy"]
+            got[counter] = {"kib": vals["c2"], "copy_kib": vals["copy"], "scale": scale, "bytes": vals["c2"] * scale}
+    total = got["FETCH_SIZE"]["bytes"] + got["WRITE_SIZE"]["bytes"]
+    return {"bytes": total, "fetch": got["FETCH_SIZE"], "write": got["WRITE_SIZE"]}
 
 
 def extended(p, soa: np.ndarray, mask: np.ndarray) -> dict:
@@ -135,7 +215,11 @@ def main() -> int:
     ap.add_argument("--boxes", type=int, default=N_BOXES)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-oracle baseline leg")
     ap.add_argument("--core-only", action="store_true", help="skip the other configurations (workloads)")
+    ap.add_argument("--no-traffic", action="store_true", help="skip the HBM counter runs (traffic = null)")
+    ap.add_argument("--traffic-probe", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
+    if args.traffic_probe:
+        return traffic_probe()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
               "(cells are independent: each GPU would take a slice of the boxes or initial cells)", file=sys.stderr)
@@ -169,6 +253,8 @@ def main() -> int:
     value = n * N_CRIT / (wall / args.steps)
     launch_ms = ev_ms / args.steps
 
+    traffic = None if args.no_traffic else measured_traffic()
+
     sample = soa[:, : min(n, 1 << 18)]
     opb = ops_per_box(sample)
     ops_launch = opb * n
@@ -190,7 +276,9 @@ def main() -> int:
         "config": {"workload": "batched-criteria-4d", "boxes": n, "criteria": N_CRIT, "seed": hex(SEED),
                    "precision": "fp64-outward", "proven_fraction": float(((mask & 0x3FF) != 0).mean())},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                     "frac": achieved / FP64_PEAK_TFLOPS,
+                     "traffic": None if traffic is None else traffic["bytes"],
+                     "algorithmic_bytes": 69.0 * n, "traffic_detail": traffic,
                      "ops_per_box": opb, "ops_per_launch": ops_launch, "launch_ms": launch_ms,
                      "kernel": "ccrd_c2_eval"},
     }

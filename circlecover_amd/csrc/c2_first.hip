@@ -65,13 +65,15 @@ extern "C" __global__ void __launch_bounds__(256) ccrd_c2_first(const double* __
   trap[i] = (uint8_t)tp.flag;
 }
 
-// FP32 pre-filter (plain order): each box is converted outward to FP32 and
-// criterion 0 (r1_strategies, which proves most boxes first) is tried in
-// FP32 intervals; a box it proves is decided there (first = 0, f32[i] = 1).
+// FP32 pre-filter (plain order): criterion 0 (r1_strategies, which proves
+// most boxes first) is run as an FP32 shadow of its FP64 computation (see
+// dev/f32.hpp); a box the shadow proves is one the FP64 criterion proves, so
+// it is decided there (first = 0, f32_decided[i] = 1).
 // Every other box is evaluated in FP64 in plain order as ccrd_c2_first
 // does.  widths (may be null) receives, per box, the widths of
 // critical_ratio(lambda) and of required_weight_for(lambda) - r1 in FP32 and
-// in FP64 (4 floats: cr32, cr64, rem32, rem64) for the inflation report.
+// in FP64 (4 floats: cr32, cr64, rem32, rem64) for the inflation report; the
+// FP32 width is that of the hull of the shadows of both FP64 bounds.
 extern "C" __global__ void __launch_bounds__(256) ccrd_c2_first_f32(const double* __restrict__ soa, uint64_t ld,
                                                                      uint64_t n, const Konst* __restrict__ kp,
                                                                      uint8_t* __restrict__ first,
@@ -87,21 +89,22 @@ extern "C" __global__ void __launch_bounds__(256) ccrd_c2_first_f32(const double
   const iv r2{soa[4 * ld + i], soa[5 * ld + i]};
   const iv r3{soa[6 * ld + i], soa[7 * ld + i]};
   const f32::KonstF KF = f32::konst(K);
-  const f32::ivf laf = f32::out(la), r1f = f32::out(r1);
-  const f32::ivf Wf = f32::weight_for(KF, laf);
-  const float ub_r2f = f32::from_ru(r2.hi);
+  const f32::siv las = f32::sv(la), r1s = f32::sv(r1);
+  f32::Sh st{true};
+  const f32::siv Ws = f32::crit_ratio(KF, las, st) * las;  // required_weight_for(la)
   if (widths) {
-    const f32::ivf crf = f32::crit_ratio(KF, laf);
+    f32::Sh st2{true};
+    const f32::siv crs = f32::crit_ratio(KF, las, st2);
     const iv cr = trc::crit_ratio(K, la);
     const iv W = trc::weight_for(K, la);
-    const f32::ivf remf = Wf - r1f;
+    const f32::siv rems = Ws - r1s;
     const iv rem = W - r1;
-    widths[4 * i + 0] = f32::add_ru(crf.hi, -crf.lo);
+    widths[4 * i + 0] = f32::add_ru(crs.hi.hi, -crs.lo.lo);
     widths[4 * i + 1] = (float)(cr.hi - cr.lo);
-    widths[4 * i + 2] = f32::add_ru(remf.hi, -remf.lo);
+    widths[4 * i + 2] = f32::add_ru(rems.hi.hi, -rems.lo.lo);
     widths[4 * i + 3] = (float)(rem.hi - rem.lo);
   }
-  if (f32::r1_strategies(KF, laf, Wf, r1f, ub_r2f)) {
+  if (st.ok && f32::r1_strategies(KF, las, Ws, r1s, f32::shd(r2.hi), st) && st.ok) {
     first[i] = 0;
     evals[i] = 1;
     trap[i] = 0;

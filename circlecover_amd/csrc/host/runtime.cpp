@@ -138,6 +138,16 @@ int cc_d2h(cc_ctx* ctx, void* dst, const void* src, size_t bytes) {
   CC_HIP(ctx, hipStreamSynchronize(ctx->stream));
   return CC_OK;
 }
+int cc_copy_async(cc_ctx* ctx, void* d_dst, const void* d_src, size_t bytes) {
+  if (!ctx || (bytes && (!d_dst || !d_src)) || bytes % 16) return CC_E_ARG;
+  if (!bytes) return CC_OK;
+  hipFunction_t f;
+  CC_HIP(ctx, kernel(ctx, "cc_copy_u4", &f));
+  const uint64_t n = bytes / 16;
+  CC_HIP(ctx, launch(f, dim3(2048), dim3(256), 0, ctx->stream, d_src, d_dst, n));
+  return CC_OK;
+}
+
 int cc_timer_start(cc_ctx* ctx) {
   if (!ctx) return CC_E_ARG;
   CC_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));

@@ -55,6 +55,7 @@ lib.cc_dev_free.argtypes = [_vp, _vp]
 lib.cc_h2d.argtypes = [_vp, _vp, _vp, _sz]
 lib.cc_d2h.argtypes = [_vp, _vp, _vp, _sz]
 lib.cc_timer_start.argtypes = [_vp]
+lib.cc_copy_async.argtypes = [_vp, _vp, _vp, _sz]
 lib.cc_timer_stop.argtypes = [_vp, _P(ctypes.c_float)]
 lib.cc_eval_batch.argtypes = [_vp, _i, _i, _vp, _sz, _sz, _vp, _vp]
 lib.cc_eval_batch_async.argtypes = [_vp, _i, _i, _vp, _sz, _sz, _vp, _vp]
@@ -217,6 +218,9 @@ class Prover:
 
     def synchronize(self) -> None:
         self._check(lib.cc_synchronize(self.ctx))
+
+    def copy_async(self, dst: "DeviceBuffer", src: "DeviceBuffer", nbytes: int) -> None:
+        self._check(lib.cc_copy_async(self.ctx, dst.ptr, src.ptr, nbytes))
 
     def timer_start(self) -> None:
         self._check(lib.cc_timer_start(self.ctx))

@@ -62,6 +62,9 @@ int cc_dev_alloc(cc_ctx* ctx, size_t bytes, void** dptr);
 int cc_dev_free(cc_ctx* ctx, void* dptr);
 int cc_h2d(cc_ctx* ctx, void* dst, const void* src, size_t bytes);
 int cc_d2h(cc_ctx* ctx, void* dst, const void* src, size_t bytes);
+/* Device-to-device copy with a plain streaming kernel (bytes a multiple of
+ * 16); its known traffic calibrates the HBM counters. */
+int cc_copy_async(cc_ctx* ctx, void* d_dst, const void* d_src, size_t bytes);
 /* Stream-ordered timing: cc_timer_start records an event on the context
  * stream, cc_timer_stop records a second one, waits for it and returns the
  * elapsed milliseconds between the two. */

@@ -59,7 +59,7 @@ def test_gpu_rsb_prove_report_matches_oracle():
 
     outer = np.array([1.0, 2.5] + [0.0, 9.0 / 64.0] * 6)
     sub = outer.copy()
-    sub[0:2] = subintervals(1.0, 2.5, 256)[0]
+    sub[0:2] = subintervals(1.0, 2.5, 256)[40]
     leaves = []
     for c in oracle.rsb_grid(sub, 1, 8, 8):
         lv, trap = oracle.rsb_find_critical(c, (64, 64, 64, 64))
@@ -67,5 +67,15 @@ def test_gpu_rsb_prove_report_matches_oracle():
         leaves.extend(lv)
     assert len(leaves) > 0
     expected = rsb_report(merge_critical(np.array(leaves), 0))
-    got = run_prover(["--only", "rsb", "--rsb-grid", "256,8,8", "--rsb-la-range", "0,1", "--rsb-generations", "0"])
+    got = run_prover(["--only", "rsb", "--rsb-grid", "256,8,8", "--rsb-la-range", "40,1", "--rsb-generations", "0"])
     assert got == expected
+
+
+@pytest.mark.gpu
+def test_gpu_rsb_prove_trap_fails_the_run():
+    """On this coarse grid some leaf of lambda piece 0 reaches a trap site
+    (the oracle agrees); the reference aborts there, the prover exits with an
+    error instead of printing a report."""
+    r = subprocess.run([EXE, "--only", "rsb", "--rsb-grid", "256,8,8", "--rsb-la-range", "0,1",
+                        "--rsb-generations", "0"], capture_output=True, text=True, timeout=600)
+    assert r.returncode == 1 and "error 3" in r.stderr
