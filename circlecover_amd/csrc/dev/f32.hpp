@@ -77,7 +77,7 @@ CCD sf smul(sf a, sf b) {
   float m1 = l1 < l2 ? l1 : l2, m2 = l3 < l4 ? l3 : l4, x1 = u1 > u2 ? u1 : u2, x2 = u3 > u4 ? u3 : u4;
   return sf{m1 < m2 ? m1 : m2, x1 > x2 ? x1 : x2};
 }
-CCD sf sdiv(sf a, sf b, Sh& st) {
+static __device__ __noinline__ sf sdiv(sf a, sf b, Sh& st) {
   if (!(b.lo > 0.f || b.hi < 0.f)) {  // the FP64 divisor's sign is not known
     st.ok = false;
     return sf{0.f, 0.f};
@@ -155,7 +155,7 @@ CCD KonstF konst(const Konst& K) { return KonstF{shd(K.be_lo), shd(K.be_hi)}; }
 
 // critical_ratio(IV la) of tight_rectangle/critical_ratio.cu:58-78, shadowed
 // (la > 0 on this path).
-CCD siv crit_ratio(const KonstF& K, siv la, Sh& st) {
+static __device__ __noinline__ siv crit_ratio(const KonstF& K, siv la, Sh& st) {
   siv a{}, b{};
   const bool lo_side = sle(la.hi, K.be_lo, st);
   const bool hi_side = !lo_side && sge(la.lo, K.be_hi, st);
@@ -171,7 +171,7 @@ CCD siv crit_ratio(const KonstF& K, siv la, Sh& st) {
   return siv{smin(a.lo, b.lo), smax(a.hi, b.hi)};
 }
 // critical_ratio(IV w, IV h): critical_ratio.cu:32-52
-CCD siv crit_ratio(const KonstF& K, siv w, siv h, Sh& st) {
+static __device__ __noinline__ siv crit_ratio(const KonstF& K, siv w, siv h, Sh& st) {
   if (sle(w.hi, h.lo, st)) return crit_ratio(K, div(h, w, st), st);
   if (!st.ok) return w;
   if (sle(h.hi, w.lo, st)) return crit_ratio(K, div(w, h, st), st);
@@ -189,7 +189,8 @@ CCD bool disk_fits(sf r_sq_ub, sf h_lb, Sh& st) {  // disk_satisfies_size_bound.
 }
 
 // can_recurse(ub_w, ub_h, ub_largest, lb_rem): tight_rectangle/can_recurse.cu:32-62
-CCD bool can_recurse(const KonstF& K, sf ub_w, sf ub_h, sf ub_largest, sf lb_rem, Sh& st) {
+static __device__ __noinline__ bool can_recurse(const KonstF& K, sf ub_w, sf ub_h, sf ub_largest, sf lb_rem,
+                                               Sh& st) {
   sf cr = crit_ratio(K, siv{ub_w, ub_w}, siv{ub_h, ub_h}, st).hi;
   const sf area = smul(ub_w, ub_h);
   const sf side = smin(ub_w, ub_h);

@@ -9,7 +9,7 @@ namespace rsbc {
 CCD double required_height(double r_sq_ub) { return div_ru(sqrt_ru(r_sq_ub), kDiskRadiusMax); }  // :30-32
 
 // can_recurse (size-bound version): rectangle_size_bound/can_recurse.cu:26-88
-static __device__ __noinline__ bool can_recurse(const Konst& K, double lb_rem, double ub_largest, double ub_w, double ub_h) {
+CC_RECURSE_FN bool can_recurse(const Konst& K, double lb_rem, double ub_largest, double ub_w, double ub_h) {
   if (lb_rem <= 0.0) {
     if (lb_rem < 0.0) return false;
     return (ub_w <= 0.0) | (ub_h <= 0.0);

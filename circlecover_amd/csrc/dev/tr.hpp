@@ -10,6 +10,19 @@
 #pragma once
 #include "iv.hpp"
 
+// The can_recurse helpers are called from many criteria.  Kept out of line
+// by default (smaller code, faster builds); a translation unit that defines
+// CC_INLINE_RECURSE 1 before including this header inlines them, which
+// removes their call stack (scratch traffic) at the cost of code size.
+#ifndef CC_INLINE_RECURSE
+#define CC_INLINE_RECURSE 0
+#endif
+#if CC_INLINE_RECURSE
+#define CC_RECURSE_FN CCD
+#else
+#define CC_RECURSE_FN static __device__ __noinline__
+#endif
+
 namespace ccd {
 
 static constexpr double kRsbRatio = 0.61;
@@ -71,7 +84,7 @@ CCD iv weight_for(const Konst& K, iv w, iv h) { return crit_ratio(K, w, h) * w *
 CCD iv weight_for(const Konst& K, iv la) { return crit_ratio(K, la) * la; }
 
 // can_recurse(ub_w, ub_h, ub_largest, lb_rem): can_recurse.cu:32-62
-static __device__ __noinline__ bool can_recurse(const Konst& K, double ub_w, double ub_h, double ub_largest, double lb_rem) {
+CC_RECURSE_FN bool can_recurse(const Konst& K, double ub_w, double ub_h, double ub_largest, double lb_rem) {
   double cr = crit_ratio(K, pt(ub_w), pt(ub_h)).hi;
   double area = mul_ru(ub_w, ub_h);
   double side = ub_w < ub_h ? ub_w : ub_h;

@@ -16,8 +16,25 @@ constexpr int kW1 = 0, kB1 = kIn * kH, kW2 = kB1 + kH, kB2 = kW2 + kH * kH, kW3 
 static_assert(kB3 + kOut == CC_RANKER_PARAMS, "parameter count");
 
 struct MlState {
-  DevBuf soa, label, perm, rec, loss, params, m, v, grad, norm, wb, bias, top1, idx, counts, first, evals, trap, bstart;
+  DevBuf soa, label, perm, rec, loss, params, m, v, grad, norm, wb, bias, top1, idx, counts, first, evals, trap, bstart,
+      partial;
 };
+constexpr int kSlice = 64;  // samples per partial gradient sum (ml.hip)
+
+// Gradient of the batch in `rec` (B samples), then Adam if update != 0.
+int grad_step(cc_ctx* ctx, MlState& M, int B, int t, float lr, int update, void* grad_out) {
+  hipFunction_t fp, fa;
+  CC_HIP(ctx, kernel(ctx, "ccml_grad_partial", &fp));
+  CC_HIP(ctx, kernel(ctx, "ccml_grad_adam", &fa));
+  const int slices = (B + kSlice - 1) / kSlice;
+  CC_HIP(ctx, M.partial.ensure((size_t)slices * CC_RANKER_PARAMS * sizeof(float)));
+  CC_HIP(ctx, launch(fp, dim3((CC_RANKER_PARAMS + 255) / 256, (unsigned)slices), dim3(256), 0, ctx->stream, M.rec.p,
+                     B, M.partial.p));
+  const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+  CC_HIP(ctx, launch(fa, dim3((CC_RANKER_PARAMS + 255) / 256), dim3(256), 0, ctx->stream, M.partial.p, slices,
+                     M.params.p, M.m.p, M.v.p, t, lr, b1, b2, eps, update, grad_out));
+  return CC_OK;
+}
 
 MlState& ml(cc_ctx* ctx) {
   Scratch& S = scratch(ctx);
@@ -150,11 +167,9 @@ int cc_ranker_train(cc_ctx* ctx, const double* soa, size_t n, size_t ld, const u
   CC_HIP(ctx, M.rec.ensure((size_t)batch * kRec * sizeof(float)));
   CC_HIP(ctx, M.loss.ensure(lab.size() * sizeof(float)));
   CC_HIP(ctx, M.perm.ensure(lab.size() * sizeof(uint32_t)));
-  hipFunction_t ffb, fga;
+  hipFunction_t ffb;
   CC_HIP(ctx, kernel(ctx, "ccml_fwd_bwd", &ffb));
-  CC_HIP(ctx, kernel(ctx, "ccml_grad_adam", &fga));
   const uint64_t ld_dev = n;
-  const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
   int t = 0;
   std::vector<uint32_t> perm = lab;
   std::vector<float> losses(lab.size());
@@ -175,10 +190,7 @@ int cc_ranker_train(cc_ctx* ctx, const double* soa, size_t n, size_t ld, const u
       float* loss = static_cast<float*>(M.loss.p) + off;
       CC_HIP(ctx, launch(ffb, dim3((unsigned)((B + 255) / 256)), dim3(256), 0, ctx->stream, M.soa.p, ld_dev, sample,
                          M.label.p, B, M.norm.p, M.params.p, M.rec.p, loss));
-      void* no_grad = nullptr;
-      const int update = 1;
-      CC_HIP(ctx, launch(fga, dim3((CC_RANKER_PARAMS + 255) / 256), dim3(256), 0, ctx->stream, M.rec.p, B,
-                         M.params.p, M.m.p, M.v.p, t, lr, b1, b2, eps, update, no_grad));
+      if ((rc = grad_step(ctx, M, B, t, lr, 1, nullptr))) return rc;
     }
     if (epoch_loss) {
       CC_HIP(ctx, hipMemcpyAsync(losses.data(), M.loss.p, losses.size() * sizeof(float), hipMemcpyDeviceToHost,
@@ -210,17 +222,12 @@ int cc_ranker_gradient(cc_ctx* ctx, const double* soa, size_t n, size_t ld, cons
   CC_HIP(ctx, M.rec.ensure((size_t)B * kRec * sizeof(float)));
   CC_HIP(ctx, M.loss.ensure((size_t)B * sizeof(float)));
   CC_HIP(ctx, M.grad.ensure(sizeof model->params));
-  hipFunction_t ffb, fga;
+  hipFunction_t ffb;
   CC_HIP(ctx, kernel(ctx, "ccml_fwd_bwd", &ffb));
-  CC_HIP(ctx, kernel(ctx, "ccml_grad_adam", &fga));
   const uint64_t ld_dev = n;
   CC_HIP(ctx, launch(ffb, dim3((unsigned)((B + 255) / 256)), dim3(256), 0, ctx->stream, M.soa.p, ld_dev, M.perm.p,
                      M.label.p, B, M.norm.p, M.params.p, M.rec.p, M.loss.p));
-  void* none = nullptr;
-  const int t = 1, update = 0;
-  const float z = 0.f;
-  CC_HIP(ctx, launch(fga, dim3((CC_RANKER_PARAMS + 255) / 256), dim3(256), 0, ctx->stream, M.rec.p, B, none, none,
-                     none, t, z, z, z, z, update, M.grad.p));
+  if ((rc = grad_step(ctx, M, B, 1, 0.f, 0, M.grad.p))) return rc;
   CC_HIP(ctx, hipMemcpyAsync(grad, M.grad.p, sizeof model->params, hipMemcpyDeviceToHost, ctx->stream));
   if (loss) CC_HIP(ctx, hipMemcpyAsync(loss, M.loss.p, (size_t)B * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
   CC_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -262,7 +269,6 @@ int cc_c2_eval_first(cc_ctx* ctx, const double* soa, size_t n, size_t ld, const 
   CC_HIP(ctx, M.first.ensure(n + 1));
   CC_HIP(ctx, M.evals.ensure(n + 1));
   CC_HIP(ctx, M.trap.ensure(n + 1));
-  CC_HIP(ctx, M.counts.ensure(2 * 16 * sizeof(unsigned long long)));
   CC_HIP(ctx, M.bstart.ensure(16 * sizeof(uint64_t)));
   hipEvent_t e[3];
   for (auto& x : e) CC_HIP(ctx, hipEventCreate(&x));
@@ -280,13 +286,14 @@ int cc_c2_eval_first(cc_ctx* ctx, const double* soa, size_t n, size_t ld, const 
     CC_HIP(ctx, kernel(ctx, "ccml_bucket_count", &fc));
     CC_HIP(ctx, kernel(ctx, "ccml_bucket_prefix", &fp));
     CC_HIP(ctx, kernel(ctx, "ccml_bucket_scatter", &fs));
+    const uint32_t nblocks = 1024;
+    const uint64_t n64 = n, chunk = (n + nblocks - 1) / nblocks;
+    CC_HIP(ctx, M.counts.ensure(2 * (size_t)nblocks * kOut * sizeof(unsigned long long)));
     unsigned long long* cnt = static_cast<unsigned long long*>(M.counts.p);
-    CC_HIP(ctx, hipMemsetAsync(cnt, 0, 16 * sizeof(unsigned long long), ctx->stream));
-    const uint64_t n64 = n;
-    CC_HIP(ctx, launch(fc, dim3(1024), dim3(256), 0, ctx->stream, M.top1.p, n64, cnt));
-    CC_HIP(ctx, launch(fp, dim3(1), dim3(64), 0, ctx->stream, cnt, cnt + 16, M.bstart.p));
-    CC_HIP(ctx, launch(fs, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, M.top1.p, n64, cnt + 16,
-                       M.idx.p));
+    unsigned long long* off = cnt + (size_t)nblocks * kOut;
+    CC_HIP(ctx, launch(fc, dim3(nblocks), dim3(256), 0, ctx->stream, M.top1.p, n64, chunk, cnt));
+    CC_HIP(ctx, launch(fp, dim3(1), dim3(1024), 0, ctx->stream, cnt, nblocks, off, M.bstart.p));
+    CC_HIP(ctx, launch(fs, dim3(nblocks), dim3(256), 0, ctx->stream, M.top1.p, n64, chunk, off, M.idx.p));
   } else {
     std::vector<uint32_t> id(n);
     for (size_t i = 0; i < n; ++i) id[i] = (uint32_t)i;

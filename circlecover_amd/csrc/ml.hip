@@ -7,7 +7,7 @@
 //   ccml_fwd_bwd    one lane per training sample: forward, softmax
 //                   cross-entropy, backward; stores what the weight
 //                   gradients need
-//   ccml_grad_adam  one lane per parameter: the batch gradient in a fixed
+//   ccml_grad_partial / ccml_grad_adam  the batch gradient in a fixed
 //                   summation order (deterministic), then an Adam step
 //   ccml_rank       batched inference on MFMA (v_mfma_f32_32x32x8bf16_1k):
 //                   each wave takes 32 boxes per tile; top-1 criterion out
@@ -35,17 +35,21 @@ __device__ __forceinline__ uint16_t bf16_bits(float x) { return (uint16_t)(__flo
 
 // Features of box i: centre and log2 width of lambda, r1, r2, r3, each
 // normalised by (value - mean) / std.  A zero width counts as 2^-60.
+// Training evaluates log2 in FP64; inference (fast = true) uses the FP32
+// log2 of the width rounded to FP32, which differs by float rounding only.
+template <bool fast>
 __device__ __forceinline__ float feature(const double* __restrict__ soa, uint64_t ld, uint64_t i, int k,
                                          const float* __restrict__ norm) {
   const double lo = soa[(2 * (k >> 1)) * ld + i], hi = soa[(2 * (k >> 1) + 1) * ld + i];
-  double v;
+  float v;
   if (k & 1) {
     const double w = hi - lo;
-    v = w > 0x1p-60 ? log2(w) : -60.0;
+    if (fast) v = w > 0x1p-60 ? log2f((float)w) : -60.0f;
+    else v = (float)(w > 0x1p-60 ? log2(w) : -60.0);
   } else {
-    v = 0.5 * (lo + hi);
+    v = (float)(0.5 * (lo + hi));
   }
-  return ((float)v - norm[k]) / norm[kIn + k];
+  return (v - norm[k]) / norm[kIn + k];
 }
 
 }  // namespace
@@ -69,7 +73,7 @@ extern "C" __global__ void __launch_bounds__(256) ccml_fwd_bwd(const double* __r
   float x[kIn];
 #pragma unroll
   for (int k = 0; k < kIn; ++k) {
-    x[k] = bf16_round(feature(soa, ld, i, k, norm));
+    x[k] = bf16_round(feature<false>(soa, ld, i, k, norm));
     r[rX + k] = x[k];
   }
   float h1[kH], h2[kH], z3[kOut];
@@ -119,17 +123,15 @@ extern "C" __global__ void __launch_bounds__(256) ccml_fwd_bwd(const double* __r
   }
 }
 
-// One lane per parameter: gradient over the B samples (sample order), then
-// Adam (lr, beta1, beta2, eps; step t >= 1) on the FP32 master unless
-// update == 0.  grad (may be null) receives the raw gradient.
-extern "C" __global__ void __launch_bounds__(256) ccml_grad_adam(const float* __restrict__ rec, int B,
-                                                                  float* __restrict__ params, float* __restrict__ m,
-                                                                  float* __restrict__ v, int t, float lr,
-                                                                  float beta1, float beta2, float eps, int update,
-                                                                  float* __restrict__ grad) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= kParams) return;
-  int a_off, b_off;  // gradient = sum_s a[s] * b[s] (a_off < 0: a = 1)
+// Gradient in two deterministic passes: ccml_grad_partial sums, for every
+// parameter p and slice q of kSlice consecutive samples, a[s] * b[s] over
+// the slice (one lane per (p, q)); ccml_grad_adam adds the slices in order,
+// then takes an Adam step (lr, beta1, beta2, eps; step t >= 1) on the FP32
+// master unless update == 0.  grad (may be null) receives the raw gradient.
+constexpr int kSlice = 64;
+
+__device__ __forceinline__ void grad_operands(int p, int& a_off, int& b_off) {
+  // gradient of parameter p = sum_s a[s] * b[s] (a_off < 0: a = 1)
   if (p < oB1) {
     a_off = rX + p / kH;
     b_off = rD1 + p % kH;
@@ -149,11 +151,33 @@ extern "C" __global__ void __launch_bounds__(256) ccml_grad_adam(const float* __
     a_off = -1;
     b_off = rD3 + (p - oB3);
   }
+}
+
+extern "C" __global__ void __launch_bounds__(256) ccml_grad_partial(const float* __restrict__ rec, int B,
+                                                                     float* __restrict__ partial) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  const int q = blockIdx.y;
+  if (p >= kParams) return;
+  int a_off, b_off;
+  grad_operands(p, a_off, b_off);
+  const int s0 = q * kSlice, s1 = s0 + kSlice < B ? s0 + kSlice : B;
   float g = 0.f;
-  for (int s = 0; s < B; ++s) {
+  for (int s = s0; s < s1; ++s) {
     const float* r = rec + (uint64_t)s * kRec;
     g += (a_off < 0 ? 1.f : r[a_off]) * r[b_off];
   }
+  partial[(uint64_t)q * kParams + p] = g;
+}
+
+extern "C" __global__ void __launch_bounds__(256) ccml_grad_adam(const float* __restrict__ partial, int nslices,
+                                                                  float* __restrict__ params, float* __restrict__ m,
+                                                                  float* __restrict__ v, int t, float lr,
+                                                                  float beta1, float beta2, float eps, int update,
+                                                                  float* __restrict__ grad) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= kParams) return;
+  float g = 0.f;
+  for (int q = 0; q < nslices; ++q) g += partial[(uint64_t)q * kParams + p];
   if (grad) grad[p] = g;
   if (!update) return;
   const float mm = beta1 * m[p] + (1.f - beta1) * g;
@@ -182,12 +206,35 @@ extern "C" __global__ void __launch_bounds__(256) ccml_rank(const double* __rest
                                                              const float* __restrict__ bias,
                                                              uint8_t* __restrict__ top1,
                                                              float* __restrict__ logits) {
-  __shared__ uint16_t act[4][32][kH];  // per wave: one activation tile, bf16
-  __shared__ float out[4][32][32];     // per wave: logits tile
+  // per wave: one activation tile (bf16) and one logits tile; rows padded
+  // by one dword so that lanes reading one row each hit distinct banks
+  __shared__ uint16_t act[4][32][kH + 2];
+  __shared__ float out[4][32][33];
   const int wv = threadIdx.x >> 6, l = threadIdx.x & 63, g = l >> 5, col = l & 31;
   const uint16_t* W1 = wb;
   const uint16_t* W2 = wb + kIn * kH;
   const uint16_t* W3 = W2 + kH * kH;
+  // B operands of every MFMA and this lane's biases stay in registers for
+  // the whole kernel (W1: 2 tiles, W2: 8 k-steps x 2 tiles, W3: 8 k-steps)
+  bf16x4 b1[2], b2[8][2], b3[8];
+  float bias1[2], bias2[2], bias3;
+#pragma unroll
+  for (int t = 0; t < 2; ++t) {
+#pragma unroll
+    for (int e = 0; e < 4; ++e) b1[t][e] = (short)W1[(4 * g + e) * kH + 32 * t + col];
+    bias1[t] = bias[32 * t + col];
+    bias2[t] = bias[kH + 32 * t + col];
+  }
+#pragma unroll
+  for (int s = 0; s < 8; ++s) {
+#pragma unroll
+    for (int t = 0; t < 2; ++t)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) b2[s][t][e] = (short)W2[(8 * s + 4 * g + e) * kH + 32 * t + col];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) b3[s][e] = (short)(col < kOut ? W3[(8 * s + 4 * g + e) * kOut + col] : 0);
+  }
+  bias3 = col < kOut ? bias[2 * kH + col] : 0.f;
   const uint64_t tiles = (n + 31) / 32;
   for (uint64_t tile = (uint64_t)blockIdx.x * 4 + wv; tile < tiles; tile += (uint64_t)gridDim.x * 4) {
     const uint64_t i = tile * 32 + col;
@@ -195,18 +242,15 @@ extern "C" __global__ void __launch_bounds__(256) ccml_rank(const double* __rest
     // layer 1: A = features (k = 4g..4g+3 of box i)
     bf16x4 a;
 #pragma unroll
-    for (int e = 0; e < 4; ++e) a[e] = (short)(live ? bf16_bits(feature(soa, ld, i, 4 * g + e, norm)) : 0);
+    for (int e = 0; e < 4; ++e) a[e] = (short)(live ? bf16_bits(feature<true>(soa, ld, i, 4 * g + e, norm)) : 0);
 #pragma unroll
     for (int t = 0; t < 2; ++t) {
-      bf16x4 b;
-#pragma unroll
-      for (int e = 0; e < 4; ++e) b[e] = (short)W1[(4 * g + e) * kH + 32 * t + col];
       f32x16 acc = {};
-      acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b1[t], acc, 0, 0, 0);
 #pragma unroll
       for (int r = 0; r < 16; ++r) {
         const int m = 8 * (r >> 2) + 4 * g + (r & 3);
-        const float z = acc[r] + bias[32 * t + col];
+        const float z = acc[r] + bias1[t];
         act[wv][m][32 * t + col] = bf16_bits(z > 0.f ? z : 0.f);
       }
     }
@@ -220,12 +264,7 @@ extern "C" __global__ void __launch_bounds__(256) ccml_rank(const double* __rest
 #pragma unroll
       for (int e = 0; e < 4; ++e) a2[e] = (short)act[wv][col][8 * s + 4 * g + e];
 #pragma unroll
-      for (int t = 0; t < 2; ++t) {
-        bf16x4 b;
-#pragma unroll
-        for (int e = 0; e < 4; ++e) b[e] = (short)W2[(8 * s + 4 * g + e) * kH + 32 * t + col];
-        acc2[t] = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a2, b, acc2[t], 0, 0, 0);
-      }
+      for (int t = 0; t < 2; ++t) acc2[t] = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a2, b2[s][t], acc2[t], 0, 0, 0);
     }
     __builtin_amdgcn_s_waitcnt(0);
     __builtin_amdgcn_wave_barrier();
@@ -234,7 +273,7 @@ extern "C" __global__ void __launch_bounds__(256) ccml_rank(const double* __rest
 #pragma unroll
       for (int r = 0; r < 16; ++r) {
         const int m = 8 * (r >> 2) + 4 * g + (r & 3);
-        const float z = acc2[t][r] + bias[kH + 32 * t + col];
+        const float z = acc2[t][r] + bias2[t];
         act[wv][m][32 * t + col] = bf16_bits(z > 0.f ? z : 0.f);
       }
     __builtin_amdgcn_s_waitcnt(0);
@@ -243,18 +282,15 @@ extern "C" __global__ void __launch_bounds__(256) ccml_rank(const double* __rest
     f32x16 acc3 = {};
 #pragma unroll
     for (int s = 0; s < 8; ++s) {
-      bf16x4 a3, b;
+      bf16x4 a3;
 #pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        a3[e] = (short)act[wv][col][8 * s + 4 * g + e];
-        b[e] = (short)(col < kOut ? W3[(8 * s + 4 * g + e) * kOut + col] : 0);
-      }
-      acc3 = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a3, b, acc3, 0, 0, 0);
+      for (int e = 0; e < 4; ++e) a3[e] = (short)act[wv][col][8 * s + 4 * g + e];
+      acc3 = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a3, b3[s], acc3, 0, 0, 0);
     }
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
       const int m = 8 * (r >> 2) + 4 * g + (r & 3);
-      out[wv][m][col] = acc3[r] + (col < kOut ? bias[2 * kH + col] : 0.f);
+      out[wv][m][col] = acc3[r] + bias3;
     }
     __builtin_amdgcn_s_waitcnt(0);
     __builtin_amdgcn_wave_barrier();
@@ -274,51 +310,60 @@ extern "C" __global__ void __launch_bounds__(256) ccml_rank(const double* __rest
   }
 }
 
-// Counting sort of box indices by predicted criterion: counts, then a
-// scatter with per-bucket cursors (order inside a bucket unspecified).
-extern "C" __global__ void ccml_bucket_count(const uint8_t* __restrict__ top1, uint64_t n,
-                                             unsigned long long* __restrict__ counts) {
+// Counting sort of box indices by predicted criterion, in three passes over
+// fixed block ranges (no contended global atomics): per-block bucket counts,
+// a prefix over (bucket, block), then a scatter from per-block offsets.
+// Order inside a bucket is unspecified.
+extern "C" __global__ void __launch_bounds__(256) ccml_bucket_count(const uint8_t* __restrict__ top1, uint64_t n,
+                                                                     uint64_t chunk,
+                                                                     unsigned long long* __restrict__ counts) {
   __shared__ unsigned int c[kOut];
   if (threadIdx.x < kOut) c[threadIdx.x] = 0;
   __syncthreads();
-  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
-    atomicAdd(&c[top1[i] < kOut ? top1[i] : 0], 1u);
+  const uint64_t lo = (uint64_t)blockIdx.x * chunk, hi = lo + chunk < n ? lo + chunk : n;
+  for (uint64_t i = lo + threadIdx.x; i < hi; i += blockDim.x) atomicAdd(&c[top1[i] < kOut ? top1[i] : 0], 1u);
   __syncthreads();
-  if (threadIdx.x < kOut && c[threadIdx.x]) atomicAdd(&counts[threadIdx.x], (unsigned long long)c[threadIdx.x]);
+  if (threadIdx.x < kOut) counts[(uint64_t)blockIdx.x * kOut + threadIdx.x] = c[threadIdx.x];
 }
 
-// Exclusive prefix of the bucket counts: cursor[b] = bstart[b] = first
-// position of bucket b, bstart[10] = total.  One lane.
-extern "C" __global__ void ccml_bucket_prefix(const unsigned long long* __restrict__ counts,
-                                              unsigned long long* __restrict__ cursor,
-                                              uint64_t* __restrict__ bstart) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  unsigned long long acc = 0;
-  for (int b = 0; b < kOut; ++b) {
-    cursor[b] = acc;
-    bstart[b] = acc;
-    acc += counts[b];
+// One workgroup of 1024 lanes (nblocks <= 1024): offsets[b*10 + k] = first
+// position of block b's boxes of bucket k (bucket-major, then block order);
+// bstart[k] = first position of bucket k, bstart[10] = total.
+extern "C" __global__ void __launch_bounds__(1024) ccml_bucket_prefix(const unsigned long long* __restrict__ counts,
+                                                                       uint32_t nblocks,
+                                                                       unsigned long long* __restrict__ offsets,
+                                                                       uint64_t* __restrict__ bstart) {
+  __shared__ unsigned long long sc[1024];
+  const uint32_t b = threadIdx.x;
+  unsigned long long base = 0;
+  for (int k = 0; k < kOut; ++k) {
+    const unsigned long long v = b < nblocks ? counts[(uint64_t)b * kOut + k] : 0ull;
+    sc[b] = v;
+    __syncthreads();
+    for (uint32_t d = 1; d < 1024; d <<= 1) {  // inclusive Hillis-Steele scan
+      const unsigned long long add = b >= d ? sc[b - d] : 0ull;
+      __syncthreads();
+      sc[b] += add;
+      __syncthreads();
+    }
+    if (b < nblocks) offsets[(uint64_t)b * kOut + k] = base + sc[b] - v;
+    if (b == 0) bstart[k] = base;
+    base += sc[1023];
+    __syncthreads();
   }
-  bstart[kOut] = acc;
+  if (b == 0) bstart[kOut] = base;
 }
 
-// Scatter of box indices into their buckets: per block, ranks inside the
-// block from LDS counters and one global atomic per bucket and block (order
-// inside a bucket unspecified).
 extern "C" __global__ void __launch_bounds__(256) ccml_bucket_scatter(const uint8_t* __restrict__ top1, uint64_t n,
-                                                                       unsigned long long* __restrict__ cursor,
+                                                                       uint64_t chunk,
+                                                                       const unsigned long long* __restrict__ offsets,
                                                                        uint32_t* __restrict__ idx) {
-  __shared__ unsigned int cnt[kOut];
-  __shared__ unsigned long long base[kOut];
-  if (threadIdx.x < kOut) cnt[threadIdx.x] = 0;
+  __shared__ unsigned long long cur[kOut];
+  if (threadIdx.x < kOut) cur[threadIdx.x] = offsets[(uint64_t)blockIdx.x * kOut + threadIdx.x];
   __syncthreads();
-  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const bool live = i < n;
-  const int b = live ? (top1[i] < kOut ? top1[i] : 0) : 0;
-  const unsigned rank = live ? atomicAdd(&cnt[b], 1u) : 0u;
-  __syncthreads();
-  if (threadIdx.x < kOut && cnt[threadIdx.x])
-    base[threadIdx.x] = atomicAdd(&cursor[threadIdx.x], (unsigned long long)cnt[threadIdx.x]);
-  __syncthreads();
-  if (live) idx[base[b] + rank] = (uint32_t)i;
+  const uint64_t lo = (uint64_t)blockIdx.x * chunk, hi = lo + chunk < n ? lo + chunk : n;
+  for (uint64_t i = lo + threadIdx.x; i < hi; i += blockDim.x) {
+    const int b = top1[i] < kOut ? top1[i] : 0;
+    idx[atomicAdd(&cur[b], 1ull)] = (uint32_t)i;
+  }
 }
