@@ -2,9 +2,15 @@
 // can_recurse helpers inlined (no call stack: this kernel runs at full
 // occupancy with 81 VGPRs and no scratch).
 #define CC_INLINE_RECURSE 1
+#define CC_EXACT_SHORTCUTS 1
+#include "dev/queue.hpp"
 #include "dev/rsb.hpp"
 
 using namespace ccd;
+
+// Bisection steps a box may take before the cover-strip shortcut applies;
+// boxes needing more go to the second pass.
+static constexpr int kStripCap = 2;
 
 // Synthetic batched-criterion workload: one lane per 4-D box
 // (lambda, r1, r2, r3) stored as 8 SoA FP64 arrays of stride ld; all ten
@@ -14,17 +20,27 @@ using namespace ccd;
 //   0 TR r1_strategies(la, r1, r2.ub)            3 TR r1_r2_r3 horizontal
 //   1 TR r1_r2_strategies_simple(.., r3.ub)      4 TR r1_r2_r3 cover_strip
 //   2 TR r1_r2_strategies_stacked(.., r3.ub)     5..9 RSB shortcuts S1..S5
+//
+// The cover-strip criterion (4) runs its width search with the exact
+// shortcut of r123_cover_strip_capped; the few boxes that would need a long
+// plain bisection first are appended to `deferred` (count in
+// *n_deferred, which must be 0 at launch) with bit 4 left clear, and
+// ccrd_c2_strip_deferred completes them.
 extern "C" __global__ void __launch_bounds__(256) ccrd_c2_eval(const double* __restrict__ soa, uint64_t n,
                                                                 uint64_t ld, const Konst* __restrict__ kp,
                                                                 uint32_t* __restrict__ mask,
-                                                                uint8_t* __restrict__ first) {
+                                                                uint8_t* __restrict__ first,
+                                                                uint32_t* __restrict__ deferred,
+                                                                unsigned long long* __restrict__ n_deferred) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+  // Lanes past n stay to the wave-wide append below and store nothing.
+  const bool live = i < n;
+  const uint64_t j = live ? i : 0;
   const Konst K = *kp;
-  const iv la{soa[0 * ld + i], soa[1 * ld + i]};
-  const iv r1{soa[2 * ld + i], soa[3 * ld + i]};
-  const iv r2{soa[4 * ld + i], soa[5 * ld + i]};
-  const iv r3{soa[6 * ld + i], soa[7 * ld + i]};
+  const iv la{soa[0 * ld + j], soa[1 * ld + j]};
+  const iv r1{soa[2 * ld + j], soa[3 * ld + j]};
+  const iv r2{soa[4 * ld + j], soa[5 * ld + j]};
+  const iv r3{soa[6 * ld + j], soa[7 * ld + j]};
 
   const iv W = trc::weight_for(K, la);
   const double rem4 = (W - r1 - r2 - r3).hi;
@@ -37,14 +53,47 @@ extern "C" __global__ void __launch_bounds__(256) ccrd_c2_eval(const double* __r
   if (tp.flag) m |= 1u << 17;
   if (trc::r1_r2_stacked(K, la, W, r1, r2, r3.hi)) m |= 1u << 2;
   if (trc::r123_horizontal(K, la, W, r1, r2, r3, ub_r4)) m |= 1u << 3;
-  if (trc::r123_cover_strip(K, la, W, r1, r2, r3, ub_r4)) m |= 1u << 4;
+  bool defer = false;
+  if (trc::r123_cover_strip_capped(K, la, W, r1, r2, r3, ub_r4, kStripCap, defer)) m |= 1u << 4;
   if (rsbc::s1_even_split(la, r1, r2)) m |= 1u << 5;
   if (rsbc::s23_uneven_split(la, r1, r2)) m |= 1u << 6;
   if (rsbc::s23_uneven_split(la, r1, r3)) m |= 1u << 7;
   if (rsbc::s4_opposite_corners(la, r1, r2, r3)) m |= 1u << 8;
   if (rsbc::s5_vertical_strip(K, la, r1, r2, r3)) m |= 1u << 9;
+  const uint64_t slot = wave_append(live && defer, n_deferred);
+  if (!live) return;
+  if (defer) deferred[slot] = (uint32_t)i;
   mask[i] = m;
   const uint32_t v = m & 0x3ffu;
   first[i] = v ? (uint8_t)__builtin_ctz(v) : (uint8_t)255;
+}
+
+// Second pass over the boxes ccrd_c2_eval deferred: the plain cover-strip
+// bisection, then bit 4 of mask and first updated.  Grid-stride over the
+// count the first pass left in *n_deferred.
+extern "C" __global__ void __launch_bounds__(256) ccrd_c2_strip_deferred(const double* __restrict__ soa,
+                                                                          uint64_t ld,
+                                                                          const Konst* __restrict__ kp,
+                                                                          const uint32_t* __restrict__ deferred,
+                                                                          const unsigned long long* __restrict__ n_deferred,
+                                                                          uint32_t* __restrict__ mask,
+                                                                          uint8_t* __restrict__ first) {
+  const uint64_t cnt = *n_deferred;
+  const Konst K = *kp;
+  for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < cnt; k += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t i = deferred[k];
+    const iv la{soa[0 * ld + i], soa[1 * ld + i]};
+    const iv r1{soa[2 * ld + i], soa[3 * ld + i]};
+    const iv r2{soa[4 * ld + i], soa[5 * ld + i]};
+    const iv r3{soa[6 * ld + i], soa[7 * ld + i]};
+    const iv W = trc::weight_for(K, la);
+    const double rem4 = (W - r1 - r2 - r3).hi;
+    const double ub_r4 = r3.hi < rem4 ? r3.hi : rem4;
+    if (trc::r123_cover_strip(K, la, W, r1, r2, r3, ub_r4)) {
+      const uint32_t m = mask[i] | (1u << 4);
+      mask[i] = m;
+      first[i] = (uint8_t)__builtin_ctz(m & 0x3ffu);
+    }
+  }
 }
 

@@ -1,5 +1,6 @@
 // FP32 pre-filter kernel of the ten-criterion box set (gfx950, round-down
 // mode for FP32 and FP64).
+#define CC_EXACT_SHORTCUTS 1
 #include "dev/c2_criteria.hpp"
 #include "dev/f32.hpp"
 

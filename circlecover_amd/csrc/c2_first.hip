@@ -5,6 +5,7 @@
 // a wave's boxes come from one bucket (except at the few bucket borders),
 // the wave runs one criterion at a time instead of a different one per lane.
 // With every box in bucket 0 this is the plain order.
+#define CC_EXACT_SHORTCUTS 1
 #include "dev/c2_criteria.hpp"
 
 using namespace ccd;

@@ -17,6 +17,11 @@
 #ifndef CC_INLINE_RECURSE
 #define CC_INLINE_RECURSE 0
 #endif
+// CC_EXACT_SHORTCUTS 1 replaces parts of loops by computations that provably
+// give the loop's own result (see r123_cover_strip); off by default.
+#ifndef CC_EXACT_SHORTCUTS
+#define CC_EXACT_SHORTCUTS 0
+#endif
 #if CC_INLINE_RECURSE
 #define CC_RECURSE_FN CCD
 #else
@@ -230,12 +235,220 @@ CCD bool strip_width_ok(double q1, double q2, double q3, double w) {
   return add_rd(sqrt_rd(w2), sqrt_rd(w3)) >= w;
 }
 
+#if CC_EXACT_SHORTCUTS
+// Estimate of the largest strip width w in [a, b] that strip_width_ok
+// accepts, from the exact real-number form of the test.  With
+// u = sqrt(q1 - w^2) (decreasing in w) and v = 1 - u the test reads
+//   u > 0,  w2 = q2 - v^2 > 0,  w3 = q3 - v^2 > 0,  sqrt(w2) + sqrt(w3) >= w,
+// and the last condition is R = w^2 - w2 - w3 <= 0 or F = 4 w2 w3 - R^2 >= 0:
+// every term is a polynomial in u, so F has no square-root singularity at
+// the height limits and Newton's method converges quadratically on it.
+// Safeguarded Newton (bisection where R <= 0 or a step leaves the bracket)
+// finds the threshold u*, and w = sqrt(q1 - u*^2) is returned.  Only a
+// starting point: arithmetic rounding here does not affect any result.
+CCD double strip_width_estimate(double q1, double q2, double q3, double a, double b) {
+  const double u_hi = sqrt_rd(fmax(add_rd(q1, -mul_rd(a, a)), 0.0));
+  const double h_b = fmax(add_rd(q1, -mul_rd(b, b)), 0.0);
+  const double qm = fmin(q2, q3);
+  const double u_w = qm < 1.0 ? add_rd(1.0, -sqrt_rd(qm)) : 0.0;
+  double x0 = fmax(sqrt_rd(h_b), u_w), x1 = u_hi, x = x0;
+  bool first = true;
+  for (int it = 0; it < 32; ++it) {
+    const double v = add_rd(1.0, -x);
+    const double vv = mul_rd(v, v);
+    const double w2 = add_rd(q2, -vv), w3 = add_rd(q3, -vv);
+    const double R = add_rd(add_rd(q1, -mul_rd(x, x)), -add_rd(w2, w3));
+    const double F = fma_rd(4.0 * w2, w3, -mul_rd(R, R));
+    if (R <= 0.0 || F >= 0.0) {
+      if (first) break;  // accepted at the narrowest end of the range
+      x1 = x;
+    } else {
+      x0 = x;
+    }
+    first = false;
+    const double dF = fma_rd(8.0 * v, add_rd(w2, w3), mul_rd(4.0 * R, fma_rd(2.0, v, x)));
+    double nx = add_rd(x, -(F / dF));  // any nearby quotient will do
+    if (R <= 0.0 || !(nx >= x0 && nx <= x1)) {
+      nx = mul_rd(0.5, add_rd(x0, x1));
+    } else if (fabs(add_rd(nx, -x)) <= 0x1p-51) {  // u is O(1): stop at a few ulps
+      x = nx;
+      break;
+    }
+    x = nx;
+    if (add_rd(x1, -x0) <= 0x1p-51) break;
+  }
+  return sqrt_rd(fmax(add_rd(q1, -mul_rd(x, x)), 0.0));
+}
+
+// strip_width_ok, also returning the covered width s = sqrt(w2) + sqrt(w3)
+// (0 when a height test fails).
+CCD bool strip_eval(double q1, double q2, double q3, double w, double& s) {
+  s = 0.0;
+  double h1 = add_rd(q1, -mul_ru(w, w));
+  if (h1 <= 0.0) return false;
+  double hrem = add_ru(1.0, -sqrt_rd(h1));
+  double hsq = mul_ru(hrem, hrem);
+  double w2 = add_rd(q2, -hsq);
+  double w3 = add_rd(q3, -hsq);
+  if (w3 <= 0.0 || w2 <= 0.0) return false;
+  s = add_rd(sqrt_rd(w2), sqrt_rd(w3));
+  return s >= w;
+}
+
+// Is the test true-then-false on [lo, b], b the largest double below hi,
+// given that it holds at lo with covered width s_lo?  Every rounded step is
+// monotone in w; with h1(w) = q1 (-) w*w (rounded down, non-increasing):
+//  * where h1(w) <= 1 the remaining height 1 - sqrt(h1) is >= 0 and grows
+//    with w, so w2, w3 and s shrink while w grows: true, then false;
+//  * where h1(w) >= 1 the remaining height is <= 0 and its square shrinks as
+//    w grows, so w2, w3 and s grow: true at lo means the height tests pass
+//    on the rest of that range, and s(w) >= s_lo >= w there as long as
+//    w <= s_lo.  That range ends at or below sqrt(q1 - 1) <= U (computed
+//    rounding up), so s_lo >= min(b, U) makes the test true on all of it.
+// The h1 >= 1 range precedes the h1 <= 1 range, hence true-then-false.
+CCD bool strip_monotone_from(double q1, double U, double lo, double hi, double s_lo) {
+  if (!(hi > lo) || !(hi < __builtin_inf())) return false;
+  if (add_rd(q1, -mul_ru(lo, lo)) <= 1.0) return true;
+  const double b = next_down(hi);
+  return s_lo >= (b < U ? b : U);
+}
+
+// The cover-strip bisection from [lo, hi] (test true at lo, covered width
+// s_lo).  Bisection steps run until strip_monotone_from holds; from then on
+// the loop's final lo is the largest double T below hi that the test
+// accepts, found by one test at the estimate, a gallop outward by 1, 2, 4,
+// ... ulps to the first change of outcome, and a bisection of the bit
+// patterns (positive doubles: bit order is numeric order).  Each lane walks
+// these phases at its own pace, but every iteration of the one loop runs
+// exactly one test, so a wave runs the test as often as its slowest lane
+// needs, not once per phase and lane group.  Returns true with the final lo,
+// or false with (lo, hi) at the state reached when more than `cap`
+// bisection steps, or 48 tests in all, would be needed.
+CCD bool strip_search(double q1, double q2, double q3, double& lo, double& hi, double s_lo, int cap) {
+  const double U = q1 > 1.0 ? sqrt_ru(add_ru(q1, -1.0)) : 0.0;
+  const double g = strip_width_estimate(q1, q2, q3, lo, next_down(hi));
+  enum { kBisect, kFirst, kUp, kDown, kBits, kDone };
+  int phase = kBisect, steps = 0;
+  double b = 0.0, t = 0.0, f = 0.0, c = 0.0;
+  uint64_t d = 1;
+  bool ok = true;
+  for (int n = 0;; ++n) {
+    if (phase == kBisect) {
+      if (strip_monotone_from(q1, U, lo, hi, s_lo)) {
+        b = next_down(hi);
+        if (b > lo) {
+          c = g > lo ? (g < b ? g : b) : lo;
+          phase = kFirst;
+        } else {
+          phase = kDone;
+        }
+      } else if (steps >= cap) {
+        ok = false;
+        phase = kDone;
+      } else {
+        c = rn_mid(lo, hi);
+        ++steps;
+        if (c <= lo || c >= hi) phase = kDone;
+      }
+    } else if (phase == kUp) {
+      if (bits(t) == bits(b)) {
+        lo = b;
+        phase = kDone;
+      } else {
+        c = from_bits(bits(b) - bits(t) > d ? bits(t) + d : bits(b));
+      }
+    } else if (phase == kDown) {
+      if (bits(f) - bits(lo) <= d) {
+        t = lo;
+        phase = kBits;
+      } else {
+        c = from_bits(bits(f) - d);
+      }
+    }
+    if (phase == kBits) {
+      if (bits(f) - bits(t) <= 1) {
+        lo = t;
+        phase = kDone;
+      } else {
+        c = from_bits(bits(t) + (bits(f) - bits(t)) / 2);
+      }
+    }
+    if (phase == kDone) break;
+    if (n >= 48) {
+      ok = false;
+      break;
+    }
+    double s;
+    const bool r = strip_eval(q1, q2, q3, c, s);
+    switch (phase) {
+      case kBisect:
+        if (r) {
+          lo = c;
+          s_lo = s;
+        } else {
+          hi = c;
+        }
+        break;
+      case kFirst:
+        if (r) t = c;
+        else f = c;
+        phase = r ? kUp : kDown;
+        d = 1;
+        break;
+      case kUp:
+        if (r) {
+          t = c;
+          d *= 2;
+        } else {
+          f = c;
+          phase = kBits;
+        }
+        break;
+      case kDown:
+        if (r) {
+          t = c;
+          phase = kBits;
+        } else {
+          f = c;
+          d *= 2;
+        }
+        break;
+      default:  // kBits
+        if (r) t = c;
+        else f = c;
+        break;
+    }
+  }
+  return ok;
+}
+#endif
+
 // r1_r2_r3_strategy_cover_strip: r1_r2_r3_strategies.cu:69-94
 // `trace` (diagnostics only, null in production): per iteration lo, hi, mid.
 CCD bool r123_cover_strip(const Konst& K, iv la, iv W, iv r1, iv r2, iv r3, double ub_r4,
                           double* trace = nullptr, int trace_cap = 0) {
   const double q1 = mul_rd(4.0, r1.lo), q2 = mul_rd(4.0, r2.lo), q3 = mul_rd(4.0, r3.lo);
   double lo = 0.3, hi = la.hi;
+#if CC_EXACT_SHORTCUTS
+  // The bisection keeps lo accepted and hi rejected (or the initial bound)
+  // and stops when no double lies strictly between them; on a range where
+  // the test is true-then-false it therefore ends at lo = the last accepted
+  // double below hi, which strip_search finds with a handful of tests.  The
+  // diagnostic trace keeps the full loop.
+  if (!trace) {
+    double s_lo;
+    if (!strip_eval(q1, q2, q3, lo, s_lo)) return false;
+    if (!strip_search(q1, q2, q3, lo, hi, s_lo, kBisectCap)) {
+      for (int it = 0; it < kBisectCap; ++it) {
+        double mid = rn_mid(lo, hi);
+        if (mid <= lo || mid >= hi) break;
+        if (strip_width_ok(q1, q2, q3, mid)) lo = mid;
+        else hi = mid;
+      }
+    }
+    return can_recurse(K, add_ru(la.hi, -lo), 1.0, ub_r4, (W - r1 - r2 - r3).lo);
+  }
+#endif
   if (!strip_width_ok(q1, q2, q3, lo)) return false;
   for (int it = 0; it < kBisectCap; ++it) {
     double mid = rn_mid(lo, hi);
@@ -250,6 +463,26 @@ CCD bool r123_cover_strip(const Konst& K, iv la, iv W, iv r1, iv r2, iv r3, doub
   }
   return can_recurse(K, add_ru(la.hi, -lo), 1.0, ub_r4, (W - r1 - r2 - r3).lo);
 }
+
+#if CC_EXACT_SHORTCUTS
+// r123_cover_strip for the batch kernel: at most `cap` bisection steps before
+// the shortcut applies.  A box that needs more is left undecided
+// (`deferred` = true, result false) for a second pass that runs the plain
+// loop on the few such boxes together, instead of holding up the 63 other
+// lanes of its wave.
+CCD bool r123_cover_strip_capped(const Konst& K, iv la, iv W, iv r1, iv r2, iv r3, double ub_r4, int cap,
+                                 bool& deferred) {
+  const double q1 = mul_rd(4.0, r1.lo), q2 = mul_rd(4.0, r2.lo), q3 = mul_rd(4.0, r3.lo);
+  double lo = 0.3, hi = la.hi, s_lo;
+  deferred = false;
+  if (!strip_eval(q1, q2, q3, lo, s_lo)) return false;
+  if (!strip_search(q1, q2, q3, lo, hi, s_lo, cap)) {
+    deferred = true;
+    return false;
+  }
+  return can_recurse(K, add_ru(la.hi, -lo), 1.0, ub_r4, (W - r1 - r2 - r3).lo);
+}
+#endif
 
 CCD bool r123_strategies(const Konst& K, iv la, iv W, iv r1, iv r2, iv r3, double ub_r4) {  // :96-99
   return r123_horizontal(K, la, W, r1, r2, r3, ub_r4) || r123_cover_strip(K, la, W, r1, r2, r3, ub_r4);

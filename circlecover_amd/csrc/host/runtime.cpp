@@ -171,10 +171,25 @@ int cc_eval_batch_async(cc_ctx* ctx, int prover, int precision, const double* d_
   if (!d_soa || !d_mask || !d_first) return fail(ctx, CC_E_ARG, "null buffer");
   const unsigned block = 256;
   const uint64_t blocks = (n + block - 1) / block;
-  if (blocks > 0x7fffffffull) return fail(ctx, CC_E_ARG, "batch too large");
+  if (blocks > 0x7fffffffull || n > 0xffffffffull) return fail(ctx, CC_E_ARG, "batch too large");
   uint64_t n64 = n, ld64 = ld;
+  // Boxes whose cover-strip search the first pass defers: a counter, then a
+  // list of up to n box indices.
+  DevBuf& db = scratch(ctx).c2_defer;
+  CC_HIP(ctx, db.ensure(256 + 4 * n));
+  unsigned long long* d_cnt = db.as<unsigned long long>();
+  uint32_t* d_list = reinterpret_cast<uint32_t*>(db.as<char>() + 256);
+  hipFunction_t f_zero, f_rest;
+  CC_HIP(ctx, kernel(ctx, "cc_zero_u64", &f_zero));
+  CC_HIP(ctx, kernel(ctx, "ccrd_c2_strip_deferred", &f_rest));
+  uint64_t one = 1;
+  CC_HIP(ctx, launch(f_zero, dim3(1), dim3(64), 0, ctx->stream, d_cnt, one));
   CC_HIP(ctx, launch(ctx->f_c2_eval, dim3((unsigned)blocks), dim3(block), 0, ctx->stream, d_soa, n64, ld64,
-                     ctx->d_konst, d_mask, d_first));
+                     ctx->d_konst, d_mask, d_first, d_list, d_cnt));
+  // The deferred boxes are few (well under 1% of a typical batch); 256
+  // blocks cover them in one sweep and stride over more.
+  CC_HIP(ctx, launch(f_rest, dim3(256), dim3(block), 0, ctx->stream, d_soa, ld64, ctx->d_konst,
+                     (const uint32_t*)d_list, (const unsigned long long*)d_cnt, d_mask, d_first));
   return CC_OK;
 }
 

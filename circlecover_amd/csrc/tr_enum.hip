@@ -9,6 +9,7 @@
 // pair is visited once, as in the reference, so the multiset of unproven
 // leaves is the same; only their order differs (the reference's is that of
 // its atomicAdd too).
+#define CC_EXACT_SHORTCUTS 1
 #include "dev/queue.hpp"
 #include "dev/tr.hpp"
 

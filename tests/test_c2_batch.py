@@ -33,6 +33,42 @@ def small_radius_boxes(n: int) -> np.ndarray:
     return soa
 
 
+def cover_strip_boxes(n: int) -> np.ndarray:
+    """Boxes aimed at the cover-strip criterion's width search: equal
+    squared radii r2 = r3 (both height tests bind together), r1 near 1/4
+    (the search starts where its test is not yet monotone), lambda just above
+    the starting width 0.3 (the test holds up to lambda), radii a hair apart,
+    and random boxes."""
+    rng = np.random.default_rng(11)
+    la = 1.0 + 7.0 * rng.random(n)
+    r1 = rng.random(n) * 0.6 * la
+    r2 = rng.random(n) * r1
+    r3 = rng.random(n) * r2
+    kind = np.arange(n) % 6
+    m = kind == 0
+    r3[m] = r2[m]
+    m = kind == 1
+    r1[m] = 0.25 + 0.05 * rng.random(m.sum())
+    r2[m] = 0.25 - 1e-3 * rng.random(m.sum())
+    r3[m] = r2[m]
+    m = kind == 2
+    la[m] = 0.3 + 0.1 * rng.random(m.sum())
+    m = kind == 3
+    r2[m] = 0.25 * (1 + 1e-12 * rng.random(m.sum()))
+    r3[m] = 0.25 * (1 - 1e-9 * rng.random(m.sum()))
+    m = kind == 4
+    la[m] = 1.0 + 0.5 * rng.random(m.sum())
+    r1[m] = 0.2 + 0.3 * rng.random(m.sum())
+    r2[m] = r1[m] * (0.5 + 0.5 * rng.random(m.sum()))
+    r3[m] = r2[m] * (0.5 + 0.5 * rng.random(m.sum()))
+    w = np.where(rng.random(n) < 0.5, 0.0, 2.0 ** -20)
+    soa = np.empty((8, n))
+    for k, v in enumerate((la, r1, r2, r3)):
+        soa[2 * k] = v
+        soa[2 * k + 1] = v + w
+    return soa
+
+
 def test_oracle_c2_deterministic_and_threads_agree():
     soa = np.concatenate([gen_c2_boxes(SEED, 3000), small_radius_boxes(3000)], axis=1)
     m1, f1 = oracle.c2_eval(soa, threads=1)
@@ -63,6 +99,17 @@ def test_gpu_c2_matches_oracle_small_radii(prover):
     om, of = oracle.c2_eval(soa)
     assert (gm == om).all() and (gf == of).all()
     assert ((om >> 6) & 7).any()  # the uneven-split / opposite-corner shortcuts are exercised
+
+
+@pytest.mark.gpu
+def test_gpu_c2_matches_oracle_cover_strip_regimes(prover):
+    soa = cover_strip_boxes(1 << 15)
+    gm, gf = prover.eval_batch(soa)
+    om, of = oracle.c2_eval(soa)
+    bad = np.flatnonzero((gm != om) | (gf != of))
+    assert bad.size == 0, f"{bad.size} boxes differ; first {bad[:5]}: gpu {gm[bad[:5]]} oracle {om[bad[:5]]}"
+    strip = (om >> 4) & 1
+    assert strip.any() and not strip.all()  # the criterion both proves and fails boxes here
 
 
 @pytest.mark.gpu
