@@ -24,7 +24,7 @@ CC_RECURSE_FN bool can_recurse(const Konst& K, double lb_rem, double ub_largest,
   if (mul_rd(kDiskWeightMax, mul_rd(side, side)) >= ub_largest) return true;
   double skew = div_ru(smax, smin);
   if (skew <= kRsbToleratedLambda) {
-    double cr = trc::crit_ratio(K, iv{div_rd(smax, smin), skew}).hi;
+    double cr = trc::crit_ratio_hi(K, iv{div_rd(smax, smin), skew});
     return mul_ru(cr, mul_ru(ub_w, ub_h)) <= lb_rem;
   }
   double sig = div_ru(ub_largest, mul_rd(smin, smin));

@@ -85,12 +85,63 @@ CCD iv crit_ratio(const Konst& K, iv w, iv h) {
   iv c2 = crit_ratio(K, wh);
   return iv{c1.lo < c2.lo ? c1.lo : c2.lo, c1.hi < c2.hi ? c2.hi : c1.hi};
 }
+// RD and RU of n/d from one quotient: RU(n/d) is RD(n/d), or the next double
+// when the residual n - q*d (exact for operands in div_rd's fast range) is
+// nonzero.  Same pair as pt(n) / pt(d) for such operands.
+CCD iv div_pt_fast(double n, double d) {  // n, d nonzero, div_fast_ok(n, d)
+  const double q = d > 0.0 ? div_rd_fast_pos(n, d) : div_rd_fast_pos(-n, -d);
+  return iv{q, fnma_rd(q, d, n) == 0.0 ? q : next_up(q)};
+}
+
+// crit_ratio(K, la).hi forming only the upper bounds that reach it (three
+// quotients instead of six), for la = [lo, hi] of one sign with both ends
+// of magnitude in [2^-500, 2^500] (checked by crit_hi_ok).  Each bound is
+// the one the interval operations of crit_ratio(K, la) select: with
+// x = (3/256)/la > 0 or < 0 and y = 16 la^2 + 40 + 9/la^2 > 0, the largest
+// corner of x*y is RU(x.hi*y.hi) or RU(x.hi*y.lo); b = la/4 + 1/(2 la) only
+// occurs for la > be_lo > 0.
+CCD bool crit_hi_ok(iv la) {
+  const uint32_t el = (uint32_t)((bits(la.lo) >> 52) & 0x7ff), eh = (uint32_t)((bits(la.hi) >> 52) & 0x7ff);
+  return (el - 523u) < 1000u && (eh - 523u) < 1000u && ((bits(la.lo) ^ bits(la.hi)) >> 63) == 0;
+}
+CCD double crit_ratio_hi_fast(const Konst& K, iv la) {
+  const bool lo_side = la.hi <= K.be_lo;
+  const bool hi_side = !lo_side && la.lo >= K.be_hi;
+  double ahi = 0.0;
+  if (!hi_side) {
+    // la > 0: y.hi = (RU(16 sq.hi + 40)) (+RU) RU(9 / sq.lo), a.hi = RU(x.hi * y.hi);
+    // la < 0: y.lo = (RD(16 sq.lo + 40)) (+RD) RD(9 / sq.hi), a.hi = RU(x.hi * y.lo).
+    // Written once with sg = -1 (la > 0) or +1: RU(u) = -RD(-u) on exact negations.
+    const bool pos = la.lo > 0.0;
+    const double sg = pos ? -1.0 : 1.0;
+    const double sql = pos ? mul_rd(la.lo, la.lo) : mul_rd(la.hi, la.hi);
+    const double sqh = pos ? mul_ru(la.hi, la.hi) : mul_ru(la.lo, la.lo);
+    const double t9 = div_rd(sg * 9.0, pos ? sql : sqh);
+    const double y = sg * add_rd(add_rd(sg * mul_rd(pos ? sqh : sql, 16.0), sg * 40.0), t9);
+    ahi = mul_ru(div_ru(3.0 / 256.0, la.lo), y);
+    if (lo_side) return ahi;
+  }
+  const double bhi = add_ru(mul_rd(la.hi, 0.25), mul_rd(rcp_ru(la.lo), 0.5));
+  if (hi_side) return bhi;
+  return ahi < bhi ? bhi : ahi;
+}
+CCD double crit_ratio_hi(const Konst& K, iv la) {
+  return crit_hi_ok(la) ? crit_ratio_hi_fast(K, la) : crit_ratio(K, la).hi;
+}
+// crit_ratio(K, pt(w), pt(h)).hi: for ordered points the reference divides
+// the larger by the smaller.
+CCD double crit_ratio_pt_hi(const Konst& K, double w, double h) {
+  const double n = w <= h ? h : w, d = w <= h ? w : h;
+  if ((w <= h || h <= w) && n != 0.0 && div_fast_ok(n, d)) return crit_ratio_hi(K, div_pt_fast(n, d));
+  return crit_ratio(K, pt(w), pt(h)).hi;
+}
+
 CCD iv weight_for(const Konst& K, iv w, iv h) { return crit_ratio(K, w, h) * w * h; }
 CCD iv weight_for(const Konst& K, iv la) { return crit_ratio(K, la) * la; }
 
 // can_recurse(ub_w, ub_h, ub_largest, lb_rem): can_recurse.cu:32-62
 CC_RECURSE_FN bool can_recurse(const Konst& K, double ub_w, double ub_h, double ub_largest, double lb_rem) {
-  double cr = crit_ratio(K, pt(ub_w), pt(ub_h)).hi;
+  double cr = crit_ratio_pt_hi(K, ub_w, ub_h);
   double area = mul_ru(ub_w, ub_h);
   double side = ub_w < ub_h ? ub_w : ub_h;
   double scale = rcp_ru(side);
