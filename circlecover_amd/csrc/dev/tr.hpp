@@ -137,6 +137,14 @@ CCD double crit_ratio_pt_hi(const Konst& K, double w, double h) {
 }
 
 CCD iv weight_for(const Konst& K, iv w, iv h) { return crit_ratio(K, w, h) * w * h; }
+// weight_for(K, pt(w), pt(h)).hi: for w, h > 0 the products' upper corners
+// are RU(RU(crit.hi * w) * h).
+CCD double weight_for_pt_hi(const Konst& K, double w, double h) {
+  const uint32_t ew = (uint32_t)(bits(w) >> 52), eh = (uint32_t)(bits(h) >> 52);  // sign and exponent
+  if ((ew - 823u) < 400u && (eh - 823u) < 400u)  // w, h in [2^-200, 2^200)
+    return mul_ru(mul_ru(crit_ratio_pt_hi(K, w, h), w), h);
+  return weight_for(K, pt(w), pt(h)).hi;
+}
 CCD iv weight_for(const Konst& K, iv la) { return crit_ratio(K, la) * la; }
 
 // can_recurse(ub_w, ub_h, ub_largest, lb_rem): can_recurse.cu:32-62
@@ -155,12 +163,26 @@ CC_RECURSE_FN bool can_recurse(const Konst& K, double ub_w, double ub_h, double 
 
 // two_disks_maximize_height (rectangle_size_bound/maximize_covered.cu:106-130)
 CCD double two_disk_height(double r1, double r2, double ub_w) {
-  double wsq = mul_ru(ub_w, ub_w);
   double dlo = add_rd(r1, -r2);
   double dhi = add_ru(r1, -r2);
   double dsq = mul_ru(dhi, dhi);
-  double w1 = add_ru(mul_ru(0.5, ub_w), mul_ru(div_ru(2.0, ub_w), dhi));
-  double w2 = add_ru(mul_ru(0.5, ub_w), -mul_rd(div_rd(2.0, ub_w), dlo));
+  if (ub_w == 1.0) {
+    // The same operations with every product and quotient by ub_w = 1
+    // (exact in any rounding) left out; call sites passing the literal fold
+    // this branch at compile time.
+    double w1 = add_ru(0.5, mul_ru(2.0, dhi));
+    double w2 = add_ru(0.5, -mul_rd(2.0, dlo));
+    w1 = mul_ru(w1, w1);
+    w2 = mul_ru(w2, w2);
+    if (r1 <= mul_ru(0.25, w1) || r2 <= mul_ru(0.25, w2)) return 0.0;
+    double h = add_rd(mul_rd(2.0, add_rd(r1, r2)), -add_ru(0.25, mul_ru(4.0, dsq)));
+    if (h <= 0) return 0.0;
+    return sqrt_rd(h);
+  }
+  double wsq = mul_ru(ub_w, ub_w);
+  const iv two_w = ub_w != 0.0 && div_fast_ok(2.0, ub_w) ? div_pt_fast(2.0, ub_w) : iv{div_rd(2.0, ub_w), div_ru(2.0, ub_w)};
+  double w1 = add_ru(mul_ru(0.5, ub_w), mul_ru(two_w.hi, dhi));
+  double w2 = add_ru(mul_ru(0.5, ub_w), -mul_rd(two_w.lo, dlo));
   w1 = mul_ru(w1, w1);
   w2 = mul_ru(w2, w2);
   if (r1 <= mul_ru(0.25, w1) || r2 <= mul_ru(0.25, w2)) return 0.0;
@@ -181,7 +203,7 @@ CCD bool r1_strategies(const Konst& K, iv la, iv W, iv r1, double ub_r2) {
   if (sq_w < 1.0) {
     double wright = add_ru(la.hi, -sq_w);
     double minright = wright < 1.0 ? wright : 1.0;
-    double cost = disk_fits(ub_r2, minright) ? mul_ru(wright, kRsbRatio) : weight_for(K, pt(wright), pt(1.0)).hi;
+    double cost = disk_fits(ub_r2, minright) ? mul_ru(wright, kRsbRatio) : weight_for_pt_hi(K, wright, 1.0);
     cost = add_ru(cost, ub_r2);
     double rem_top = add_rd(rem, -cost);
     if (rem_top > 0 && can_recurse(K, sq_w, add_ru(1.0, -sq_w), ub_r2, rem_top)) return true;
@@ -256,7 +278,7 @@ CCD bool r1_r2_stacked(const Konst& K, iv la, iv W, iv r1, iv r2, double ub_r3) 
     double left = add_rd(R3, -need);
     return left > 0.0 && can_recurse(K, w_top, top_h, ub_r3, left);
   }
-  double need = add_ru(weight_for(K, pt(w_top), pt(top_h)).hi, ub_r3);
+  double need = add_ru(weight_for_pt_hi(K, w_top, top_h), ub_r3);
   double left = add_rd(R3, -need);
   return left > 0.0 && can_recurse(K, w_bot, side, ub_r3, left);
 }
