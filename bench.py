@@ -9,8 +9,10 @@ Boxes are generated on the host and uploaded before the timed region.
 
 Prints ONE JSON line.  `roofline.achieved` counts the algorithmic directed
 FP64 operations (reference op sequence, counted by the CPU oracle's counting
-build on a seeded sample and scaled to the batch) per kernel launch, divided
-by the launch duration measured with HIP events on the launch stream.
+build on a seeded sample and scaled to the batch) per batch, divided by the
+batch's duration measured with HIP events on the launch stream.  A batch is
+ccrd_c2_eval plus its two small companions (the counter reset and the
+second pass over the boxes whose cover-strip search it deferred).
 `cpu_baseline` times the CPU oracle on the host cores on the same boxes.
 
 Unless --core-only is given, the line also carries `workloads`: the other
@@ -132,7 +134,8 @@ def measured_traffic() -> dict | None:
             vals = {}
             for r in rows:
                 name = r.get("Kernel_Name", "")
-                key = "copy" if "cc_copy_u4" in name else "c2" if "ccrd_c2_eval" in name else None
+                batch = ("ccrd_c2_eval", "ccrd_c2_strip_deferred", "cc_zero_u64")  # one batch's launches
+                key = "copy" if "cc_copy_u4" in name else "c2" if any(k in name for k in batch) else None
                 if key and r.get("Counter_Name", "") == counter:
                     vals[key] = vals.get(key, 0.0) + float(r["Counter_Value"])
             if "copy" not in vals or "c2" not in vals or vals["copy"] <= 0:

@@ -10,7 +10,7 @@ using namespace ccd;
 
 // Bisection steps a box may take before the cover-strip shortcut applies;
 // boxes needing more go to the second pass.
-static constexpr int kStripCap = 2;
+static constexpr int kStripCap = 4;
 
 // Synthetic batched-criterion workload: one lane per 4-D box
 // (lambda, r1, r2, r3) stored as 8 SoA FP64 arrays of stride ld; all ten
@@ -69,8 +69,17 @@ extern "C" __global__ void __launch_bounds__(256) ccrd_c2_eval(const double* __r
 }
 
 // Second pass over the boxes ccrd_c2_eval deferred: the plain cover-strip
-// bisection, then bit 4 of mask and first updated.  Grid-stride over the
-// count the first pass left in *n_deferred.
+// bisection, then bit 4 of mask and first updated.  The bisection is
+// sequential, and the deferred boxes are few, so one lane per box would
+// leave the GPU mostly idle; instead a group of 16 lanes takes a box and
+// advances its bisection four levels per round: lane j (1..15) tests the
+// midpoint of node j of the next four levels' binary tree (heap order; the
+// bits of j below its leading one are the path, 1 = test accepted, lo moves
+// up), then every lane walks the tree with the shuffled outcomes.  Each
+// midpoint is computed exactly as the sequential loop computes it, so the
+// result is the sequential one, test by test.
+static constexpr int kGroup = 16, kLevels = 4;
+
 extern "C" __global__ void __launch_bounds__(256) ccrd_c2_strip_deferred(const double* __restrict__ soa,
                                                                           uint64_t ld,
                                                                           const Konst* __restrict__ kp,
@@ -80,20 +89,64 @@ extern "C" __global__ void __launch_bounds__(256) ccrd_c2_strip_deferred(const d
                                                                           uint8_t* __restrict__ first) {
   const uint64_t cnt = *n_deferred;
   const Konst K = *kp;
-  for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < cnt; k += (uint64_t)gridDim.x * blockDim.x) {
+  const int lane = threadIdx.x % kGroup;
+  const uint64_t groups = (uint64_t)gridDim.x * blockDim.x / kGroup;
+  for (uint64_t k = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / kGroup; k < cnt; k += groups) {
     const uint64_t i = deferred[k];
     const iv la{soa[0 * ld + i], soa[1 * ld + i]};
     const iv r1{soa[2 * ld + i], soa[3 * ld + i]};
     const iv r2{soa[4 * ld + i], soa[5 * ld + i]};
     const iv r3{soa[6 * ld + i], soa[7 * ld + i]};
+    const double q1 = mul_rd(4.0, r1.lo), q2 = mul_rd(4.0, r2.lo), q3 = mul_rd(4.0, r3.lo);
+    double lo = 0.3, hi = la.hi;
+    if (!trc::strip_width_ok(q1, q2, q3, lo)) continue;  // (deferred boxes pass this test)
+    for (int round = 0; round <= kBisectCap / kLevels; ++round) {
+      bool valid = false;
+      double pt = 0.0;
+      if (lane != 0) {
+        const int depth = 31 - __builtin_clz((unsigned)lane);
+        double l = lo, h = hi;
+        valid = true;
+        for (int b = depth - 1; b >= 0; --b) {
+          const double m = rn_mid(l, h);
+          if (m <= l || m >= h) {
+            valid = false;
+            break;
+          }
+          if ((lane >> b) & 1) l = m;
+          else h = m;
+        }
+        if (valid) {
+          pt = rn_mid(l, h);
+          valid = !(pt <= l || pt >= h);
+        }
+      }
+      const int r = valid && trc::strip_width_ok(q1, q2, q3, pt);
+      int node = 1;
+      bool done = false;
+      for (int lev = 0; lev < kLevels; ++lev) {
+        const double m = rn_mid(lo, hi);
+        if (m <= lo || m >= hi) {
+          done = true;
+          break;
+        }
+        if (__shfl(r, node, kGroup)) {
+          lo = m;
+          node = 2 * node + 1;
+        } else {
+          hi = m;
+          node = 2 * node;
+        }
+      }
+      if (done) break;
+    }
     const iv W = trc::weight_for(K, la);
     const double rem4 = (W - r1 - r2 - r3).hi;
     const double ub_r4 = r3.hi < rem4 ? r3.hi : rem4;
-    if (trc::r123_cover_strip(K, la, W, r1, r2, r3, ub_r4)) {
+    if (lane == 0 && trc::can_recurse(K, add_ru(la.hi, -lo), 1.0, ub_r4, (W - r1 - r2 - r3).lo)) {
       const uint32_t m = mask[i] | (1u << 4);
       mask[i] = m;
       first[i] = (uint8_t)__builtin_ctz(m & 0x3ffu);
     }
   }
 }
-

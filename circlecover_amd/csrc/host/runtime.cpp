@@ -186,9 +186,10 @@ int cc_eval_batch_async(cc_ctx* ctx, int prover, int precision, const double* d_
   CC_HIP(ctx, launch(f_zero, dim3(1), dim3(64), 0, ctx->stream, d_cnt, one));
   CC_HIP(ctx, launch(ctx->f_c2_eval, dim3((unsigned)blocks), dim3(block), 0, ctx->stream, d_soa, n64, ld64,
                      ctx->d_konst, d_mask, d_first, d_list, d_cnt));
-  // The deferred boxes are few (well under 1% of a typical batch); 256
-  // blocks cover them in one sweep and stride over more.
-  CC_HIP(ctx, launch(f_rest, dim3(256), dim3(block), 0, ctx->stream, d_soa, ld64, ctx->d_konst,
+  // The deferred boxes are few (well under 1% of a typical batch); 1024
+  // blocks of 16 groups of 16 lanes cover 16384 of them in one sweep and
+  // stride over more.
+  CC_HIP(ctx, launch(f_rest, dim3(1024), dim3(block), 0, ctx->stream, d_soa, ld64, ctx->d_konst,
                      (const uint32_t*)d_list, (const unsigned long long*)d_cnt, d_mask, d_first));
   return CC_OK;
 }
