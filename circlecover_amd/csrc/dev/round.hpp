@@ -164,33 +164,21 @@ CCD double rsq_seed(double x) {
 
 // ------------------------------------------------------------------ division
 // Round-down quotient for b > 0, a and b finite with |a|,|b| in
-// [2^-500, 2^500]: Newton approximation then residual stepping.
+// [2^-500, 2^500]: Newton approximation then residual stepping.  The seed is
+// good to about 2^-25, one Newton step makes y good to about 2^-50, and the
+// corrected quotient q + r*y then misses a/b by at most 2^-48 ulp, so q is
+// RD(a/b) or a neighbour of it: one residual test each way settles it
+// (sign(a - q*b) == sign(a/b - q), exact from one FMA here).
 CCD double div_rd_fast_pos(double a, double b) {
   double y = rcp_seed(b);
-  double e = fnma_rd(b, y, 1.0);
-  y = fma_rd(y, e, y);
-  e = fnma_rd(b, y, 1.0);
+  const double e = fnma_rd(b, y, 1.0);
   y = fma_rd(y, e, y);
   double q = mul_rd(a, y);
-  double r = fnma_rd(b, q, a);
-  q = fma_rd(r, y, q);
-  // sign(a - q*b) == sign(a/b - q); one FMA gives it exactly here.
-  r = fnma_rd(q, b, a);
-  if (r < 0.0) {
-    q = next_down(q);
-    r = fnma_rd(q, b, a);
-    if (r < 0.0) q = next_down(q);
-  } else {
-    double qn = next_up(q);
-    double rn = fnma_rd(qn, b, a);
-    if (rn >= 0.0) {
-      q = qn;
-      qn = next_up(q);
-      rn = fnma_rd(qn, b, a);
-      if (rn >= 0.0) q = qn;
-    }
-  }
-  return q;
+  const double r0 = fnma_rd(b, q, a);
+  q = fma_rd(r0, y, q);
+  const double qu = next_up(q);
+  const double r = fnma_rd(q, b, a), ru = fnma_rd(qu, b, a);
+  return r < 0.0 ? next_down(q) : (ru >= 0.0 ? qu : q);
 }
 
 static __device__ __noinline__ double div_rd_slow(double a, double b);
@@ -237,35 +225,24 @@ static __device__ __noinline__ double div_rd_slow(double a, double b) {
 }
 
 // --------------------------------------------------------------------- sqrt
-// Round-down square root for x in [2^-960, DBL_MAX].
+// Round-down square root for x in [2^-960, DBL_MAX].  One Goldschmidt step
+// from the ~2^-25 rsq seed gives g and h = 1/(2 sqrt x) to about 2^-49, one
+// Newton step then leaves g + d*h within 2^-48 ulp of sqrt(x), so g is
+// RD(sqrt x) or a neighbour of it (a non-square's root lies at least 2^-54
+// ulp from every double, so both neighbours can occur, rarely): one residual
+// test each way settles it (sign(x - g*g) == sign(sqrt(x) - g); g > 0).
 CCD double sqrt_rd_fast(double x) {
-  double y = rsq_seed(x);
+  const double y = rsq_seed(x);
   double g = mul_rd(x, y);
   double h = mul_rd(y, 0.5);
-  double r = fnma_rd(g, h, 0.5);
+  const double r = fnma_rd(g, h, 0.5);
   g = fma_rd(g, r, g);
   h = fma_rd(h, r, h);
-  double d = fnma_rd(g, g, x);
-  g = fma_rd(d, h, g);
-  d = fnma_rd(g, g, x);
-  g = fma_rd(d, h, g);
-  // sign(x - g*g) == sign(sqrt(x) - g)
-  d = fnma_rd(g, g, x);
-  if (d < 0.0) {
-    g = next_down(g);
-    d = fnma_rd(g, g, x);
-    if (d < 0.0) g = next_down(g);
-  } else {
-    double gn = next_up(g);
-    double dn = fnma_rd(gn, gn, x);
-    if (dn >= 0.0) {
-      g = gn;
-      gn = next_up(g);
-      dn = fnma_rd(gn, gn, x);
-      if (dn >= 0.0) g = gn;
-    }
-  }
-  return g;
+  const double d0 = fnma_rd(g, g, x);
+  g = fma_rd(d0, h, g);
+  const double gu = from_bits(bits(g) + 1u);
+  const double d = fnma_rd(g, g, x), du = fnma_rd(gu, gu, x);
+  return d < 0.0 ? from_bits(bits(g) - 1u) : (du >= 0.0 ? gu : g);
 }
 
 static __device__ __noinline__ double sqrt_rd_slow(double x);
