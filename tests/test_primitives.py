@@ -108,6 +108,37 @@ def test_gpu_primitives_match_oracle_random(prover):
 
 
 @pytest.mark.gpu
+def test_gpu_div_sqrt_near_exact_results(prover):
+    """Operands whose quotient or root lies within a few ulps of a double (or
+    on one): the cases where a directed div/sqrt must step its Newton result
+    by one ulp.  Near squares r*r +- k ulp, exact squares, near-exact
+    quotients (q*b rounded, +- k ulp) and exact ones, both signs."""
+    rng = np.random.default_rng(777)
+    n = 6000
+    r = rng.uniform(1, 2, n) * np.exp2(rng.integers(-200, 200, n).astype(float))
+    sq = r * r
+    k = rng.integers(-3, 4, n)
+    sq = (sq.view(np.int64) + k).view(np.float64)
+    q = rng.uniform(1, 2, n) * np.exp2(rng.integers(-100, 100, n).astype(float))
+    b = rng.uniform(1, 2, n) * np.exp2(rng.integers(-100, 100, n).astype(float))
+    b[: n // 3] = rng.integers(1, 1 << 20, n // 3).astype(float)   # exact small quotients below
+    q[: n // 3] = rng.integers(1, 1 << 30, n // 3).astype(float)
+    a = q * b
+    a = (a.view(np.int64) + rng.integers(-2, 3, n)).view(np.float64)
+    sgn_a, sgn_b = rng.choice([-1.0, 1.0], n), rng.choice([-1.0, 1.0], n)
+    a, b = a * sgn_a, b * sgn_b
+    # columns 0..5 use (a, b); the sqrt columns use a: run the roots as a separate batch
+    got_div = prover.prim_selftest(a, b)
+    got_sqrt = prover.prim_selftest(sq, b)
+    exp_div = oracle_table(a, b)
+    exp_sqrt = oracle_table(sq, b)
+    bad = ~same_bits(got_div[:, 4:6], exp_div[:, 4:6])
+    assert not bad.any(), f"{bad.sum()} division mismatches"
+    bad = ~same_bits(got_sqrt[:, 6:8], exp_sqrt[:, 6:8])
+    assert not bad.any(), f"{bad.sum()} sqrt mismatches"
+
+
+@pytest.mark.gpu
 def test_gpu_round_nearest_helpers(prover):
     """Bisection midpoints, fused get_subinterval steps and the subinterval
     width must be the round-to-nearest results (checked against the oracle)."""
