@@ -145,6 +145,19 @@ static __device__ __noinline__ void split_largest_r7(const Konst& K, iv& r7, dou
     return;
   }
   double lo = r7.lo, hi = r7.hi;
+#if CC_EXACT_SHORTCUTS
+  // split_largest_works passes, then fails, as its r7 bound grows (for
+  // finite operands): bound_crit_ratio is non-decreasing in the largest
+  // disk's bound (every candidate ratio grows with it and feasibility only
+  // gets lost), so the remaining weight shrinks, and the size-bound
+  // can_recurse is non-decreasing in the weight and non-increasing in the
+  // largest disk.
+  if (lo >= 0.0 && lo < hi && hi < __builtin_inf() && __builtin_isfinite(wrem) && __builtin_isfinite(wf) &&
+      __builtin_isfinite(hf) && __builtin_isfinite(ws) && __builtin_isfinite(hs)) {
+    tighten_lo(r7, monotone_last_pass(lo, hi, [&](double m) { return split_largest_works(K, m, wrem, wf, hf, ws, hs); }));
+    return;
+  }
+#endif
   for (int it = 0; it < kBisectCap; ++it) {
     double mid = rn_mid(lo, hi);
     if (mid <= lo || mid >= hi) {
@@ -176,6 +189,15 @@ static __device__ __noinline__ void split_recursion(const Konst& K, const cell7&
   tighten_lo(r7, recursion_bound(K, add_rd(wrem, -add_ru(need, r7.hi)), ws, hs));
   if (empty(r7)) return;
   double hi = r7.hi, lo = r7.lo;
+#if CC_EXACT_SHORTCUTS
+  // recursion_bound is non-decreasing in the weight it gets, which shrinks
+  // as mid grows, so "recursion_bound(...) >= mid" passes, then fails.
+  if (lo >= 0.0 && lo < hi && hi < __builtin_inf() && __builtin_isfinite(wrem) && __builtin_isfinite(need) &&
+      __builtin_isfinite(ws) && __builtin_isfinite(hs)) {
+    r7.lo = monotone_last_pass(lo, hi, [&](double m) { return !(recursion_bound(K, add_rd(wrem, -add_ru(need, m)), ws, hs) < m); });
+    return;
+  }
+#endif
   for (int it = 0; it < kBisectCap; ++it) {
     double mid = rn_mid(lo, hi);
     if (mid <= lo || mid >= hi) {

@@ -144,6 +144,30 @@ CCD bool nd_can_cover_no_small(double ub_w, double ub_h, const double* d, int n)
   }
   return htot >= ub_h || wtot >= ub_w;
 }
+#if CC_EXACT_SHORTCUTS
+// The reference's bisection loop `mid = RN((lo + hi) / 2); stop once mid is
+// not strictly inside (lo, hi); test(mid) ? lo = mid : hi = mid` keeps lo at
+// a passing midpoint (or its start) and hi at a failing one (or its start)
+// until no double lies between them.  For a test that passes, then fails,
+// on (lo, hi) it therefore ends at the largest passing double of (lo, hi),
+// or at lo if none passes.  That value is found here with one test at the
+// double above lo and a bisection of the bit patterns, at most 64 tests,
+// where the loop needs up to about 1075 when that value is tiny or lo itself.
+// Requires 0 <= lo < hi < inf.
+template <class Test>
+CCD double monotone_last_pass(double lo, double hi, Test test) {
+  uint64_t L = lo == 0.0 ? 0u : bits(lo), H = bits(hi);  // -0.0 orders as +0.0
+  if (H - L <= 1u || !test(from_bits(L + 1u))) return lo;
+  ++L;
+  while (H - L > 1u) {
+    const uint64_t M = L + (H - L) / 2u;
+    if (test(from_bits(M))) L = M;
+    else H = M;
+  }
+  return from_bits(L);
+}
+#endif
+
 // three_disks_can_cover: maximize_covered.cu:171-196
 CCD bool three_can_cover(double r1, double r2, double r3, double w, double h) {
   double hsq = mul_ru(h, h);
