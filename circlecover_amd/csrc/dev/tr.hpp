@@ -319,12 +319,32 @@ CCD bool strip_width_ok(double q1, double q2, double q3, double w) {
 // Safeguarded Newton (bisection where R <= 0 or a step leaves the bracket)
 // finds the threshold u*, and w = sqrt(q1 - u*^2) is returned.  Only a
 // starting point: arithmetic rounding here does not affect any result.
+// Approximate square roots and quotient for the estimate: a Goldschmidt step
+// from the rsq seed (relative error about 2^-49), plus a Newton step where
+// the result must be good to an ulp or two; a Newton step from the rcp seed.
+// est_sqrt(0) = 0; a zero divisor gives inf or NaN, which the caller rejects.
+CCD double est_sqrt(double x, bool refine = false) {
+  const double y = rsq_seed(x);
+  double g = mul_rd(x, y), h = mul_rd(y, 0.5);
+  const double r = fnma_rd(g, h, 0.5);
+  g = fma_rd(g, r, g);
+  if (refine) {
+    h = fma_rd(h, r, h);
+    g = fma_rd(fnma_rd(g, g, x), h, g);
+  }
+  return x > 0.0 ? g : 0.0;
+}
+CCD double est_div(double n, double d) {
+  double y = rcp_seed(d);
+  y = fma_rd(y, fnma_rd(d, y, 1.0), y);
+  return mul_rd(n, y);
+}
 CCD double strip_width_estimate(double q1, double q2, double q3, double a, double b) {
-  const double u_hi = sqrt_rd(fmax(add_rd(q1, -mul_rd(a, a)), 0.0));
+  const double u_hi = est_sqrt(fmax(add_rd(q1, -mul_rd(a, a)), 0.0));
   const double h_b = fmax(add_rd(q1, -mul_rd(b, b)), 0.0);
   const double qm = fmin(q2, q3);
-  const double u_w = qm < 1.0 ? add_rd(1.0, -sqrt_rd(qm)) : 0.0;
-  double x0 = fmax(sqrt_rd(h_b), u_w), x1 = u_hi, x = x0;
+  const double u_w = qm < 1.0 ? add_rd(1.0, -est_sqrt(qm)) : 0.0;
+  double x0 = fmax(est_sqrt(h_b), u_w), x1 = u_hi, x = x0;
   bool first = true;
   for (int it = 0; it < 32; ++it) {
     const double v = add_rd(1.0, -x);
@@ -340,7 +360,7 @@ CCD double strip_width_estimate(double q1, double q2, double q3, double a, doubl
     }
     first = false;
     const double dF = fma_rd(8.0 * v, add_rd(w2, w3), mul_rd(4.0 * R, fma_rd(2.0, v, x)));
-    double nx = add_rd(x, -(F / dF));  // any nearby quotient will do
+    double nx = add_rd(x, -est_div(F, dF));
     if (R <= 0.0 || !(nx >= x0 && nx <= x1)) {
       nx = mul_rd(0.5, add_rd(x0, x1));
     } else if (fabs(add_rd(nx, -x)) <= 0x1p-51) {  // u is O(1): stop at a few ulps
@@ -350,7 +370,7 @@ CCD double strip_width_estimate(double q1, double q2, double q3, double a, doubl
     x = nx;
     if (add_rd(x1, -x0) <= 0x1p-51) break;
   }
-  return sqrt_rd(fmax(add_rd(q1, -mul_rd(x, x)), 0.0));
+  return est_sqrt(fmax(add_rd(q1, -mul_rd(x, x)), 0.0), true);
 }
 
 // strip_width_ok, also returning the covered width s = sqrt(w2) + sqrt(w3)
