@@ -268,19 +268,20 @@ CCD bool r1_r2_stacked(const Konst& K, iv la, iv W, iv r1, iv r2, double ub_r3) 
   double top_min = w_top < top_h ? w_top : top_h;
   double bot_min = w_bot < side ? w_bot : side;
   double R3 = (W - r1 - r2).lo;
+  // The three cases differ only in what the recursion test gets; one call
+  // site keeps a wave from running the test once per case.
+  double need, cw = w_bot, ch = side;
   if (disk_fits(ub_r3, top_min)) {
-    double need = add_ru(mul_ru(kRsbRatio, mul_ru(w_top, top_h)), ub_r3);
-    double left = add_rd(R3, -need);
-    return left > 0.0 && can_recurse(K, w_bot, side, ub_r3, left);
+    need = add_ru(mul_ru(kRsbRatio, mul_ru(w_top, top_h)), ub_r3);
+  } else if (disk_fits(ub_r3, bot_min)) {
+    need = add_ru(mul_ru(kRsbRatio, mul_ru(w_bot, side)), ub_r3);
+    cw = w_top;
+    ch = top_h;
+  } else {
+    need = add_ru(weight_for_pt_hi(K, w_top, top_h), ub_r3);
   }
-  if (disk_fits(ub_r3, bot_min)) {
-    double need = add_ru(mul_ru(kRsbRatio, mul_ru(w_bot, side)), ub_r3);
-    double left = add_rd(R3, -need);
-    return left > 0.0 && can_recurse(K, w_top, top_h, ub_r3, left);
-  }
-  double need = add_ru(weight_for_pt_hi(K, w_top, top_h), ub_r3);
   double left = add_rd(R3, -need);
-  return left > 0.0 && can_recurse(K, w_bot, side, ub_r3, left);
+  return left > 0.0 && can_recurse(K, cw, ch, ub_r3, left);
 }
 
 // r1_r2_r3_strategy_horizontal: r1_r2_r3_strategies.cu:28-46
