@@ -24,7 +24,10 @@ configurations measured in the same run (not part of `value`):
                64/32/32/32 pieces, three bisection levels;
   fp32_prefilter  the FP32 pre-filter on the 2^22 boxes (first-success order);
   ranker       the learned criterion ranker trained on 2^20 boxes of another
-               seed, ranked vs plain first-success evaluation of the 2^22 boxes.
+               seed, ranked vs plain first-success evaluation of the 2^22 boxes;
+  full_proof   both of the reference's proofs end to end through the drop-in
+               executable (`circlecover_prover --only tr|rsb`), wall time and
+               number of critical intervals left.
 """
 from __future__ import annotations
 
@@ -134,7 +137,7 @@ def measured_traffic() -> dict | None:
             vals = {}
             for r in rows:
                 name = r.get("Kernel_Name", "")
-                batch = ("ccrd_c2_eval", "ccrd_c2_strip_deferred", "cc_zero_u64")  # one batch's launches
+                batch = ("ccrd_c2_eval", "ccrd_c2_strip_deferred", "ccrd_c2_r1_pass", "cc_zero_u64")  # one batch's launches
                 key = "copy" if "cc_copy_u4" in name else "c2" if any(k in name for k in batch) else None
                 if key and r.get("Counter_Name", "") == counter:
                     vals[key] = vals.get(key, 0.0) + float(r["Counter_Value"])
@@ -210,6 +213,28 @@ def extended(p, soa: np.ndarray, mask: np.ndarray) -> dict:
     return out
 
 
+def full_proof() -> dict:
+    """Both of the reference's proofs, end to end, through the drop-in
+    executable (reference constants: tight_rectangle 256 lambda launches of
+    128^4 pieces plus its refinement generations; rectangle_size_bound the
+    256x128x128 grid with 64^4 pieces per cell plus one generation).  Wall
+    time from process start to exit, including context set-up; the critical
+    intervals are counted from the stdout report, whose rows sit between the
+    banner and the closing rule."""
+    import subprocess
+
+    exe = os.path.join(ROOT, "circlecover_amd", "circlecover_prover")
+    out = {}
+    for proof in ("tr", "rsb"):
+        t = time.perf_counter()
+        r = subprocess.run([exe, "--only", proof], capture_output=True, text=True, timeout=600)
+        wall = time.perf_counter() - t
+        lines = [ln for ln in r.stdout.splitlines() if ln.strip()]
+        rows = max(0, len(lines) - 2) if r.returncode == 0 else None
+        out[proof] = {"wall_s": wall, "exit_status": r.returncode, "critical_intervals": rows}
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -219,6 +244,7 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-oracle baseline leg")
     ap.add_argument("--core-only", action="store_true", help="skip the other configurations (workloads)")
     ap.add_argument("--no-traffic", action="store_true", help="skip the HBM counter runs (traffic = null)")
+    ap.add_argument("--no-full-proof", action="store_true", help="skip the end-to-end proofs (workloads.full_proof)")
     ap.add_argument("--traffic-probe", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.traffic_probe:
@@ -286,6 +312,8 @@ def main() -> int:
                      "kernel": "ccrd_c2_eval"},
     }
     if workloads is not None:
+        if not args.no_full_proof:
+            workloads["full_proof"] = full_proof()
         out["workloads"] = workloads
     if not args.no_cpu:
         threads = len(os.sched_getaffinity(0))

@@ -25,7 +25,10 @@ static constexpr int kStripCap = 4;
 // shortcut of r123_cover_strip_capped; the few boxes that would need a long
 // plain bisection first are appended to `deferred` (count in
 // *n_deferred, which must be 0 at launch) with bit 4 left clear, and
-// ccrd_c2_strip_deferred completes them.
+// ccrd_c2_strip_deferred completes them.  Criterion 0 runs only its first
+// strategy here, which proves most boxes; ccrd_c2_r1_pass tries the others
+// on the boxes whose bit 0 is clear, so that a wave does not run them for
+// the one or two of its lanes that need them.
 extern "C" __global__ void __launch_bounds__(256) ccrd_c2_eval(const double* __restrict__ soa, uint64_t n,
                                                                 uint64_t ld, const Konst* __restrict__ kp,
                                                                 uint32_t* __restrict__ mask,
@@ -48,7 +51,7 @@ extern "C" __global__ void __launch_bounds__(256) ccrd_c2_eval(const double* __r
 
   uint32_t m = 0;
   Trap tp{0};
-  if (trc::r1_strategies(K, la, W, r1, r2.hi)) m |= 1u << 0;
+  if (trc::r1_strip(K, la, W, r1, r2.hi)) m |= 1u << 0;
   if (trc::r1_r2_simple(K, la, W, r1, r2, r3.hi, tp)) m |= 1u << 1;
   if (tp.flag) m |= 1u << 17;
   if (trc::r1_r2_stacked(K, la, W, r1, r2, r3.hi)) m |= 1u << 2;
@@ -148,5 +151,49 @@ extern "C" __global__ void __launch_bounds__(256) ccrd_c2_strip_deferred(const d
       mask[i] = m;
       first[i] = (uint8_t)__builtin_ctz(m & 0x3ffu);
     }
+  }
+}
+
+// Criterion 0 second pass: r1_strategies' remaining strategies for the boxes
+// whose bit 0 is still clear, then bit 0 of mask and first.  Each block
+// gathers the pending boxes of a 2048-box chunk into LDS (a ballot and one
+// LDS atomic per wave) and works on them densely.
+static constexpr uint32_t kR1Chunk = 2048;
+
+extern "C" __global__ void __launch_bounds__(256) ccrd_c2_r1_pass(const double* __restrict__ soa, uint64_t n,
+                                                                   uint64_t ld, const Konst* __restrict__ kp,
+                                                                   uint32_t* __restrict__ mask,
+                                                                   uint8_t* __restrict__ first) {
+  __shared__ uint32_t list[kR1Chunk];
+  __shared__ uint32_t count;
+  const Konst K = *kp;
+  for (uint64_t base = (uint64_t)blockIdx.x * kR1Chunk; base < n; base += (uint64_t)gridDim.x * kR1Chunk) {
+    if (threadIdx.x == 0) count = 0;
+    __syncthreads();
+    for (uint32_t j = threadIdx.x; j < kR1Chunk; j += blockDim.x) {  // kR1Chunk is a multiple of the block
+      const uint64_t i = base + j;
+      const bool keep = i < n && (mask[i] & 1u) == 0;
+      const uint64_t b = __ballot(keep);
+      if (b) {
+        const int lane = __lane_id();
+        const int leader = __ffsll((unsigned long long)b) - 1;
+        uint32_t at = 0;
+        if (lane == leader) at = atomicAdd(&count, (uint32_t)__popcll(b));
+        at = __shfl(at, leader);
+        if (keep) list[at + (uint32_t)__popcll(b & ((1ull << lane) - 1ull))] = j;
+      }
+    }
+    __syncthreads();
+    const uint32_t total = count;
+    for (uint32_t k = threadIdx.x; k < total; k += blockDim.x) {
+      const uint64_t i = base + list[k];
+      const iv la{soa[0 * ld + i], soa[1 * ld + i]};
+      const iv r1{soa[2 * ld + i], soa[3 * ld + i]};
+      if (trc::r1_rest(K, la, trc::weight_for(K, la), r1, soa[5 * ld + i])) {
+        mask[i] |= 1u;
+        first[i] = 0;
+      }
+    }
+    __syncthreads();
   }
 }

@@ -192,14 +192,18 @@ CCD double two_disk_height(double r1, double r2, double ub_w) {
 }
 
 // r1_strategies: r1_strategies.cu:27-87.  W = required_weight_for(la).
-CCD bool r1_strategies(const Konst& K, iv la, iv W, iv r1, double ub_r2) {
-  double sq_w = sqrt_rd(mul_rd(2.0, r1.lo));
+// r1_strategies = r1_strip || r1_rest: the first strategy (a full-height
+// strip) proves most boxes, so the batch kernel runs it alone and leaves the
+// others to a second pass over the boxes it does not prove.
+CCD bool r1_strip(const Konst& K, iv la, iv W, iv r1, double ub_r2) {
   double strip_w = add_rd(mul_rd(4.0, r1.lo), -1.0);
+  if (!(strip_w > 0.0)) return false;
+  strip_w = sqrt_rd(strip_w);
+  return can_recurse(K, add_ru(la.hi, -strip_w), 1.0, ub_r2, (W - r1).lo);
+}
+CCD bool r1_rest(const Konst& K, iv la, iv W, iv r1, double ub_r2) {
+  double sq_w = sqrt_rd(mul_rd(2.0, r1.lo));
   const double rem = (W - r1).lo;
-  if (strip_w > 0.0) {
-    strip_w = sqrt_rd(strip_w);
-    if (can_recurse(K, add_ru(la.hi, -strip_w), 1.0, ub_r2, rem)) return true;
-  }
   if (sq_w < 1.0) {
     double wright = add_ru(la.hi, -sq_w);
     double minright = wright < 1.0 ? wright : 1.0;
@@ -222,6 +226,9 @@ CCD bool r1_strategies(const Konst& K, iv la, iv W, iv r1, double ub_r2) {
     }
   }
   return false;
+}
+CCD bool r1_strategies(const Konst& K, iv la, iv W, iv r1, double ub_r2) {
+  return r1_strip(K, la, W, r1, ub_r2) || r1_rest(K, la, W, r1, ub_r2);
 }
 
 // manually_proved_two_pocket: r1_r2_strategies.cu:29-44

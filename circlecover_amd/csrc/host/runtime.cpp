@@ -191,6 +191,12 @@ int cc_eval_batch_async(cc_ctx* ctx, int prover, int precision, const double* d_
   // stride over more.
   CC_HIP(ctx, launch(f_rest, dim3(1024), dim3(block), 0, ctx->stream, d_soa, ld64, ctx->d_konst,
                      (const uint32_t*)d_list, (const unsigned long long*)d_cnt, d_mask, d_first));
+  // Criterion 0's remaining strategies: one 2048-box chunk per block.
+  hipFunction_t f_r1;
+  CC_HIP(ctx, kernel(ctx, "ccrd_c2_r1_pass", &f_r1));
+  const uint64_t r1_chunks = (n + 2047) / 2048;
+  CC_HIP(ctx, launch(f_r1, dim3((unsigned)(r1_chunks < 8192 ? r1_chunks : 8192)), dim3(block), 0, ctx->stream, d_soa,
+                     n64, ld64, ctx->d_konst, d_mask, d_first));
   return CC_OK;
 }
 
