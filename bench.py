@@ -250,7 +250,7 @@ def main() -> int:
     if args.traffic_probe:
         return traffic_probe()
     if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
+        print(f"bench.py: measures one GPU; --gpus {args.gpus} is not supported "
               "(cells are independent: each GPU would take a slice of the boxes or initial cells)", file=sys.stderr)
         return 2
 

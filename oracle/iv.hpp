@@ -285,7 +285,7 @@ static inline Point center(const Point& a, const Point& b) {
 // i-th of n near-equal pieces of [lb,ub], the last snapped to ub
 // (reference Prover/subintervals.hpp:32-43).  The reference's `lb + diff*i`
 // is FMA-contracted by nvcc's default device code generation; the fused form
-// is used on both sides of this build.
+// is used here and in the GPU kernels alike.
 static inline IV get_subinterval(const IV& x, int i, int n) {
   double diff = rn_div(rn_sub(x.ub, x.lb), (double)n);
   double l = rn_fma(diff, (double)i, x.lb);
