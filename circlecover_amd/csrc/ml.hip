@@ -192,11 +192,38 @@ extern "C" __global__ void __launch_bounds__(256) ccml_grad_adam(const float* __
 // ---------------------------------------------------------------- inference
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef int i32x2 __attribute__((ext_vector_type(2)));
 
-// Batched forward on MFMA.  Operand layouts of v_mfma_f32_32x32x8bf16_1k for
-// lane l (g = l / 32): A[m][k] with m = l % 32, k = 4g + e (e = element
-// 0..3); B[k][n] with n = l % 32, k = 4g + e; D[m][n] with n = l % 32,
-// m = 8 (r / 4) + 4g + (r % 4) for accumulator r = 0..15.
+// Two floats to a packed BF16 pair, rounding to nearest even (one
+// v_cvt_pk_bf16_f32 on gfx950); with relu, negative inputs become 0 first.
+template <bool relu>
+__device__ __forceinline__ int bf16_pair(float a, float b) {
+  if (relu) {
+    a = __builtin_fmaxf(a, 0.f);
+    b = __builtin_fmaxf(b, 0.f);
+  }
+  const f32x2 v = {a, b};
+  return __builtin_bit_cast(int, __builtin_convertvector(v, bf16x2));
+}
+template <bool relu>
+__device__ __forceinline__ bf16x4 bf16_quad(float a, float b, float c, float d) {
+  const i32x2 w = {bf16_pair<relu>(a, b), bf16_pair<relu>(c, d)};
+  return __builtin_bit_cast(bf16x4, w);
+}
+
+// Batched forward on MFMA, with activations kept in registers.  Operand
+// layouts of v_mfma_f32_32x32x8bf16_1k (D = A B + C) for lane l, g = l / 32,
+// j = l % 32: A[i][k] with i = j, k = 4g + e (element e = 0..3); B[k][j'] with
+// j' = j, k = 4g + e; D[i][j'] with j' = j, i = 8 (r / 4) + 4g + (r % 4) for
+// accumulator r = 0..15.  Each layer is computed transposed, D = W^T X^T
+// (rows: units, columns: the wave's 32 boxes), because then the rows a lane
+// holds of one layer's output, units 8s + 4g + e for step s, are exactly the
+// k-slice 8s + 4g + e that it supplies as B for the next layer's step s:
+// bias, ReLU and the BF16 rounding are applied in registers and no
+// activation goes through LDS.  The biases enter as the accumulators' start;
+// float-to-BF16 conversions use the hardware pair conversion.
 // wb: BF16 weights as [W1 (8x64) | W2 (64x64) | W3 (64x10)] row-major (k, n);
 // bias: FP32 [b1 | b2 | b3].  top1[i] = argmax logit (lowest index on
 // ties); logits (may be null) receives the 10 logits of each box.
@@ -206,107 +233,125 @@ extern "C" __global__ void __launch_bounds__(256) ccml_rank(const double* __rest
                                                              const float* __restrict__ bias,
                                                              uint8_t* __restrict__ top1,
                                                              float* __restrict__ logits) {
-  // per wave: one activation tile (bf16) and one logits tile; rows padded
-  // by one dword so that lanes reading one row each hit distinct banks
-  __shared__ uint16_t act[4][32][kH + 2];
-  __shared__ float out[4][32][33];
-  const int wv = threadIdx.x >> 6, l = threadIdx.x & 63, g = l >> 5, col = l & 31;
+  // biases in LDS (read four consecutive units at a time as accumulator
+  // starts), so that the weights and two tiles of work fit in the registers
+  // of two waves per SIMD
+  __shared__ float sb[2 * kH + 16];
+  for (int k = threadIdx.x; k < 2 * kH + 16; k += blockDim.x) sb[k] = k < 2 * kH + kOut ? bias[k] : 0.f;
+  __syncthreads();
+  const int l = threadIdx.x & 63, g = l >> 5, col = l & 31, wv = threadIdx.x >> 6;
   const uint16_t* W1 = wb;
   const uint16_t* W2 = wb + kIn * kH;
   const uint16_t* W3 = W2 + kH * kH;
-  // B operands of every MFMA and this lane's biases stay in registers for
-  // the whole kernel (W1: 2 tiles, W2: 8 k-steps x 2 tiles, W3: 8 k-steps)
-  bf16x4 b1[2], b2[8][2], b3[8];
-  float bias1[2], bias2[2], bias3;
+  bf16x4 a1[2], a2[8][2], a3[8];
 #pragma unroll
-  for (int t = 0; t < 2; ++t) {
+  for (int t = 0; t < 2; ++t)
 #pragma unroll
-    for (int e = 0; e < 4; ++e) b1[t][e] = (short)W1[(4 * g + e) * kH + 32 * t + col];
-    bias1[t] = bias[32 * t + col];
-    bias2[t] = bias[kH + 32 * t + col];
-  }
+    for (int e = 0; e < 4; ++e) a1[t][e] = (short)W1[(4 * g + e) * kH + 32 * t + col];
 #pragma unroll
   for (int s = 0; s < 8; ++s) {
 #pragma unroll
     for (int t = 0; t < 2; ++t)
 #pragma unroll
-      for (int e = 0; e < 4; ++e) b2[s][t][e] = (short)W2[(8 * s + 4 * g + e) * kH + 32 * t + col];
+      for (int e = 0; e < 4; ++e) a2[s][t][e] = (short)W2[(8 * s + 4 * g + e) * kH + 32 * t + col];
 #pragma unroll
-    for (int e = 0; e < 4; ++e) b3[s][e] = (short)(col < kOut ? W3[(8 * s + 4 * g + e) * kOut + col] : 0);
+    for (int e = 0; e < 4; ++e) a3[s][e] = (short)(col < kOut ? W3[(8 * s + 4 * g + e) * kOut + col] : 0);
   }
-  bias3 = col < kOut ? bias[2 * kH + col] : 0.f;
+  // accumulator start r..r+3 of the units 8q + 4g + (0..3), q = r / 4
+  auto start = [&](int off, f32x16& c) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const float4 v = *reinterpret_cast<const float4*>(&sb[off + 8 * q + 4 * g]);
+      c[4 * q] = v.x;
+      c[4 * q + 1] = v.y;
+      c[4 * q + 2] = v.z;
+      c[4 * q + 3] = v.w;
+    }
+  };
   const uint64_t tiles = (n + 31) / 32;
-  for (uint64_t tile = (uint64_t)blockIdx.x * 4 + wv; tile < tiles; tile += (uint64_t)gridDim.x * 4) {
+  const uint64_t stride = (uint64_t)gridDim.x * 4;
+  uint64_t tile = (uint64_t)blockIdx.x * 4 + wv;
+  // this lane's two dimensions (2g, 2g + 1) of its box, loaded one tile ahead
+  double v[4] = {0.0, 0.0, 0.0, 0.0};
+  auto load = [&](uint64_t t) {
+    const uint64_t i = t * 32 + col;
+    if (t < tiles && i < n)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) v[e] = soa[(4 * g + e) * ld + i];
+  };
+  load(tile);
+  for (; tile < tiles; tile += stride) {
     const uint64_t i = tile * 32 + col;
     const bool live = i < n;
-    // layer 1: A = features (k = 4g..4g+3 of box i)
-    bf16x4 a;
+    // layer 1: B = features 4g..4g+3 of box i (centre and log2 width of
+    // dimensions 2g and 2g + 1)
+    float f[4];
 #pragma unroll
-    for (int e = 0; e < 4; ++e) a[e] = (short)(live ? bf16_bits(feature<true>(soa, ld, i, 4 * g + e, norm)) : 0);
+    for (int d = 0; d < 2; ++d) {
+      const double lo = v[2 * d], hi = v[2 * d + 1];
+      const double w = hi - lo;
+      const float c = (float)(0.5 * (lo + hi));
+      const float lw = w > 0x1p-60 ? log2f((float)w) : -60.0f;
+      const int k = 4 * g + 2 * d;
+      f[2 * d] = live ? (c - norm[k]) / norm[kIn + k] : 0.f;
+      f[2 * d + 1] = live ? (lw - norm[k + 1]) / norm[kIn + k + 1] : 0.f;
+    }
+    const bf16x4 x = bf16_quad<false>(f[0], f[1], f[2], f[3]);
+    load(tile + stride);
+    f32x16 h1[2];
 #pragma unroll
     for (int t = 0; t < 2; ++t) {
-      f32x16 acc = {};
-      acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b1[t], acc, 0, 0, 0);
-#pragma unroll
-      for (int r = 0; r < 16; ++r) {
-        const int m = 8 * (r >> 2) + 4 * g + (r & 3);
-        const float z = acc[r] + bias1[t];
-        act[wv][m][32 * t + col] = bf16_bits(z > 0.f ? z : 0.f);
-      }
+      start(32 * t, h1[t]);
+      h1[t] = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a1[t], x, h1[t], 0, 0, 0);
     }
-    __builtin_amdgcn_s_waitcnt(0);
-    __builtin_amdgcn_wave_barrier();
-    // layer 2: K = 64 in 8 steps of 8
-    f32x16 acc2[2] = {{}, {}};
+    // layer 2: step s takes units 8s + 4g + e, held in h1[s / 4][4 (s % 4) + e]
+    f32x16 h2[2];
+    start(kH, h2[0]);
+    start(kH + 32, h2[1]);
 #pragma unroll
     for (int s = 0; s < 8; ++s) {
-      bf16x4 a2;
+      const f32x16& h = h1[s >> 2];
+      const int r = 4 * (s & 3);
+      const bf16x4 b = bf16_quad<true>(h[r], h[r + 1], h[r + 2], h[r + 3]);
 #pragma unroll
-      for (int e = 0; e < 4; ++e) a2[e] = (short)act[wv][col][8 * s + 4 * g + e];
-#pragma unroll
-      for (int t = 0; t < 2; ++t) acc2[t] = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a2, b2[s][t], acc2[t], 0, 0, 0);
+      for (int t = 0; t < 2; ++t) h2[t] = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a2[s][t], b, h2[t], 0, 0, 0);
     }
-    __builtin_amdgcn_s_waitcnt(0);
-    __builtin_amdgcn_wave_barrier();
-#pragma unroll
-    for (int t = 0; t < 2; ++t)
-#pragma unroll
-      for (int r = 0; r < 16; ++r) {
-        const int m = 8 * (r >> 2) + 4 * g + (r & 3);
-        const float z = acc2[t][r] + bias2[t];
-        act[wv][m][32 * t + col] = bf16_bits(z > 0.f ? z : 0.f);
-      }
-    __builtin_amdgcn_s_waitcnt(0);
-    __builtin_amdgcn_wave_barrier();
-    // layer 3: N = 10 padded to 32
-    f32x16 acc3 = {};
+    // layer 3: logits of classes 8 (r / 4) + 4g + (r % 4) of box i
+    f32x16 z3;
+    start(2 * kH, z3);
 #pragma unroll
     for (int s = 0; s < 8; ++s) {
-      bf16x4 a3;
-#pragma unroll
-      for (int e = 0; e < 4; ++e) a3[e] = (short)act[wv][col][8 * s + 4 * g + e];
-      acc3 = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a3, b3[s], acc3, 0, 0, 0);
+      const f32x16& h = h2[s >> 2];
+      const int r = 4 * (s & 3);
+      z3 = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a3[s], bf16_quad<true>(h[r], h[r + 1], h[r + 2], h[r + 3]), z3, 0,
+                                                  0, 0);
     }
+    // classes here: g = 0 holds 0..3 and 8, 9; g = 1 holds 4..7.  Best of
+    // each lane in increasing class order, then the pair of lanes l, l ^ 32
+    // combined (larger logit; on equal logits the lower class).
+    float bv = z3[0];
+    int best = 4 * g;
 #pragma unroll
-    for (int r = 0; r < 16; ++r) {
-      const int m = 8 * (r >> 2) + 4 * g + (r & 3);
-      out[wv][m][col] = acc3[r] + bias3;
+    for (int r = 1; r < 6; ++r) {
+      const int c = 8 * (r >> 2) + 4 * g + (r & 3);
+      if (c < kOut && z3[r] > bv) {
+        bv = z3[r];
+        best = c;
+      }
     }
-    __builtin_amdgcn_s_waitcnt(0);
-    __builtin_amdgcn_wave_barrier();
-    if (g == 0 && live) {
-      int best = 0;
-      float bv = out[wv][col][0];
-      for (int c = 1; c < kOut; ++c)
-        if (out[wv][col][c] > bv) {
-          bv = out[wv][col][c];
-          best = c;
+    const float ov = __shfl_xor(bv, 32);
+    const int ob = __shfl_xor(best, 32);
+    if (ov > bv || (ov == bv && ob < best)) best = ob;
+    if (live) {
+      if (g == 0) top1[i] = (uint8_t)best;
+      if (logits) {
+#pragma unroll
+        for (int r = 0; r < 6; ++r) {
+          const int c = 8 * (r >> 2) + 4 * g + (r & 3);
+          if (c < kOut) logits[i * kOut + c] = z3[r];
         }
-      top1[i] = (uint8_t)best;
-      if (logits)
-        for (int c = 0; c < kOut; ++c) logits[i * kOut + c] = out[wv][col][c];
+      }
     }
-    __builtin_amdgcn_wave_barrier();
   }
 }
 
