@@ -24,7 +24,8 @@ configurations measured in the same run (not part of `value`):
                64/32/32/32 pieces, three bisection levels;
   fp32_prefilter  the FP32 pre-filter on the 2^22 boxes (first-success order);
   ranker       the learned criterion ranker trained on 2^20 boxes of another
-               seed, ranked vs plain first-success evaluation of the 2^22 boxes;
+               seed; first-success evaluation of the 2^22 boxes in plain order,
+               plain cascade, ranked and ranked cascade (same proven set);
   full_proof   both of the reference's proofs end to end through the drop-in
                executable (`circlecover_prover --only tr|rsb`), wall time and
                number of critical intervals left.
@@ -191,25 +192,41 @@ def extended(p, soa: np.ndarray, mask: np.ndarray) -> dict:
                                                                   float(np.percentile(r_cr, 99))],
                                  "r1_strategies.remaining_weight": [float(np.median(r_rem)),
                                                                     float(np.percentile(r_rem, 99))]}}
-    # learned ranker
+    # learned ranker: first-success evaluation of the 2^22 boxes in plain
+    # order, plain cascade (criterion 0 on every box, the rest on the boxes
+    # it leaves open), ranked, and ranked cascade (the ranker trained on, and
+    # applied to, only the boxes criterion 0 leaves open)
     train = cc.gen_c2_boxes(SEED + 1, 1 << 20)
-    tf, _, _, _ = p.eval_first(train)  # labels: first proving criterion in plain order
+    tf, _, _, _ = p.eval_first(train)
+    # labels: first proving criterion in plain order, 10 where none proves
+    tl = np.where(tf == 255, 10, tf).astype(np.uint8)
     t = time.perf_counter()
-    model, losses = p.ranker_train(train, tf, epochs=20, batch=4096, seed=1)
+    model, losses = p.ranker_train(train, tl, epochs=20, batch=4096, seed=1)
     t_train = time.perf_counter() - t
-    p.eval_first(soa, model)
+    tf_c = np.where(tl == 0, 255, tl).astype(np.uint8)
     t = time.perf_counter()
-    rf, re, _, sr = p.eval_first(soa, model)
-    t_r = time.perf_counter() - t
-    t = time.perf_counter()
-    pf, pe, _, sp1 = p.eval_first(soa)
-    t_pl = time.perf_counter() - t
-    out["ranker"] = {"train_s": t_train, "final_loss": float(losses[-1]), "train_boxes": int((tf < 10).sum()),
-                     "evals_per_proven_plain": sp1.evals_proven / sp1.proven,
-                     "evals_per_proven_ranked": sr.evals_proven / sr.proven,
-                     "device_ms_plain": sp1.eval_ms, "device_ms_ranked": sr.rank_ms + sr.eval_ms,
-                     "rank_ms": sr.rank_ms, "wall_s_ranked": t_r, "wall_s_plain": t_pl,
-                     "same_proven_set": bool(((rf < 10) == (pf < 10)).all())}
+    model_c, losses_c = p.ranker_train(train, tf_c, epochs=20, batch=4096, seed=1)
+    t_train_c = time.perf_counter() - t
+    variants = {}
+    proven_sets = []
+    for name, m, casc in (("plain", None, False), ("plain_cascade", None, True), ("ranked", model, False),
+                          ("ranked_cascade", model_c, True)):
+        p.eval_first(soa, m, cascade=casc)
+        t = time.perf_counter()
+        f, e, _, st = p.eval_first(soa, m, cascade=casc)
+        wall = time.perf_counter() - t
+        proven_sets.append(f < 10)
+        variants[name] = {"evals_per_proven": st.evals_proven / st.proven, "device_ms": st.rank_ms + st.eval_ms,
+                          "rank_ms": st.rank_ms, "eval_ms": st.eval_ms, "second_stage_boxes": int(st.second_stage),
+                          "wall_s": wall}
+    out["ranker"] = {"train_s": t_train, "final_loss": float(losses[-1]), "train_boxes": int((tl <= 10).sum()),
+                     "train_s_cascade": t_train_c, "final_loss_cascade": float(losses_c[-1]),
+                     "train_boxes_cascade": int((tf_c <= 10).sum()), "variants": variants,
+                     "evals_per_proven_plain": variants["plain"]["evals_per_proven"],
+                     "evals_per_proven_ranked": variants["ranked_cascade"]["evals_per_proven"],
+                     "device_ms_plain": variants["plain"]["device_ms"],
+                     "device_ms_ranked": variants["ranked_cascade"]["device_ms"],
+                     "same_proven_set": bool(all((x == proven_sets[0]).all() for x in proven_sets))}
     return out
 
 

@@ -11,13 +11,14 @@ using namespace cchost;
 
 namespace {
 
-constexpr int kIn = 8, kH = 64, kOut = 10, kRec = kIn + 4 * kH + kOut;
+// classes 0..9: the criterion that proves a box first; class 10: none does
+constexpr int kIn = 8, kH = 64, kOut = 11, kCrit = 10, kRec = kIn + 4 * kH + kOut;
 constexpr int kW1 = 0, kB1 = kIn * kH, kW2 = kB1 + kH, kB2 = kW2 + kH * kH, kW3 = kB2 + kH, kB3 = kW3 + kH * kOut;
 static_assert(kB3 + kOut == CC_RANKER_PARAMS, "parameter count");
 
 struct MlState {
   DevBuf soa, label, perm, rec, loss, params, m, v, grad, norm, wb, bias, top1, idx, counts, first, evals, trap, bstart,
-      partial;
+      partial, pend, list, stage, stage_id, packed;
 };
 constexpr int kSlice = 64;  // samples per partial gradient sum (ml.hip)
 
@@ -106,17 +107,23 @@ int upload_inference(cc_ctx* ctx, MlState& M, const cc_ranker* model) {
   return rc;
 }
 
-int rank_dev(cc_ctx* ctx, MlState& M, const void* d_soa, size_t n, float* d_logits) {
+// Ranks the positions [0, n) (n_dev not null: [0, *n_dev), at most n) of
+// the device SoA (stride ld): box i, or box list[i] when list is not null;
+// top1 (and logits) by position.  The kernel is persistent: at most 512
+// workgroups of four waves, the two waves per SIMD its registers allow, so
+// that each wave loads the weights once for several 32-box tiles.
+int rank_dev(cc_ctx* ctx, MlState& M, const void* d_soa, size_t ld, const uint32_t* list, size_t n,
+             const uint32_t* n_dev, float* d_logits) {
   hipFunction_t f;
   CC_HIP(ctx, kernel(ctx, "ccml_rank", &f));
   CC_HIP(ctx, M.top1.ensure(n + 1));
-  const uint64_t n64 = n, ld = n;
+  const uint64_t n64 = n, ld64 = ld;
   const uint64_t tiles = (n + 31) / 32;
   unsigned blocks = (unsigned)((tiles + 3) / 4);
-  if (blocks > 4096) blocks = 4096;
+  if (blocks > 512) blocks = 512;
   if (blocks == 0) blocks = 1;
-  CC_HIP(ctx, launch(f, dim3(blocks), dim3(256), 0, ctx->stream, d_soa, n64, ld, M.norm.p, M.wb.p, M.bias.p, M.top1.p,
-                     d_logits));
+  CC_HIP(ctx, launch(f, dim3(blocks), dim3(256), 0, ctx->stream, d_soa, n64, ld64, list, n_dev, M.norm.p, M.wb.p,
+                     M.bias.p, M.top1.p, d_logits));
   return CC_OK;
 }
 
@@ -248,7 +255,7 @@ int cc_ranker_rank(cc_ctx* ctx, const double* soa, size_t n, size_t ld, const cc
     CC_HIP(ctx, M.grad.ensure(n * kOut * sizeof(float)));
     d_logits = static_cast<float*>(M.grad.p);
   }
-  if ((rc = rank_dev(ctx, M, M.soa.p, n, d_logits))) return rc;
+  if ((rc = rank_dev(ctx, M, M.soa.p, n, nullptr, n, nullptr, d_logits))) return rc;
   CC_HIP(ctx, hipMemcpyAsync(top1, M.top1.p, n, hipMemcpyDeviceToHost, ctx->stream));
   if (logits)
     CC_HIP(ctx, hipMemcpyAsync(logits, d_logits, n * kOut * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
@@ -258,6 +265,11 @@ int cc_ranker_rank(cc_ctx* ctx, const double* soa, size_t n, size_t ld, const cc
 
 int cc_c2_eval_first(cc_ctx* ctx, const double* soa, size_t n, size_t ld, const cc_ranker* model, uint8_t* first,
                      uint8_t* evals, uint8_t* trap, cc_first_stats* stats) {
+  return cc_c2_eval_first_ex(ctx, soa, n, ld, model, 0, first, evals, trap, stats);
+}
+
+int cc_c2_eval_first_ex(cc_ctx* ctx, const double* soa, size_t n, size_t ld, const cc_ranker* model, int cascade,
+                        uint8_t* first, uint8_t* evals, uint8_t* trap, cc_first_stats* stats) {
   if (!ctx || !soa || !first || !evals || !trap || ld < n) return CC_E_ARG;
   if (n > 0xffffffffull) return fail(ctx, CC_E_ARG, "too many boxes");
   CC_HIP(ctx, hipSetDevice(ctx->device));
@@ -270,60 +282,108 @@ int cc_c2_eval_first(cc_ctx* ctx, const double* soa, size_t n, size_t ld, const 
   CC_HIP(ctx, M.evals.ensure(n + 1));
   CC_HIP(ctx, M.trap.ensure(n + 1));
   CC_HIP(ctx, M.bstart.ensure(16 * sizeof(uint64_t)));
-  hipEvent_t e[3];
+  hipEvent_t e[4];
   for (auto& x : e) CC_HIP(ctx, hipEventCreate(&x));
   struct Cleanup {
     hipEvent_t* e;
     ~Cleanup() {
-      for (int k = 0; k < 3; ++k) (void)hipEventDestroy(e[k]);
+      for (int k = 0; k < 4; ++k) (void)hipEventDestroy(e[k]);
     }
   } cleanup{e};
-  uint64_t bstart[16] = {0};
+  const uint64_t ld_dev = n, n64 = n;
+  // stage 1 (cascade only): criterion 0 on every box, then the boxes it
+  // leaves open packed densely (M.packed, stride n, with their indices in
+  // M.list) and their number, all in device memory: the kernels after it
+  // read the count there.  Without the cascade the positions are the boxes.
   CC_HIP(ctx, hipEventRecord(e[0], ctx->stream));
+  const void* data = M.soa.p;       // SoA the second stage reads
+  int by_pos = 0;                   // data indexed by position (packed) or by box
+  const uint32_t* list = nullptr;   // box of each position (null: the position)
+  const uint32_t* m_dev = nullptr;  // null: n positions
+  if (cascade && n) {
+    constexpr uint32_t kChunk = 2048;  // boxes per ccrd_c2_first0 block
+    const uint64_t nb = (n + kChunk - 1) / kChunk;
+    CC_HIP(ctx, M.pend.ensure(2 * (nb + 1) * sizeof(uint32_t)));
+    CC_HIP(ctx, M.list.ensure(n * sizeof(uint32_t) + 4));
+    CC_HIP(ctx, M.stage.ensure(8 * n * sizeof(double)));
+    CC_HIP(ctx, M.stage_id.ensure(n * sizeof(uint32_t)));
+    CC_HIP(ctx, M.packed.ensure(8 * n * sizeof(double)));
+    uint32_t* pend = static_cast<uint32_t*>(M.pend.p);
+    uint32_t* offs = pend + nb + 1;
+    hipFunction_t f0, fs, fl;
+    CC_HIP(ctx, kernel(ctx, "ccrd_c2_first0", &f0));
+    CC_HIP(ctx, kernel(ctx, "cc_scan_u32", &fs));
+    CC_HIP(ctx, kernel(ctx, "ccrd_c2_first0_list", &fl));
+    const uint32_t nb32 = (uint32_t)nb;
+    CC_HIP(ctx, launch(f0, dim3(nb32), dim3(256), 0, ctx->stream, M.soa.p, ld_dev, n64, ctx->d_konst, M.first.p,
+                       M.evals.p, M.trap.p, pend, M.stage.p, M.stage_id.p));
+    CC_HIP(ctx, launch(fs, dim3(1), dim3(1024), 0, ctx->stream, pend, nb32, offs));
+    CC_HIP(ctx, launch(fl, dim3(nb32), dim3(256), 0, ctx->stream, M.stage.p, M.stage_id.p, n64, pend, offs,
+                       M.packed.p, M.list.p));
+    data = M.packed.p;
+    by_pos = 1;
+    list = static_cast<const uint32_t*>(M.list.p);
+    m_dev = offs + nb;
+  }
+  CC_HIP(ctx, hipEventRecord(e[1], ctx->stream));
+  // ranking: the positions grouped into buckets by predicted class (box
+  // indices in M.idx); the cascade's packed boxes move with them (M.stage),
+  // while the full batch is read in place (moving it costs more than the
+  // scattered reads save)
+  const uint32_t* idx = list;
+  const uint64_t* bstart = nullptr;  // null: every position in bucket 0 (plain order)
   if (model && n) {
-    if ((rc = rank_dev(ctx, M, M.soa.p, n, nullptr))) return rc;
+    if ((rc = rank_dev(ctx, M, data, ld_dev, by_pos ? nullptr : list, n, m_dev, nullptr))) return rc;
     hipFunction_t fc, fp, fs;
     CC_HIP(ctx, kernel(ctx, "ccml_bucket_count", &fc));
     CC_HIP(ctx, kernel(ctx, "ccml_bucket_prefix", &fp));
     CC_HIP(ctx, kernel(ctx, "ccml_bucket_scatter", &fs));
-    const uint32_t nblocks = 1024;
-    const uint64_t n64 = n, chunk = (n + nblocks - 1) / nblocks;
+    // about 4096 positions per block, at most 1024 blocks (one prefix group)
+    uint64_t nb = (n + 4095) / 4096;
+    if (cascade) nb = (nb + 7) / 8;  // the cascade leaves a small fraction
+    const uint32_t nblocks = (uint32_t)(nb < 1 ? 1 : nb > 1024 ? 1024 : nb);
     CC_HIP(ctx, M.counts.ensure(2 * (size_t)nblocks * kOut * sizeof(unsigned long long)));
     unsigned long long* cnt = static_cast<unsigned long long*>(M.counts.p);
     unsigned long long* off = cnt + (size_t)nblocks * kOut;
-    CC_HIP(ctx, launch(fc, dim3(nblocks), dim3(256), 0, ctx->stream, M.top1.p, n64, chunk, cnt));
+    CC_HIP(ctx, launch(fc, dim3(nblocks), dim3(256), 0, ctx->stream, M.top1.p, n64, m_dev, cnt));
     CC_HIP(ctx, launch(fp, dim3(1), dim3(1024), 0, ctx->stream, cnt, nblocks, off, M.bstart.p));
-    CC_HIP(ctx, launch(fs, dim3(nblocks), dim3(256), 0, ctx->stream, M.top1.p, n64, chunk, off, M.idx.p));
-  } else {
-    std::vector<uint32_t> id(n);
-    for (size_t i = 0; i < n; ++i) id[i] = (uint32_t)i;
-    for (int b = 1; b <= kOut; ++b) bstart[b] = n;  // everything in bucket 0: plain order
-    if (n) CC_HIP(ctx, hipMemcpyAsync(M.idx.p, id.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice, ctx->stream));
-    CC_HIP(ctx, hipMemcpyAsync(M.bstart.p, bstart, sizeof bstart, hipMemcpyHostToDevice, ctx->stream));
+    CC_HIP(ctx, launch(fs, dim3(nblocks), dim3(256), 0, ctx->stream, M.top1.p, n64, m_dev, off, list, M.idx.p,
+                       by_pos ? data : nullptr, ld_dev, by_pos ? M.stage.p : nullptr));
+    if (by_pos) data = M.stage.p;
+    idx = static_cast<const uint32_t*>(M.idx.p);
+    bstart = static_cast<const uint64_t*>(M.bstart.p);
   }
-  CC_HIP(ctx, hipEventRecord(e[1], ctx->stream));
+  CC_HIP(ctx, hipEventRecord(e[2], ctx->stream));
   if (n) {
     hipFunction_t f;
     CC_HIP(ctx, kernel(ctx, "ccrd_c2_first", &f));
-    const uint64_t n64 = n, ld_dev = n;
-    CC_HIP(ctx, launch(f, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, M.soa.p, ld_dev, M.idx.p, n64,
-                       M.bstart.p, ctx->d_konst, M.first.p, M.evals.p, M.trap.p));
+    const int skip0 = cascade ? 1 : 0;
+    // one position per lane; the cascade's count is known on the device
+    // only, so its grid covers a quarter of the boxes, grid-stride
+    uint64_t blocks = (n + 255) / 256;
+    if (cascade) blocks = (blocks + 3) / 4;
+    CC_HIP(ctx, launch(f, dim3((unsigned)blocks), dim3(256), 0, ctx->stream, data, ld_dev, idx, n64, m_dev, bstart,
+                       ctx->d_konst, M.first.p, M.evals.p, M.trap.p, skip0, by_pos));
   }
-  CC_HIP(ctx, hipEventRecord(e[2], ctx->stream));
+  CC_HIP(ctx, hipEventRecord(e[3], ctx->stream));
+  uint32_t m = (uint32_t)n;
+  if (m_dev) CC_HIP(ctx, hipMemcpyAsync(&m, m_dev, sizeof m, hipMemcpyDeviceToHost, ctx->stream));
   CC_HIP(ctx, hipMemcpyAsync(first, M.first.p, n, hipMemcpyDeviceToHost, ctx->stream));
   CC_HIP(ctx, hipMemcpyAsync(evals, M.evals.p, n, hipMemcpyDeviceToHost, ctx->stream));
   CC_HIP(ctx, hipMemcpyAsync(trap, M.trap.p, n, hipMemcpyDeviceToHost, ctx->stream));
   CC_HIP(ctx, hipStreamSynchronize(ctx->stream));
   if (stats) {
-    float a = 0, b = 0;
-    CC_HIP(ctx, hipEventElapsedTime(&a, e[0], e[1]));
-    CC_HIP(ctx, hipEventElapsedTime(&b, e[1], e[2]));
+    float s1 = 0, rk = 0, s2 = 0;
+    CC_HIP(ctx, hipEventElapsedTime(&s1, e[0], e[1]));
+    CC_HIP(ctx, hipEventElapsedTime(&rk, e[1], e[2]));
+    CC_HIP(ctx, hipEventElapsedTime(&s2, e[2], e[3]));
     std::memset(stats, 0, sizeof *stats);
-    stats->rank_ms = model ? a : 0.0;
-    stats->eval_ms = b;
+    stats->rank_ms = model ? rk : 0.0;
+    stats->eval_ms = (double)s1 + s2;
+    stats->second_stage = m;
     for (size_t i = 0; i < n; ++i) {
       stats->criterion_evals += evals[i];
-      if (first[i] < kOut) {
+      if (first[i] < kCrit) {
         stats->proven += 1;
         stats->evals_proven += evals[i];
       }
@@ -370,7 +430,7 @@ int cc_c2_eval_first_f32(cc_ctx* ctx, const double* soa, size_t n, size_t ld, ui
     stats->eval_ms = ms;
     for (size_t i = 0; i < n; ++i) {
       stats->criterion_evals += evals[i];
-      if (first[i] < kOut) {
+      if (first[i] < kCrit) {
         stats->proven += 1;
         stats->evals_proven += evals[i];
       }

@@ -1,5 +1,6 @@
-// Learned criterion ranker: an 8 -> 64 -> 64 -> 10 MLP (ReLU) that predicts,
-// from a 4-D box, which criterion of the ten-criterion set proves it first.
+// Learned criterion ranker: an 8 -> 64 -> 64 -> 11 MLP (ReLU) that predicts,
+// from a 4-D box, which criterion of the ten-criterion set proves it first
+// (classes 0..9), or that none does (class 10).
 // Weights are kept as FP32 masters and used as BF16 (round to nearest even)
 // in every forward pass; accumulation is FP32.  Kernels here run in the
 // default round-to-nearest mode (no ccrd_ prefix).
@@ -17,7 +18,7 @@
 
 namespace {
 
-constexpr int kIn = 8, kH = 64, kOut = 10;
+constexpr int kIn = 8, kH = 64, kOut = 11;
 // packed parameter offsets (floats)
 constexpr int oW1 = 0, oB1 = oW1 + kIn * kH, oW2 = oB1 + kH, oB2 = oW2 + kH * kH, oW3 = oB2 + kH,
               oB3 = oW3 + kH * kOut, kParams = oB3 + kOut;
@@ -226,8 +227,12 @@ __device__ __forceinline__ bf16x4 bf16_quad(float a, float b, float c, float d) 
 // float-to-BF16 conversions use the hardware pair conversion.
 // wb: BF16 weights as [W1 (8x64) | W2 (64x64) | W3 (64x10)] row-major (k, n);
 // bias: FP32 [b1 | b2 | b3].  top1[i] = argmax logit (lowest index on
-// ties); logits (may be null) receives the 10 logits of each box.
+// ties); logits (may be null) receives the 10 logits of each box.  list
+// (may be null): position i ranks box list[i] (outputs at position i); the
+// positions are [0, n), n = *n_dev when n_dev is not null.
 extern "C" __global__ void __launch_bounds__(256) ccml_rank(const double* __restrict__ soa, uint64_t n, uint64_t ld,
+                                                             const uint32_t* __restrict__ list,
+                                                             const uint32_t* __restrict__ n_dev,
                                                              const float* __restrict__ norm,
                                                              const uint16_t* __restrict__ wb,
                                                              const float* __restrict__ bias,
@@ -268,6 +273,7 @@ extern "C" __global__ void __launch_bounds__(256) ccml_rank(const double* __rest
       c[4 * q + 3] = v.w;
     }
   };
+  if (n_dev) n = *n_dev;
   const uint64_t tiles = (n + 31) / 32;
   const uint64_t stride = (uint64_t)gridDim.x * 4;
   uint64_t tile = (uint64_t)blockIdx.x * 4 + wv;
@@ -275,9 +281,11 @@ extern "C" __global__ void __launch_bounds__(256) ccml_rank(const double* __rest
   double v[4] = {0.0, 0.0, 0.0, 0.0};
   auto load = [&](uint64_t t) {
     const uint64_t i = t * 32 + col;
-    if (t < tiles && i < n)
+    if (t < tiles && i < n) {
+      const uint64_t box = list ? list[i] : i;
 #pragma unroll
-      for (int e = 0; e < 4; ++e) v[e] = soa[(4 * g + e) * ld + i];
+      for (int e = 0; e < 4; ++e) v[e] = soa[(4 * g + e) * ld + box];
+    }
   };
   load(tile);
   for (; tile < tiles; tile += stride) {
@@ -326,13 +334,13 @@ extern "C" __global__ void __launch_bounds__(256) ccml_rank(const double* __rest
       z3 = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a3[s], bf16_quad<true>(h[r], h[r + 1], h[r + 2], h[r + 3]), z3, 0,
                                                   0, 0);
     }
-    // classes here: g = 0 holds 0..3 and 8, 9; g = 1 holds 4..7.  Best of
+    // classes here: g = 0 holds 0..3 and 8..10; g = 1 holds 4..7.  Best of
     // each lane in increasing class order, then the pair of lanes l, l ^ 32
     // combined (larger logit; on equal logits the lower class).
     float bv = z3[0];
     int best = 4 * g;
 #pragma unroll
-    for (int r = 1; r < 6; ++r) {
+    for (int r = 1; r < 7; ++r) {
       const int c = 8 * (r >> 2) + 4 * g + (r & 3);
       if (c < kOut && z3[r] > bv) {
         bv = z3[r];
@@ -346,7 +354,7 @@ extern "C" __global__ void __launch_bounds__(256) ccml_rank(const double* __rest
       if (g == 0) top1[i] = (uint8_t)best;
       if (logits) {
 #pragma unroll
-        for (int r = 0; r < 6; ++r) {
+        for (int r = 0; r < 7; ++r) {
           const int c = 8 * (r >> 2) + 4 * g + (r & 3);
           if (c < kOut) logits[i * kOut + c] = z3[r];
         }
@@ -358,57 +366,85 @@ extern "C" __global__ void __launch_bounds__(256) ccml_rank(const double* __rest
 // Counting sort of box indices by predicted criterion, in three passes over
 // fixed block ranges (no contended global atomics): per-block bucket counts,
 // a prefix over (bucket, block), then a scatter from per-block offsets.
-// Order inside a bucket is unspecified.
+// Order inside a bucket is unspecified.  Positions [0, n), n = *n_dev when
+// n_dev is not null; block b takes the b-th of gridDim.x equal ranges.
 extern "C" __global__ void __launch_bounds__(256) ccml_bucket_count(const uint8_t* __restrict__ top1, uint64_t n,
-                                                                     uint64_t chunk,
+                                                                     const uint32_t* __restrict__ n_dev,
                                                                      unsigned long long* __restrict__ counts) {
   __shared__ unsigned int c[kOut];
   if (threadIdx.x < kOut) c[threadIdx.x] = 0;
   __syncthreads();
+  if (n_dev) n = *n_dev;
+  const uint64_t chunk = (n + gridDim.x - 1) / gridDim.x;
   const uint64_t lo = (uint64_t)blockIdx.x * chunk, hi = lo + chunk < n ? lo + chunk : n;
   for (uint64_t i = lo + threadIdx.x; i < hi; i += blockDim.x) atomicAdd(&c[top1[i] < kOut ? top1[i] : 0], 1u);
   __syncthreads();
   if (threadIdx.x < kOut) counts[(uint64_t)blockIdx.x * kOut + threadIdx.x] = c[threadIdx.x];
 }
 
-// One workgroup of 1024 lanes (nblocks <= 1024): offsets[b*10 + k] = first
-// position of block b's boxes of bucket k (bucket-major, then block order);
-// bstart[k] = first position of bucket k, bstart[10] = total.
+// One workgroup of 1024 lanes: offsets[b*11 + k] = first position of block
+// b's boxes of bucket k (bucket-major, then block order); bstart[k] = first
+// position of bucket k, bstart[11] = total.  Wave k scans bucket k's counts
+// 64 blocks at a time (shuffles, no workgroup barrier inside the scan), then
+// adds the bucket's base once every bucket total is known.
 extern "C" __global__ void __launch_bounds__(1024) ccml_bucket_prefix(const unsigned long long* __restrict__ counts,
                                                                        uint32_t nblocks,
                                                                        unsigned long long* __restrict__ offsets,
                                                                        uint64_t* __restrict__ bstart) {
-  __shared__ unsigned long long sc[1024];
-  const uint32_t b = threadIdx.x;
-  unsigned long long base = 0;
-  for (int k = 0; k < kOut; ++k) {
-    const unsigned long long v = b < nblocks ? counts[(uint64_t)b * kOut + k] : 0ull;
-    sc[b] = v;
-    __syncthreads();
-    for (uint32_t d = 1; d < 1024; d <<= 1) {  // inclusive Hillis-Steele scan
-      const unsigned long long add = b >= d ? sc[b - d] : 0ull;
-      __syncthreads();
-      sc[b] += add;
-      __syncthreads();
+  static_assert(kOut <= 16, "one wave per bucket");
+  __shared__ unsigned long long tot[kOut];
+  const int w = threadIdx.x >> 6, lane = __lane_id();
+  if (w < kOut) {
+    unsigned long long run = 0;
+    for (uint32_t c = 0; c < nblocks; c += 64) {
+      const uint32_t b = c + lane;
+      const unsigned long long v = b < nblocks ? counts[(uint64_t)b * kOut + w] : 0ull;
+      unsigned long long x = v;  // inclusive scan over the wave
+#pragma unroll
+      for (int d = 1; d < 64; d <<= 1) {
+        const unsigned long long y = __shfl_up(x, d);
+        if (lane >= d) x += y;
+      }
+      if (b < nblocks) offsets[(uint64_t)b * kOut + w] = run + x - v;
+      run += __shfl(x, 63);
     }
-    if (b < nblocks) offsets[(uint64_t)b * kOut + k] = base + sc[b] - v;
-    if (b == 0) bstart[k] = base;
-    base += sc[1023];
-    __syncthreads();
+    if (lane == 0) tot[w] = run;
   }
-  if (b == 0) bstart[kOut] = base;
+  __syncthreads();
+  if (w < kOut) {
+    unsigned long long base = 0;
+    for (int k = 0; k < w; ++k) base += tot[k];
+    // each lane revisits the entries it wrote above
+    for (uint32_t b = lane; b < nblocks; b += 64) offsets[(uint64_t)b * kOut + w] += base;
+    if (lane == 0) {
+      bstart[w] = base;
+      if (w == kOut - 1) bstart[kOut] = base + tot[w];
+    }
+  }
 }
 
+// list (may be null): position i stands for box list[i].  src (may be
+// null): a packed SoA of stride ld by position, copied into dst in bucket
+// order along with the indices.
 extern "C" __global__ void __launch_bounds__(256) ccml_bucket_scatter(const uint8_t* __restrict__ top1, uint64_t n,
-                                                                       uint64_t chunk,
+                                                                       const uint32_t* __restrict__ n_dev,
                                                                        const unsigned long long* __restrict__ offsets,
-                                                                       uint32_t* __restrict__ idx) {
+                                                                       const uint32_t* __restrict__ list,
+                                                                       uint32_t* __restrict__ idx,
+                                                                       const double* __restrict__ src, uint64_t ld,
+                                                                       double* __restrict__ dst) {
   __shared__ unsigned long long cur[kOut];
   if (threadIdx.x < kOut) cur[threadIdx.x] = offsets[(uint64_t)blockIdx.x * kOut + threadIdx.x];
   __syncthreads();
+  if (n_dev) n = *n_dev;
+  const uint64_t chunk = (n + gridDim.x - 1) / gridDim.x;
   const uint64_t lo = (uint64_t)blockIdx.x * chunk, hi = lo + chunk < n ? lo + chunk : n;
   for (uint64_t i = lo + threadIdx.x; i < hi; i += blockDim.x) {
     const int b = top1[i] < kOut ? top1[i] : 0;
-    idx[atomicAdd(&cur[b], 1ull)] = (uint32_t)i;
+    const unsigned long long at = atomicAdd(&cur[b], 1ull);
+    idx[at] = list ? list[i] : (uint32_t)i;
+    if (src)
+#pragma unroll
+      for (int r = 0; r < 8; ++r) dst[r * ld + at] = src[r * ld + i];
   }
 }

@@ -81,7 +81,8 @@ class FastParams(ctypes.Structure):
 
 lib.cc_rsb_bnb_batched.argtypes = [_vp, _vp, _sz, _i, _vp, _vp, _sz, _P(_sz), _vp]
 lib.cc_rsb_autotune.argtypes = [_vp, _vp, _sz, _i, _vp, _P(ctypes.c_double)]
-RANKER_PARAMS = 5386
+RANKER_PARAMS = 5451
+RANKER_CLASSES = 11  # criteria 0..9 first proving a box, 10: none proves it
 
 
 class Ranker(ctypes.Structure):
@@ -93,13 +94,14 @@ class Ranker(ctypes.Structure):
 
 class FirstStats(ctypes.Structure):
     _fields_ = [("proven", ctypes.c_uint64), ("criterion_evals", ctypes.c_uint64), ("evals_proven", ctypes.c_uint64),
-                ("rank_ms", ctypes.c_double), ("eval_ms", ctypes.c_double)]
+                ("rank_ms", ctypes.c_double), ("eval_ms", ctypes.c_double), ("second_stage", ctypes.c_uint64)]
 
 
 lib.cc_ranker_train.argtypes = [_vp, _vp, _sz, _sz, _vp, _i, _i, ctypes.c_float, _u64, _vp, _vp]
 lib.cc_ranker_gradient.argtypes = [_vp, _vp, _sz, _sz, _vp, _vp, _i, _vp, _vp, _vp]
 lib.cc_ranker_rank.argtypes = [_vp, _vp, _sz, _sz, _vp, _vp, _vp]
 lib.cc_c2_eval_first.argtypes = [_vp, _vp, _sz, _sz, _vp, _vp, _vp, _vp, _vp]
+lib.cc_c2_eval_first_ex.argtypes = [_vp, _vp, _sz, _sz, _vp, _i, _vp, _vp, _vp, _vp]
 lib.cc_c2_eval_first_f32.argtypes = [_vp, _vp, _sz, _sz, _vp, _vp, _vp, _vp, _vp, _vp]
 lib.cc_tr_find_critical_in.argtypes = [_vp, _vp, _i, _vp, _sz, _P(_sz)]
 lib.cc_tr_enumerate.argtypes = [_vp, _vp, _sz, _vp, _vp, _sz, _P(_sz)]
@@ -359,22 +361,24 @@ class Prover:
         soa = np.ascontiguousarray(soa, dtype=np.float64)
         n = soa.shape[1]
         top1 = np.empty(n, dtype=np.uint8)
-        lg = np.empty((n, 10), dtype=np.float32) if with_logits else None
+        lg = np.empty((n, RANKER_CLASSES), dtype=np.float32) if with_logits else None
         self._check(lib.cc_ranker_rank(self.ctx, _ptr(soa), n, n, ctypes.byref(model), _ptr(top1),
                                        _ptr(lg) if with_logits else None))
         return (top1, lg) if with_logits else top1
 
-    def eval_first(self, soa: np.ndarray, model: "Ranker" = None):
-        """First-success evaluation (plain order if model is None): returns
-        (first, evals, trap, FirstStats)."""
+    def eval_first(self, soa: np.ndarray, model: "Ranker" = None, cascade: bool = False):
+        """First-success evaluation (plain order if model is None; cascade:
+        criterion 0 on every box first, the rest on the boxes it leaves
+        open): returns (first, evals, trap, FirstStats)."""
         soa = np.ascontiguousarray(soa, dtype=np.float64)
         n = soa.shape[1]
         first = np.empty(n, dtype=np.uint8)
         evals = np.empty(n, dtype=np.uint8)
         trap = np.empty(n, dtype=np.uint8)
         st = FirstStats()
-        self._check(lib.cc_c2_eval_first(self.ctx, _ptr(soa), n, n, ctypes.byref(model) if model is not None else None,
-                                         _ptr(first), _ptr(evals), _ptr(trap), ctypes.byref(st)))
+        self._check(lib.cc_c2_eval_first_ex(self.ctx, _ptr(soa), n, n,
+                                            ctypes.byref(model) if model is not None else None, int(bool(cascade)),
+                                            _ptr(first), _ptr(evals), _ptr(trap), ctypes.byref(st)))
         return first, evals, trap, st
 
     def eval_first_f32(self, soa: np.ndarray, with_widths: bool = False):

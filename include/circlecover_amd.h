@@ -212,18 +212,19 @@ int cc_rsb_autotune(cc_ctx* ctx, const cc_rsb_cell* cells, size_t ncells, int ma
                     double* best_ms);
 
 /* ---- learned criterion ranker (no reference counterpart) ----
- * An 8 -> 64 -> 64 -> 10 MLP (ReLU; BF16 weights, FP32 accumulate) that
+ * An 8 -> 64 -> 64 -> 11 MLP (ReLU; BF16 weights, FP32 accumulate) that
  * predicts from a box (centre and log2 width of lambda, r1, r2, r3) which of
- * the ten cc_eval_batch criteria proves it first.  It only reorders work:
- * every verdict is still decided by the interval criteria. */
-#define CC_RANKER_PARAMS 5386 /* W1 8x64, b1 64, W2 64x64, b2 64, W3 64x10, b3 10 (row-major k, n) */
+ * the ten cc_eval_batch criteria proves it first (classes 0..9) or that none
+ * does (class 10).  It only reorders work: every verdict is still decided by
+ * the interval criteria. */
+#define CC_RANKER_PARAMS 5451 /* W1 8x64, b1 64, W2 64x64, b2 64, W3 64x11, b3 11 (row-major k, n) */
 typedef struct cc_ranker {
   float params[CC_RANKER_PARAMS]; /* FP32 master weights */
   float norm[16];                 /* feature mean[8], std[8] */
 } cc_ranker;
 
-/* Trains on the boxes with label[i] < 10 (the first proving criterion):
- * softmax cross-entropy, Adam (beta 0.9/0.999, eps 1e-8), `batch` boxes per
+/* Trains on the boxes with label[i] <= 10 (the first proving criterion, 10:
+ * none proves the box; other labels leave the box out): softmax cross-entropy, Adam (beta 0.9/0.999, eps 1e-8), `batch` boxes per
  * step, He-uniform initial weights from `seed`.  epoch_loss (epochs entries,
  * may be NULL) receives the mean training loss of each epoch. */
 int cc_ranker_train(cc_ctx* ctx, const double* soa, size_t n, size_t ld, const uint8_t* label, int epochs,
@@ -232,8 +233,8 @@ int cc_ranker_train(cc_ctx* ctx, const double* soa, size_t n, size_t ld, const u
  * every parameter (one forward and backward pass, no update). */
 int cc_ranker_gradient(cc_ctx* ctx, const double* soa, size_t n, size_t ld, const uint8_t* label,
                        const uint32_t* sample, int B, const cc_ranker* model, float* grad, float* loss);
-/* Batched MFMA inference: top1[i] = predicted criterion of box i; logits
- * (n*10, may be NULL) receives the raw outputs. */
+/* Batched MFMA inference: top1[i] = predicted class of box i; logits
+ * (n*11, may be NULL) receives the raw outputs. */
 int cc_ranker_rank(cc_ctx* ctx, const double* soa, size_t n, size_t ld, const cc_ranker* model, uint8_t* top1,
                    float* logits);
 
@@ -243,17 +244,27 @@ typedef struct cc_first_stats {
   uint64_t evals_proven;    /* criteria evaluated over the proven boxes */
   double rank_ms;           /* inference and grouping (HIP events) */
   double eval_ms;           /* criterion evaluation (HIP events) */
+  uint64_t second_stage;    /* boxes past criterion 0 in the cascade form (else all) */
 } cc_first_stats;
 
 /* First-success evaluation of the ten-criterion set: criteria are tried one
  * at a time until one proves the box.  model == NULL: plain order 0..9.
- * Otherwise boxes are grouped by the ranker's predicted criterion, which is
- * tried first, then the others in plain order.  Whether a box is proven
+ * Otherwise boxes are grouped by the ranker's predicted class: a predicted
+ * criterion is tried first, then the others in plain order; the boxes
+ * predicted to be unprovable (class 10) form the last group, in plain order,
+ * so that they do not hold up the waves of boxes that are proven early.  Whether a box is proven
  * does not depend on the order; first[i] = the criterion that proved it
  * (255: none), evals[i] = criteria evaluated, trap[i] = 1 if an evaluated
  * criterion reached a trap site. */
 int cc_c2_eval_first(cc_ctx* ctx, const double* soa, size_t n, size_t ld, const cc_ranker* model, uint8_t* first,
                      uint8_t* evals, uint8_t* trap, cc_first_stats* stats);
+/* As cc_c2_eval_first; cascade != 0 first tries criterion 0 (which proves
+ * most boxes) on every box, then runs the remaining criteria, in plain or
+ * ranked order, only on the boxes it leaves open, ranking only those.  Plain
+ * cascade gives exactly plain order's first/evals/trap; ranked cascade
+ * proves the same boxes. */
+int cc_c2_eval_first_ex(cc_ctx* ctx, const double* soa, size_t n, size_t ld, const cc_ranker* model, int cascade,
+                        uint8_t* first, uint8_t* evals, uint8_t* trap, cc_first_stats* stats);
 
 /* ---- whole proofs (replace find_critical_intervals(), see also the C++
  *      declarations at the end of this header) ---- */

@@ -8,7 +8,7 @@ Pinned by a central finite-difference check of its own gradient.
 """
 import numpy as np
 
-K_IN, K_H, K_OUT = 8, 64, 10
+K_IN, K_H, K_OUT = 8, 64, 11
 SHAPES = [("W1", (K_IN, K_H)), ("b1", (K_H,)), ("W2", (K_H, K_H)), ("b2", (K_H,)), ("W3", (K_H, K_OUT)),
           ("b3", (K_OUT,))]
 N_PARAMS = sum(int(np.prod(s)) for _, s in SHAPES)
