@@ -22,7 +22,8 @@ configurations measured in the same run (not part of `value`):
                time, identical unproven set required;
   tr_small     tight_rectangle: 64 lambda cells of [1, 1.5], r1..r4 in
                64/32/32/32 pieces, three bisection levels;
-  fp32_prefilter  the FP32 pre-filter on the 2^22 boxes (first-success order);
+  fp32_prefilter  the FP32 pre-filter on the 2^22 boxes (first-success order,
+               per-box and cascade forms, each against its FP64 form);
   ranker       the learned criterion ranker trained on 2^20 boxes of another
                seed; first-success evaluation of the 2^22 boxes in plain order,
                plain cascade, ranked and ranked cascade (same proven set);
@@ -177,16 +178,22 @@ def extended(p, soa: np.ndarray, mask: np.ndarray) -> dict:
     lv, per = p.tr_bnb(tc, (64, 32, 32, 32), 3)
     out["tr_small"] = {"wall_s": time.perf_counter() - t, "unproven": int(lv.shape[0]),
                        "unproven_per_level": [int(x) for x in per]}
-    # FP32 pre-filter vs plain FP64 first-success
+    # FP32 pre-filter vs FP64 first-success, per-box and cascade forms
     p.eval_first(soa)
     f64_first, _, _, st64 = p.eval_first(soa)
-    first, evals, trap, dec, w, st32 = p.eval_first_f32(soa, with_widths=True)
+    _, _, _, st64c = p.eval_first(soa, cascade=True)
+    _, _, _, _, w, _ = p.eval_first_f32(soa, with_widths=True)  # widths for the inflation report, untimed
+    first, evals, trap, dec, _, st32 = p.eval_first_f32(soa)
+    p.eval_first_f32(soa, cascade=True)
+    first_c, _, _, dec_c, _, st32c = p.eval_first_f32(soa, cascade=True)
     bad = int(((dec == 1) & ((mask & 1) == 0)).sum())
     r_cr = w[:, 0] / np.maximum(w[:, 1], 1e-300)
     r_rem = w[:, 2] / np.maximum(w[:, 3], 1e-300)
     out["fp32_prefilter"] = {"fraction_decided_fp32": float(dec.mean()), "fp32_accept_fp64_reject": bad,
-                             "identical_first": bool((first == f64_first).all()),
+                             "identical_first": bool((first == f64_first).all() and (first_c == f64_first).all()
+                                                     and (dec_c == dec).all()),
                              "eval_ms_fp32_then_fp64": st32.eval_ms, "eval_ms_fp64": st64.eval_ms,
+                             "eval_ms_fp32_then_fp64_cascade": st32c.eval_ms, "eval_ms_fp64_cascade": st64c.eval_ms,
                              "width_inflation": {
                                  "r1_strategies.critical_ratio": [float(np.median(r_cr)),
                                                                   float(np.percentile(r_cr, 99))],
@@ -223,9 +230,8 @@ def extended(p, soa: np.ndarray, mask: np.ndarray) -> dict:
                      "train_s_cascade": t_train_c, "final_loss_cascade": float(losses_c[-1]),
                      "train_boxes_cascade": int((tf_c <= 10).sum()), "variants": variants,
                      "evals_per_proven_plain": variants["plain"]["evals_per_proven"],
-                     "evals_per_proven_ranked": variants["ranked_cascade"]["evals_per_proven"],
-                     "device_ms_plain": variants["plain"]["device_ms"],
-                     "device_ms_ranked": variants["ranked_cascade"]["device_ms"],
+                     "evals_per_proven_ranked": variants["ranked"]["evals_per_proven"],
+                     "fastest": min(variants, key=lambda k: variants[k]["device_ms"]),
                      "same_proven_set": bool(all((x == proven_sets[0]).all() for x in proven_sets))}
     return out
 

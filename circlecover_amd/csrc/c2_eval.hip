@@ -24,17 +24,26 @@ static constexpr int kStripCap = 4;
 // The cover-strip criterion (4) runs its width search with the exact
 // shortcut of r123_cover_strip_capped; the few boxes that would need a long
 // plain bisection first are appended to `deferred` (count in
-// *n_deferred, which must be 0 at launch) with bit 4 left clear, and
+// *n_deferred, which must be 0 at launch), their data to the same slot of
+// d_stage (8 rows of stride n), with bit 4 left clear, and
 // ccrd_c2_strip_deferred completes them.  Criterion 0 runs only its first
 // strategy here, which proves most boxes; ccrd_c2_r1_pass tries the others
 // on the boxes whose bit 0 is clear, so that a wave does not run them for
-// the one or two of its lanes that need them.
+// the one or two of its lanes that need them.  Each wave stages those boxes
+// (lambda, r1, r2.hi and the box index) at the front of its own 64 slots of
+// r1_stage (5 rows of stride ns = the launch's lane count) and r1_id, and
+// their number in r1_cnt[wave]: the second pass reads them densely, with no
+// atomics on either side.
 extern "C" __global__ void __launch_bounds__(256) ccrd_c2_eval(const double* __restrict__ soa, uint64_t n,
                                                                 uint64_t ld, const Konst* __restrict__ kp,
                                                                 uint32_t* __restrict__ mask,
                                                                 uint8_t* __restrict__ first,
                                                                 uint32_t* __restrict__ deferred,
-                                                                unsigned long long* __restrict__ n_deferred) {
+                                                                unsigned long long* __restrict__ n_deferred,
+                                                                double* __restrict__ d_stage,
+                                                                double* __restrict__ r1_stage,
+                                                                uint32_t* __restrict__ r1_id,
+                                                                uint32_t* __restrict__ r1_cnt, uint64_t ns) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   // Lanes past n stay to the wave-wide append below and store nothing.
   const bool live = i < n;
@@ -64,14 +73,38 @@ extern "C" __global__ void __launch_bounds__(256) ccrd_c2_eval(const double* __r
   if (rsbc::s4_opposite_corners(la, r1, r2, r3)) m |= 1u << 8;
   if (rsbc::s5_vertical_strip(K, la, r1, r2, r3)) m |= 1u << 9;
   const uint64_t slot = wave_append(live && defer, n_deferred);
+  const bool r1_open = live && !(m & 1u);
+  const uint64_t ob = __ballot(r1_open);
+  const uint64_t wave = i >> 6;  // the block is a whole number of waves
+  if (r1_open) {
+    const uint64_t d = wave * 64 + (uint64_t)__popcll(ob & ((1ull << __lane_id()) - 1ull));  // < ns
+    r1_stage[0 * ns + d] = la.lo;
+    r1_stage[1 * ns + d] = la.hi;
+    r1_stage[2 * ns + d] = r1.lo;
+    r1_stage[3 * ns + d] = r1.hi;
+    r1_stage[4 * ns + d] = r2.hi;
+    r1_id[d] = (uint32_t)i;
+  }
+  if (__lane_id() == 0) r1_cnt[wave] = (uint32_t)__popcll(ob);
   if (!live) return;
-  if (defer) deferred[slot] = (uint32_t)i;
+  if (defer) {
+    deferred[slot] = (uint32_t)i;  // slot < n: one slot per deferring box
+    d_stage[0 * n + slot] = la.lo;
+    d_stage[1 * n + slot] = la.hi;
+    d_stage[2 * n + slot] = r1.lo;
+    d_stage[3 * n + slot] = r1.hi;
+    d_stage[4 * n + slot] = r2.lo;
+    d_stage[5 * n + slot] = r2.hi;
+    d_stage[6 * n + slot] = r3.lo;
+    d_stage[7 * n + slot] = r3.hi;
+  }
   mask[i] = m;
   const uint32_t v = m & 0x3ffu;
   first[i] = v ? (uint8_t)__builtin_ctz(v) : (uint8_t)255;
 }
 
-// Second pass over the boxes ccrd_c2_eval deferred: the plain cover-strip
+// Second pass over the boxes ccrd_c2_eval deferred (their data read densely
+// from d_stage, stride ld = the batch size): the plain cover-strip
 // bisection, then bit 4 of mask and first updated.  The bisection is
 // sequential, and the deferred boxes are few, so one lane per box would
 // leave the GPU mostly idle; instead a group of 16 lanes takes a box and
@@ -83,7 +116,7 @@ extern "C" __global__ void __launch_bounds__(256) ccrd_c2_eval(const double* __r
 // result is the sequential one, test by test.
 static constexpr int kGroup = 16, kLevels = 4;
 
-extern "C" __global__ void __launch_bounds__(256) ccrd_c2_strip_deferred(const double* __restrict__ soa,
+extern "C" __global__ void __launch_bounds__(256) ccrd_c2_strip_deferred(const double* __restrict__ d_stage,
                                                                           uint64_t ld,
                                                                           const Konst* __restrict__ kp,
                                                                           const uint32_t* __restrict__ deferred,
@@ -96,10 +129,10 @@ extern "C" __global__ void __launch_bounds__(256) ccrd_c2_strip_deferred(const d
   const uint64_t groups = (uint64_t)gridDim.x * blockDim.x / kGroup;
   for (uint64_t k = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / kGroup; k < cnt; k += groups) {
     const uint64_t i = deferred[k];
-    const iv la{soa[0 * ld + i], soa[1 * ld + i]};
-    const iv r1{soa[2 * ld + i], soa[3 * ld + i]};
-    const iv r2{soa[4 * ld + i], soa[5 * ld + i]};
-    const iv r3{soa[6 * ld + i], soa[7 * ld + i]};
+    const iv la{d_stage[0 * ld + k], d_stage[1 * ld + k]};
+    const iv r1{d_stage[2 * ld + k], d_stage[3 * ld + k]};
+    const iv r2{d_stage[4 * ld + k], d_stage[5 * ld + k]};
+    const iv r3{d_stage[6 * ld + k], d_stage[7 * ld + k]};
     const double q1 = mul_rd(4.0, r1.lo), q2 = mul_rd(4.0, r2.lo), q3 = mul_rd(4.0, r3.lo);
     double lo = 0.3, hi = la.hi;
     if (!trc::strip_width_ok(q1, q2, q3, lo)) continue;  // (deferred boxes pass this test)
@@ -155,45 +188,46 @@ extern "C" __global__ void __launch_bounds__(256) ccrd_c2_strip_deferred(const d
 }
 
 // Criterion 0 second pass: r1_strategies' remaining strategies for the boxes
-// whose bit 0 is still clear, then bit 0 of mask and first.  Each block
-// gathers the pending boxes of a 2048-box chunk into LDS (a ballot and one
-// LDS atomic per wave) and works on them densely.
-static constexpr uint32_t kR1Chunk = 2048;
+// ccrd_c2_eval staged (those whose bit 0 is clear), then bit 0 of mask and
+// first.  Block b takes the staging slots of waves [32 b, 32 b + 32): a
+// prefix over their 32 counts in LDS, then one lane per staged box.
+static constexpr uint32_t kR1Waves = 32;
 
-extern "C" __global__ void __launch_bounds__(256) ccrd_c2_r1_pass(const double* __restrict__ soa, uint64_t n,
-                                                                   uint64_t ld, const Konst* __restrict__ kp,
+extern "C" __global__ void __launch_bounds__(256) ccrd_c2_r1_pass(const double* __restrict__ r1_stage,
+                                                                   const uint32_t* __restrict__ r1_id,
+                                                                   const uint32_t* __restrict__ r1_cnt,
+                                                                   uint64_t nwaves, uint64_t ns,
+                                                                   const Konst* __restrict__ kp,
                                                                    uint32_t* __restrict__ mask,
                                                                    uint8_t* __restrict__ first) {
-  __shared__ uint32_t list[kR1Chunk];
-  __shared__ uint32_t count;
+  __shared__ uint32_t pre[kR1Waves + 1];
+  const uint64_t w0 = (uint64_t)blockIdx.x * kR1Waves;
+  if (threadIdx.x < 64) {  // wave 0: inclusive scan of the 32 counts
+    const uint32_t t = threadIdx.x;
+    uint32_t x = t < kR1Waves && w0 + t < nwaves ? r1_cnt[w0 + t] : 0u;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const uint32_t y = __shfl_up(x, d);
+      if ((int)t >= d) x += y;
+    }
+    if (t == 0) pre[0] = 0;
+    if (t < kR1Waves) pre[t + 1] = x;
+  }
+  __syncthreads();
+  const uint32_t total = pre[kR1Waves];
   const Konst K = *kp;
-  for (uint64_t base = (uint64_t)blockIdx.x * kR1Chunk; base < n; base += (uint64_t)gridDim.x * kR1Chunk) {
-    if (threadIdx.x == 0) count = 0;
-    __syncthreads();
-    for (uint32_t j = threadIdx.x; j < kR1Chunk; j += blockDim.x) {  // kR1Chunk is a multiple of the block
-      const uint64_t i = base + j;
-      const bool keep = i < n && (mask[i] & 1u) == 0;
-      const uint64_t b = __ballot(keep);
-      if (b) {
-        const int lane = __lane_id();
-        const int leader = __ffsll((unsigned long long)b) - 1;
-        uint32_t at = 0;
-        if (lane == leader) at = atomicAdd(&count, (uint32_t)__popcll(b));
-        at = __shfl(at, leader);
-        if (keep) list[at + (uint32_t)__popcll(b & ((1ull << lane) - 1ull))] = j;
-      }
+  for (uint32_t k = threadIdx.x; k < total; k += blockDim.x) {
+    uint32_t j = 0;  // the wave whose slots hold entry k: pre[j] <= k < pre[j + 1]
+#pragma unroll
+    for (uint32_t step = kR1Waves / 2; step > 0; step >>= 1)
+      if (pre[j + step] <= k) j += step;
+    const uint64_t d = (w0 + j) * 64 + (k - pre[j]);
+    const iv la{r1_stage[0 * ns + d], r1_stage[1 * ns + d]};
+    const iv r1{r1_stage[2 * ns + d], r1_stage[3 * ns + d]};
+    if (trc::r1_rest(K, la, trc::weight_for(K, la), r1, r1_stage[4 * ns + d])) {
+      const uint32_t i = r1_id[d];
+      mask[i] |= 1u;
+      first[i] = 0;
     }
-    __syncthreads();
-    const uint32_t total = count;
-    for (uint32_t k = threadIdx.x; k < total; k += blockDim.x) {
-      const uint64_t i = base + list[k];
-      const iv la{soa[0 * ld + i], soa[1 * ld + i]};
-      const iv r1{soa[2 * ld + i], soa[3 * ld + i]};
-      if (trc::r1_rest(K, la, trc::weight_for(K, la), r1, soa[5 * ld + i])) {
-        mask[i] |= 1u;
-        first[i] = 0;
-      }
-    }
-    __syncthreads();
   }
 }

@@ -8,7 +8,7 @@
 // With every box in bucket 0 this is the plain order.
 //
 // Cascade form (c2_cascade.hip): criterion 0 first on every box, then this
-// kernel with skip0 = 1 on the list of boxes it leaves open.
+// kernel with done0 > 0 on the packed boxes it leaves open.
 #define CC_EXACT_SHORTCUTS 1
 #include "dev/c2_criteria.hpp"
 
@@ -20,9 +20,10 @@ using namespace ccd;
 // SoA).  bucket_start[b] = first position of bucket b (0..10),
 // bucket_start[11] = n (bucket_start null: every position in bucket 0).
 // first[box] = proving criterion or 255, evals[box] = criteria evaluated,
-// trap[box] = 1 if an evaluated criterion reached a trap site.  skip0 = 1:
+// trap[box] = 1 if an evaluated criterion reached a trap site.  done0 > 0:
 // criterion 0, which has no trap site, was already tried and failed in
-// ccrd_c2_first0, so it is skipped here and counted once.
+// ccrd_c2_first0 (done0 = 1) or ccrd_c2_first0_f32 (done0 = 2, the FP32
+// attempt included), so it is skipped here and evals starts at done0.
 extern "C" __global__ void __launch_bounds__(256) ccrd_c2_first(const double* __restrict__ soa, uint64_t ld,
                                                                  const uint32_t* __restrict__ idx, uint64_t n,
                                                                  const uint32_t* __restrict__ n_dev,
@@ -30,7 +31,7 @@ extern "C" __global__ void __launch_bounds__(256) ccrd_c2_first(const double* __
                                                                  const Konst* __restrict__ kp,
                                                                  uint8_t* __restrict__ first,
                                                                  uint8_t* __restrict__ evals,
-                                                                 uint8_t* __restrict__ trap, int skip0, int by_pos) {
+                                                                 uint8_t* __restrict__ trap, int done0, int by_pos) {
   const uint64_t cnt = n_dev ? *n_dev : n;
   const Konst K = *kp;
   for (uint64_t pos = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; pos < cnt;
@@ -48,11 +49,11 @@ extern "C" __global__ void __launch_bounds__(256) ccrd_c2_first(const double* __
     const double rem4 = (W - r1 - r2 - r3).hi;
     const double ub_r4 = r3.hi < rem4 ? r3.hi : rem4;
     Trap tp{0};
-    int found = 255, count = skip0 ? 1 : 0;
+    int found = 255, count = done0;
     for (int s = 0; s < 10; ++s) {
       // step 0 tries the bucket's criterion, then 0..9 without it
       const int k = b == 10 ? s : s == 0 ? b : (s - 1 < b ? s - 1 : s);
-      if (skip0 && k == 0) continue;
+      if (done0 && k == 0) continue;
       ++count;
       if (c2_criterion(k, K, la, W, r1, r2, r3, ub_r4, tp)) {
         found = k;

@@ -18,7 +18,7 @@ static_assert(kB3 + kOut == CC_RANKER_PARAMS, "parameter count");
 
 struct MlState {
   DevBuf soa, label, perm, rec, loss, params, m, v, grad, norm, wb, bias, top1, idx, counts, first, evals, trap, bstart,
-      partial, pend, list, stage, stage_id, packed;
+      partial, pend, list, stage, stage_id, packed, dec, widths;
 };
 constexpr int kSlice = 64;  // samples per partial gradient sum (ml.hip)
 
@@ -268,9 +268,16 @@ int cc_c2_eval_first(cc_ctx* ctx, const double* soa, size_t n, size_t ld, const 
   return cc_c2_eval_first_ex(ctx, soa, n, ld, model, 0, first, evals, trap, stats);
 }
 
-int cc_c2_eval_first_ex(cc_ctx* ctx, const double* soa, size_t n, size_t ld, const cc_ranker* model, int cascade,
-                        uint8_t* first, uint8_t* evals, uint8_t* trap, cc_first_stats* stats) {
+// First-success evaluation.  stage1 = 0: none (every box to the second
+// stage); 1: FP64 criterion 0 on every box (cascade); 2: FP32 shadow of
+// criterion 0, then FP64 criterion 0 on the boxes it leaves undecided
+// (cascade; f32_decided and widths as cc_c2_eval_first_f32 writes them).
+static int first_success(cc_ctx* ctx, const double* soa, size_t n, size_t ld, const cc_ranker* model, int stage1,
+                         uint8_t* first, uint8_t* evals, uint8_t* trap, uint8_t* f32_decided, float* widths,
+                         cc_first_stats* stats) {
   if (!ctx || !soa || !first || !evals || !trap || ld < n) return CC_E_ARG;
+  if (stage1 == 2 && (!f32_decided || model)) return CC_E_ARG;
+  const bool cascade = stage1 != 0;
   if (n > 0xffffffffull) return fail(ctx, CC_E_ARG, "too many boxes");
   CC_HIP(ctx, hipSetDevice(ctx->device));
   MlState& M = ml(ctx);
@@ -311,12 +318,19 @@ int cc_c2_eval_first_ex(cc_ctx* ctx, const double* soa, size_t n, size_t ld, con
     uint32_t* pend = static_cast<uint32_t*>(M.pend.p);
     uint32_t* offs = pend + nb + 1;
     hipFunction_t f0, fs, fl;
-    CC_HIP(ctx, kernel(ctx, "ccrd_c2_first0", &f0));
+    CC_HIP(ctx, kernel(ctx, stage1 == 2 ? "ccrd_c2_first0_f32" : "ccrd_c2_first0", &f0));
     CC_HIP(ctx, kernel(ctx, "cc_scan_u32", &fs));
     CC_HIP(ctx, kernel(ctx, "ccrd_c2_first0_list", &fl));
     const uint32_t nb32 = (uint32_t)nb;
-    CC_HIP(ctx, launch(f0, dim3(nb32), dim3(256), 0, ctx->stream, M.soa.p, ld_dev, n64, ctx->d_konst, M.first.p,
-                       M.evals.p, M.trap.p, pend, M.stage.p, M.stage_id.p));
+    if (stage1 == 2) {
+      CC_HIP(ctx, M.dec.ensure(n + 1));
+      if (widths) CC_HIP(ctx, M.widths.ensure(4 * n * sizeof(float)));
+      CC_HIP(ctx, launch(f0, dim3(nb32), dim3(256), 0, ctx->stream, M.soa.p, ld_dev, n64, ctx->d_konst, M.first.p,
+                         M.evals.p, M.trap.p, M.dec.p, widths ? M.widths.p : nullptr, pend, M.stage.p, M.stage_id.p));
+    } else {
+      CC_HIP(ctx, launch(f0, dim3(nb32), dim3(256), 0, ctx->stream, M.soa.p, ld_dev, n64, ctx->d_konst, M.first.p,
+                         M.evals.p, M.trap.p, pend, M.stage.p, M.stage_id.p));
+    }
     CC_HIP(ctx, launch(fs, dim3(1), dim3(1024), 0, ctx->stream, pend, nb32, offs));
     CC_HIP(ctx, launch(fl, dim3(nb32), dim3(256), 0, ctx->stream, M.stage.p, M.stage_id.p, n64, pend, offs,
                        M.packed.p, M.list.p));
@@ -357,13 +371,13 @@ int cc_c2_eval_first_ex(cc_ctx* ctx, const double* soa, size_t n, size_t ld, con
   if (n) {
     hipFunction_t f;
     CC_HIP(ctx, kernel(ctx, "ccrd_c2_first", &f));
-    const int skip0 = cascade ? 1 : 0;
+    const int done0 = stage1;  // evaluations spent on criterion 0 in stage 1
     // one position per lane; the cascade's count is known on the device
     // only, so its grid covers a quarter of the boxes, grid-stride
     uint64_t blocks = (n + 255) / 256;
     if (cascade) blocks = (blocks + 3) / 4;
     CC_HIP(ctx, launch(f, dim3((unsigned)blocks), dim3(256), 0, ctx->stream, data, ld_dev, idx, n64, m_dev, bstart,
-                       ctx->d_konst, M.first.p, M.evals.p, M.trap.p, skip0, by_pos));
+                       ctx->d_konst, M.first.p, M.evals.p, M.trap.p, done0, by_pos));
   }
   CC_HIP(ctx, hipEventRecord(e[3], ctx->stream));
   uint32_t m = (uint32_t)n;
@@ -371,6 +385,11 @@ int cc_c2_eval_first_ex(cc_ctx* ctx, const double* soa, size_t n, size_t ld, con
   CC_HIP(ctx, hipMemcpyAsync(first, M.first.p, n, hipMemcpyDeviceToHost, ctx->stream));
   CC_HIP(ctx, hipMemcpyAsync(evals, M.evals.p, n, hipMemcpyDeviceToHost, ctx->stream));
   CC_HIP(ctx, hipMemcpyAsync(trap, M.trap.p, n, hipMemcpyDeviceToHost, ctx->stream));
+  if (stage1 == 2 && n) {
+    CC_HIP(ctx, hipMemcpyAsync(f32_decided, M.dec.p, n, hipMemcpyDeviceToHost, ctx->stream));
+    if (widths)
+      CC_HIP(ctx, hipMemcpyAsync(widths, M.widths.p, 4 * n * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+  }
   CC_HIP(ctx, hipStreamSynchronize(ctx->stream));
   if (stats) {
     float s1 = 0, rk = 0, s2 = 0;
@@ -390,6 +409,18 @@ int cc_c2_eval_first_ex(cc_ctx* ctx, const double* soa, size_t n, size_t ld, con
     }
   }
   return CC_OK;
+}
+
+int cc_c2_eval_first_ex(cc_ctx* ctx, const double* soa, size_t n, size_t ld, const cc_ranker* model, int cascade,
+                        uint8_t* first, uint8_t* evals, uint8_t* trap, cc_first_stats* stats) {
+  return first_success(ctx, soa, n, ld, model, cascade ? 1 : 0, first, evals, trap, nullptr, nullptr, stats);
+}
+
+int cc_c2_eval_first_f32_ex(cc_ctx* ctx, const double* soa, size_t n, size_t ld, int cascade, uint8_t* first,
+                            uint8_t* evals, uint8_t* trap, uint8_t* f32_decided, float* widths,
+                            cc_first_stats* stats) {
+  if (cascade) return first_success(ctx, soa, n, ld, nullptr, 2, first, evals, trap, f32_decided, widths, stats);
+  return cc_c2_eval_first_f32(ctx, soa, n, ld, first, evals, trap, f32_decided, widths, stats);
 }
 
 int cc_c2_eval_first_f32(cc_ctx* ctx, const double* soa, size_t n, size_t ld, uint8_t* first, uint8_t* evals,

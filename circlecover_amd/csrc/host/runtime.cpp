@@ -173,30 +173,40 @@ int cc_eval_batch_async(cc_ctx* ctx, int prover, int precision, const double* d_
   const uint64_t blocks = (n + block - 1) / block;
   if (blocks > 0x7fffffffull || n > 0xffffffffull) return fail(ctx, CC_E_ARG, "batch too large");
   uint64_t n64 = n, ld64 = ld;
-  // Boxes whose cover-strip search the first pass defers: a counter, then a
-  // list of up to n box indices.
+  // Boxes whose cover-strip search the first pass defers: a counter, a list
+  // of up to n box indices, and their data (8 rows of stride n).
   DevBuf& db = scratch(ctx).c2_defer;
-  CC_HIP(ctx, db.ensure(256 + 4 * n));
+  CC_HIP(ctx, db.ensure(256 + 4 * n + 8 * 8 * n));
   unsigned long long* d_cnt = db.as<unsigned long long>();
-  uint32_t* d_list = reinterpret_cast<uint32_t*>(db.as<char>() + 256);
+  double* d_dstage = reinterpret_cast<double*>(db.as<char>() + 256);
+  uint32_t* d_list = reinterpret_cast<uint32_t*>(d_dstage + 8 * n);
   hipFunction_t f_zero, f_rest;
   CC_HIP(ctx, kernel(ctx, "cc_zero_u64", &f_zero));
   CC_HIP(ctx, kernel(ctx, "ccrd_c2_strip_deferred", &f_rest));
   uint64_t one = 1;
+  // Per-wave staging for criterion 0's second pass: 5 FP64 rows and the box
+  // indices over all ns lanes of the launch, then one count per wave.
+  const uint64_t ns = blocks * block, nwaves = ns / 64;
+  DevBuf& rb = scratch(ctx).c2_r1;
+  CC_HIP(ctx, rb.ensure(ns * (5 * sizeof(double) + sizeof(uint32_t)) + nwaves * sizeof(uint32_t)));
+  double* d_r1_stage = rb.as<double>();
+  uint32_t* d_r1_id = reinterpret_cast<uint32_t*>(d_r1_stage + 5 * ns);
+  uint32_t* d_r1_cnt = d_r1_id + ns;
   CC_HIP(ctx, launch(f_zero, dim3(1), dim3(64), 0, ctx->stream, d_cnt, one));
   CC_HIP(ctx, launch(ctx->f_c2_eval, dim3((unsigned)blocks), dim3(block), 0, ctx->stream, d_soa, n64, ld64,
-                     ctx->d_konst, d_mask, d_first, d_list, d_cnt));
+                     ctx->d_konst, d_mask, d_first, d_list, d_cnt, d_dstage, d_r1_stage, d_r1_id, d_r1_cnt, ns));
   // The deferred boxes are few (well under 1% of a typical batch); 1024
   // blocks of 16 groups of 16 lanes cover 16384 of them in one sweep and
   // stride over more.
-  CC_HIP(ctx, launch(f_rest, dim3(1024), dim3(block), 0, ctx->stream, d_soa, ld64, ctx->d_konst,
+  CC_HIP(ctx, launch(f_rest, dim3(1024), dim3(block), 0, ctx->stream, (const double*)d_dstage, n64, ctx->d_konst,
                      (const uint32_t*)d_list, (const unsigned long long*)d_cnt, d_mask, d_first));
-  // Criterion 0's remaining strategies: one 2048-box chunk per block.
+  // Criterion 0's remaining strategies: the staging slots of 32 waves per block.
   hipFunction_t f_r1;
   CC_HIP(ctx, kernel(ctx, "ccrd_c2_r1_pass", &f_r1));
-  const uint64_t r1_chunks = (n + 2047) / 2048;
-  CC_HIP(ctx, launch(f_r1, dim3((unsigned)(r1_chunks < 8192 ? r1_chunks : 8192)), dim3(block), 0, ctx->stream, d_soa,
-                     n64, ld64, ctx->d_konst, d_mask, d_first));
+  const uint64_t r1_blocks = (nwaves + 31) / 32;
+  CC_HIP(ctx, launch(f_r1, dim3((unsigned)r1_blocks), dim3(block), 0, ctx->stream, (const double*)d_r1_stage,
+                     (const uint32_t*)d_r1_id, (const uint32_t*)d_r1_cnt, nwaves, ns, ctx->d_konst, d_mask,
+                     d_first));
   return CC_OK;
 }
 

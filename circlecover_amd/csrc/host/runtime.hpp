@@ -102,7 +102,9 @@ hipError_t launch(hipFunction_t f, dim3 grid, dim3 block, size_t shmem, hipStrea
 // Level buffers shared by the enumeration and branch-and-bound drivers.
 struct Scratch {
   DevBuf cells, aux, s1, s2, s3, leaves, next, verdict;
-  DevBuf c2_defer;                      // batch kernel: deferred-box counter (first 256 B) and list
+  DevBuf c2_defer;                      // batch kernel: deferred-box counter (first 256 B), data, list
+  DevBuf c2_r1;                         // batch kernel: per-wave staging of the boxes criterion 0's
+                                        // first strategy leaves open
   void* fast = nullptr;                 // batched branch-and-bound state (rsb_fast.cpp)
   void (*fast_free)(void*) = nullptr;
   void* ml = nullptr;                   // ranker buffers (ml.cpp)

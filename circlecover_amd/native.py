@@ -103,6 +103,7 @@ lib.cc_ranker_rank.argtypes = [_vp, _vp, _sz, _sz, _vp, _vp, _vp]
 lib.cc_c2_eval_first.argtypes = [_vp, _vp, _sz, _sz, _vp, _vp, _vp, _vp, _vp]
 lib.cc_c2_eval_first_ex.argtypes = [_vp, _vp, _sz, _sz, _vp, _i, _vp, _vp, _vp, _vp]
 lib.cc_c2_eval_first_f32.argtypes = [_vp, _vp, _sz, _sz, _vp, _vp, _vp, _vp, _vp, _vp]
+lib.cc_c2_eval_first_f32_ex.argtypes = [_vp, _vp, _sz, _sz, _i, _vp, _vp, _vp, _vp, _vp, _vp]
 lib.cc_tr_find_critical_in.argtypes = [_vp, _vp, _i, _vp, _sz, _P(_sz)]
 lib.cc_tr_enumerate.argtypes = [_vp, _vp, _sz, _vp, _vp, _sz, _P(_sz)]
 lib.cc_tr_bnb.argtypes = [_vp, _vp, _sz, _vp, _i, _vp, _sz, _P(_sz), _vp]
@@ -381,16 +382,18 @@ class Prover:
                                             _ptr(first), _ptr(evals), _ptr(trap), ctypes.byref(st)))
         return first, evals, trap, st
 
-    def eval_first_f32(self, soa: np.ndarray, with_widths: bool = False):
-        """FP32 pre-filter + FP64 plain first-success: returns (first, evals,
-        trap, f32_decided, widths or None, FirstStats)."""
+    def eval_first_f32(self, soa: np.ndarray, with_widths: bool = False, cascade: bool = False):
+        """FP32 pre-filter + FP64 plain first-success (cascade: in the
+        cascade form, same outputs): returns (first, evals, trap,
+        f32_decided, widths or None, FirstStats)."""
         soa = np.ascontiguousarray(soa, dtype=np.float64)
         n = soa.shape[1]
         first, evals, trap, dec = (np.empty(n, dtype=np.uint8) for _ in range(4))
         w = np.empty((n, 4), dtype=np.float32) if with_widths else None
         st = FirstStats()
-        self._check(lib.cc_c2_eval_first_f32(self.ctx, _ptr(soa), n, n, _ptr(first), _ptr(evals), _ptr(trap),
-                                             _ptr(dec), _ptr(w) if with_widths else None, ctypes.byref(st)))
+        self._check(lib.cc_c2_eval_first_f32_ex(self.ctx, _ptr(soa), n, n, int(bool(cascade)), _ptr(first),
+                                                _ptr(evals), _ptr(trap), _ptr(dec), _ptr(w) if with_widths else None,
+                                                ctypes.byref(st)))
         return first, evals, trap, dec, w, st
 
     def rsb_eval_masks(self, cells: np.ndarray) -> np.ndarray:

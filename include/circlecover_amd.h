@@ -303,6 +303,13 @@ int cc_interrupt(cc_ctx* ctx);
  * in FP64 (cr32, cr64, rem32, rem64). */
 int cc_c2_eval_first_f32(cc_ctx* ctx, const double* soa, size_t n, size_t ld, uint8_t* first, uint8_t* evals,
                          uint8_t* trap, uint8_t* f32_decided, float* widths, cc_first_stats* stats);
+/* As cc_c2_eval_first_f32; cascade != 0 runs it in the cascade form of
+ * cc_c2_eval_first_ex: the FP32 shadow of criterion 0 on every box, FP64
+ * criterion 0 on the boxes it leaves undecided, the other criteria on the
+ * boxes still open.  Same outputs as cascade = 0. */
+int cc_c2_eval_first_f32_ex(cc_ctx* ctx, const double* soa, size_t n, size_t ld, int cascade, uint8_t* first,
+                            uint8_t* evals, uint8_t* trap, uint8_t* f32_decided, float* widths,
+                            cc_first_stats* stats);
 
 /* get_subinterval (Prover/subintervals.hpp:32-43) on the host: the n pieces
  * of [lb, ub] as 2n doubles (lb + diff*i with one rounding, the last piece

@@ -35,3 +35,17 @@ def test_gpu_f32_prefilter_identity(prover):
     # reference's inward scalar-product quirk, so this is checked on the bulk)
     assert (w[:, 0] >= w[:, 1]).mean() > 0.999 and (w[:, 2] >= w[:, 3]).mean() > 0.999
     assert np.median(w[:, 2] / np.maximum(w[:, 3], 1e-300)) > 1.0
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("n", [0, 1, 2049, 1 << 15])
+def test_gpu_f32_cascade_matches_per_box_form(prover, n):
+    """The cascade form of the pre-filter (FP32 shadow of criterion 0 on every
+    box, FP64 criterion 0 on the undecided ones, the rest on the boxes still
+    open) gives the per-box form's outputs exactly."""
+    soa = gen_c2_boxes(SEED + 7, n)
+    a = prover.eval_first_f32(soa, with_widths=True)
+    b = prover.eval_first_f32(soa, with_widths=True, cascade=True)
+    for x, y in zip(a[:5], b[:5]):
+        assert (x == y).all()
+    assert a[5].proven == b[5].proven and a[5].evals_proven == b[5].evals_proven
