@@ -35,14 +35,9 @@ CCD int rsb_chain_verdict(const Konst& K, const cell7& in, Trap& tp) {
   return -1;
 }
 
-// Steps 0..18 of rsb_cell_verdict (everything before C14): the verdict if
-// one of them proves the cell, -2 if only C14 is left to try.
-CCD int rsb_verdict_before_c14(const Konst& K, const cell7& in, Trap& tp) {
-  if (rsbc::s1_even_split(in.la, in.r[0], in.r[1])) return 0;
-  if (rsbc::s23_uneven_split(in.la, in.r[0], in.r[1])) return 1;
-  if (rsbc::s23_uneven_split(in.la, in.r[0], in.r[2])) return 2;
-  if (rsbc::s4_opposite_corners(in.la, in.r[0], in.r[1], in.r[2])) return 3;
-  if (rsbc::s5_vertical_strip(K, in.la, in.r[0], in.r[1], in.r[2])) return 4;
+// The leaf chain of rsb_chain_verdict up to C13: its verdict if one of
+// those steps proves the cell, -2 if only C14 is left to try.
+CCD int rsb_chain_before_c14(const Konst& K, const cell7& in, Trap& tp) {
   const cell7 c = rsbc::tighten(in);
   vals7 v;
   v.R = rsbc::remaining_weight(c);
@@ -64,6 +59,17 @@ CCD int rsb_verdict_before_c14(const Konst& K, const cell7& in, Trap& tp) {
   if (rsbc::c12_five_disk(K, c, v, tp)) return 17;
   if (rsbc::c13_six_disk(K, c, v, tp)) return 18;
   return -2;
+}
+
+// Steps 0..18 of rsb_cell_verdict (everything before C14): the verdict if
+// one of them proves the cell, -2 if only C14 is left to try.
+CCD int rsb_verdict_before_c14(const Konst& K, const cell7& in, Trap& tp) {
+  if (rsbc::s1_even_split(in.la, in.r[0], in.r[1])) return 0;
+  if (rsbc::s23_uneven_split(in.la, in.r[0], in.r[1])) return 1;
+  if (rsbc::s23_uneven_split(in.la, in.r[0], in.r[2])) return 2;
+  if (rsbc::s4_opposite_corners(in.la, in.r[0], in.r[1], in.r[2])) return 3;
+  if (rsbc::s5_vertical_strip(K, in.la, in.r[0], in.r[1], in.r[2])) return 4;
+  return rsb_chain_before_c14(K, in, tp);
 }
 
 // C14 alone on a cell that reached it: 19 if it proves the cell, else -1.

@@ -71,18 +71,26 @@ int read_counts(cc_ctx* ctx, int k) {
 }
 
 // Enumeration of the device cells with counts n[4] (r3..r6); leaves land in
-// scratch().leaves.  Counter slots: 0 level A, 1 leaves, 2 trap (int).
+// scratch().leaves.  Counter slots: 0 level A, 1 leaves, 2 trap (int), 3
+// listed non-empty leaves, 4 leaves listed for C14 (both lists 8 bytes per
+// leaf, in scratch().enum_list / enum_b).  Level B runs over the level-A
+// records in chunks whose leaves fit the lists even if none were empty, so
+// the lists cannot overflow and the counts stay on the device until the end.
 int enumerate_dev(cc_ctx* ctx, const void* d_cells, size_t ncells, const int n[4], size_t* n_leaves, int* trap) {
   Scratch& S = scratch(ctx);
   unsigned long long* cnt = static_cast<unsigned long long*>(ctx->d_counts);
   int* d_trap = reinterpret_cast<int*>(cnt + 2);
-  hipFunction_t fa, fb;
+  hipFunction_t fa, frows, fla, flb;
   CC_HIP(ctx, kernel(ctx, "ccrd_rsb_enum_r3", &fa));
-  CC_HIP(ctx, kernel(ctx, "ccrd_rsb_enum_leaf", &fb));
+  CC_HIP(ctx, kernel(ctx, "ccrd_rsb_enum_rows", &frows));
+  CC_HIP(ctx, kernel(ctx, "ccrd_rsb_enum_leaf_a", &fla));
+  CC_HIP(ctx, kernel(ctx, "ccrd_rsb_enum_leaf_b", &flb));
   *n_leaves = 0;
   *trap = 0;
-  CC_HIP(ctx, hipMemsetAsync(cnt, 0, 4 * sizeof(unsigned long long), ctx->stream));
+  CC_HIP(ctx, hipMemsetAsync(cnt, 0, 5 * sizeof(unsigned long long), ctx->stream));
   if (ncells == 0) return CC_OK;
+  for (int k = 1; k < 4; ++k)
+    if (n[k] < 1 || n[k] > 256) return fail(ctx, CC_E_ARG, "r4..r6 piece counts must be 1..256");
   const uint64_t nc = ncells, ta = nc * (uint64_t)n[0];
   CC_HIP(ctx, S.s1.ensure(ta * kS3));
   CC_HIP(ctx, launch(fa, dim3(blocks_for(ta, 128)), dim3(128), 0, ctx->stream, d_cells, nc, n[0], ctx->d_konst,
@@ -90,16 +98,34 @@ int enumerate_dev(cc_ctx* ctx, const void* d_cells, size_t ncells, const int n[4
   int rc = read_counts(ctx, 1);
   if (rc) return rc;
   const uint64_t ca = ctx->h_counts[0];
-  const uint64_t tb = ca * (uint64_t)n[1] * (uint64_t)n[2] * (uint64_t)n[3];
+  if (ca > 0xffffffffull) return fail(ctx, CC_E_ARG, "enumeration too large");
+  const uint64_t per = (uint64_t)n[1] * (uint64_t)n[2] * (uint64_t)n[3];  // leaves per level-A record
+  constexpr uint64_t kListLeaves = 1ull << 25;                           // list room: 256 MiB each
+  constexpr size_t kRef = 8;                                              // bytes per listed leaf
+  const uint64_t chunk = per >= kListLeaves ? 1 : kListLeaves / per;     // records per chunk
+  const uint64_t room = chunk * per;
   uint64_t cb = 0;
-  if (tb) {
-    if (blocks_for(tb, 128) >= 0x7fffffffu) return fail(ctx, CC_E_ARG, "enumeration too large for one launch");
+  if (ca) {
+    CC_HIP(ctx, S.enum_list.ensure(room * kRef));
+    CC_HIP(ctx, S.enum_b.ensure(room * kRef));
     if (!S.leaves.p) CC_HIP(ctx, S.leaves.ensure((size_t)1 << 20));
-    for (int pass = 0; pass < 2; ++pass) {
+    const unsigned grid = 8192;  // stages A and B: grid-stride over the lists
+    for (int pass = 0; pass < 2; ++pass) {  // again with room for every unproven leaf if they overflowed
       const uint64_t cap = S.leaves.cap / kRec;
       CC_HIP(ctx, hipMemsetAsync(cnt + 1, 0, sizeof(unsigned long long), ctx->stream));
-      CC_HIP(ctx, launch(fb, dim3(blocks_for(tb, 128)), dim3(128), 0, ctx->stream, d_cells, S.s1.p, ca, n[1], n[2],
-                         n[3], ctx->d_konst, S.leaves.p, cap, cnt + 1, d_trap));
+      for (uint64_t r0 = 0; r0 < ca; r0 += chunk) {
+        const uint64_t nr = ca - r0 < chunk ? ca - r0 : chunk;
+        const uint64_t rows = nr * (uint64_t)n[1] * (uint64_t)n[2];
+        CC_HIP(ctx, hipMemsetAsync(cnt + 3, 0, 2 * sizeof(unsigned long long), ctx->stream));
+        CC_HIP(ctx, launch(frows, dim3(blocks_for(rows, 128)), dim3(128), 0, ctx->stream, d_cells, S.s1.p, r0, nr,
+                           n[1], n[2], n[3], S.enum_list.p, cnt + 3));
+        CC_HIP(ctx, launch(fla, dim3(grid), dim3(128), 0, ctx->stream, d_cells, S.s1.p, S.enum_list.p,
+                           (const unsigned long long*)(cnt + 3), n[1], n[2], n[3], ctx->d_konst, S.enum_b.p, cnt + 4,
+                           d_trap));
+        CC_HIP(ctx, launch(flb, dim3(grid), dim3(128), 0, ctx->stream, d_cells, S.s1.p, S.enum_b.p,
+                           (const unsigned long long*)(cnt + 4), n[1], n[2], n[3], ctx->d_konst, S.leaves.p, cap,
+                           cnt + 1, d_trap));
+      }
       if ((rc = read_counts(ctx, 3))) return rc;
       cb = ctx->h_counts[1];
       if (cb <= cap) break;

@@ -94,6 +94,11 @@ struct DevBuf {
 // Launch helper over hipModuleLaunchKernel with a packed argument list.
 template <typename... A>
 hipError_t launch(hipFunction_t f, dim3 grid, dim3 block, size_t shmem, hipStream_t s, A... args) {
+  // The dispatch packet holds the grid in work-items as 32-bit numbers per
+  // dimension: a larger grid would wrap and silently run fewer work-items.
+  if ((uint64_t)grid.x * block.x > 0xffffffffull || (uint64_t)grid.y * block.y > 0xffffffffull ||
+      (uint64_t)grid.z * block.z > 0xffffffffull)
+    return hipErrorInvalidConfiguration;
   void* argv[] = {(void*)&args...};
   return hipModuleLaunchKernel(f, grid.x, grid.y, grid.z, block.x, block.y, block.z, (unsigned)shmem, s, argv,
                                nullptr);
@@ -103,6 +108,7 @@ hipError_t launch(hipFunction_t f, dim3 grid, dim3 block, size_t shmem, hipStrea
 struct Scratch {
   DevBuf cells, aux, s1, s2, s3, leaves, next, verdict;
   DevBuf c2_defer;                      // batch kernel: deferred-box counter (first 256 B), data, list
+  DevBuf enum_list, enum_b;             // RSB enumeration: non-empty leaves, leaves for C14 (8 B each)
   DevBuf c2_r1;                         // batch kernel: per-wave staging of the boxes criterion 0's
                                         // first strategy leaves open
   void* fast = nullptr;                 // batched branch-and-bound state (rsb_fast.cpp)

@@ -1,0 +1,199 @@
+// The enumeration of the reference's per-cell kernel
+// (kernel_find_critical_intervals_in,
+// Prover/rectangle_size_bound/rectangle_size_bound.cu:161-236) flattened
+// into two levels over a batch of cells (gfx950, round-down mode), for the
+// drop-in proof.  Built with the exact shortcuts: every criterion returns
+// what the reference's operation sequence returns, sooner.
+#define CC_EXACT_SHORTCUTS 1
+#include "dev/queue.hpp"
+#include "dev/rsb_rec.hpp"
+
+using namespace ccd;
+
+// Level A, one lane per (cell, r3 piece): the cell shortcuts S1/S2
+// (rectangle_size_bound.cu:162) and the r3 shortcuts S3..S5 (:172-178);
+// surviving (cell, r3) pairs are compacted.
+struct rsb_s3 {
+  uint32_t cell, pad;
+  iv r3;
+};
+
+extern "C" __global__ void __launch_bounds__(128) ccrd_rsb_enum_r3(const rsb_rec* __restrict__ cells, uint64_t ncells,
+                                                                    int n3, const Konst* __restrict__ kp,
+                                                                    rsb_s3* __restrict__ out,
+                                                                    unsigned long long* __restrict__ count,
+                                                                    int* __restrict__ trap) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ncells * (uint64_t)n3) return;
+  const Konst K = *kp;
+  const uint32_t ci = (uint32_t)(i / (uint64_t)n3);
+  const int i3 = (int)(i % (uint64_t)n3);
+  const cell7 c = load_cell7(cells[ci].v);
+  rsb_s3 rec;
+  rec.cell = ci;
+  rec.pad = 0;
+  rec.r3 = subinterval(c.r[2], i3, n3);
+  const bool keep = !rsbc::s1_even_split(c.la, c.r[0], c.r[1]) && !rsbc::s23_uneven_split(c.la, c.r[0], c.r[1]) &&
+                    !rsbc::s23_uneven_split(c.la, c.r[0], rec.r3) &&
+                    !rsbc::s4_opposite_corners(c.la, c.r[0], c.r[1], rec.r3) &&
+                    !rsbc::s5_vertical_strip(K, c.la, c.r[0], c.r[1], rec.r3);
+  const uint64_t slot = wave_append(keep, count);
+  if (keep) out[slot] = rec;
+  (void)trap;
+}
+
+// Level B: the leaves (level-A record, r4, r5, r6 pieces) and their chain
+// (rectangle_size_bound.cu:180-233), in three steps, each dense on the
+// leaves the step before leaves open:
+//   rows      one lane per (record, r4, r5): the r6 pieces whose leaf is
+//             not empty after tightening (the chain's first test; about 94%
+//             of the leaves of a proof are empty) are listed, 8 bytes each;
+//   stage A   the chain up to C13 on the listed leaves; those only C14 can
+//             still prove are listed again;
+//   stage B   C14 on that list.
+// Run per leaf, every wave would wait for its slowest leaf: nearly every
+// wave of 64 leaves holds one that goes deep into the chain.  Unproven
+// leaves, tightened as the reference pushes them, go to `leaves` (counter
+// counts all; records past cap are dropped).
+struct leaf_ref {
+  uint32_t rec;
+  uint8_t i4, i5, i6, pad;
+};
+
+CCD cell7 leaf_of(const rsb_rec* __restrict__ cells, const rsb_s3& s, int i4, int n4, int i5, int n5, int i6,
+                  int n6) {
+  const cell7 c = load_cell7(cells[s.cell].v);
+  cell7 v;
+  v.la = c.la;
+  v.r[0] = c.r[0];
+  v.r[1] = c.r[1];
+  v.r[2] = s.r3;
+  v.r[3] = subinterval(c.r[3], i4, n4);
+  v.r[4] = subinterval(c.r[4], i5, n5);
+  v.r[5] = subinterval(c.r[5], i6, n6);
+  return rsbc::tighten(v);
+}
+
+// Rows of the level-A records [rec0, rec0 + nrec) (n4, n5, n6 <= 256).  The
+// list has room for every leaf of those records.
+extern "C" __global__ void __launch_bounds__(128) ccrd_rsb_enum_rows(const rsb_rec* __restrict__ cells,
+                                                                      const rsb_s3* __restrict__ in, uint64_t rec0,
+                                                                      uint64_t nrec, int n4, int n5, int n6,
+                                                                      leaf_ref* __restrict__ list,
+                                                                      unsigned long long* __restrict__ count_list) {
+  const uint64_t per_rec = (uint64_t)n4 * n5;
+  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool live = t < nrec * per_rec;
+  const int lane = __lane_id();
+  uint32_t rec = 0;
+  int i4 = 0, i5 = 0;
+  cell7 v{};
+  iv r6{0.0, 0.0};
+  if (live) {
+    const uint64_t r = t / per_rec;
+    const uint32_t q = (uint32_t)(t - r * per_rec);
+    i4 = (int)(q / (uint32_t)n5);
+    i5 = (int)(q % (uint32_t)n5);
+    rec = (uint32_t)(rec0 + r);
+    const rsb_s3 s = in[rec];
+    const cell7 c = load_cell7(cells[s.cell].v);
+    v.la = c.la;
+    v.r[0] = c.r[0];
+    v.r[1] = c.r[1];
+    v.r[2] = s.r3;
+    v.r[3] = subinterval(c.r[3], i4, n4);
+    v.r[4] = subinterval(c.r[4], i5, n5);
+    r6 = c.r[5];
+  }
+  for (int b6 = 0; b6 < n6; b6 += 64) {  // the same trip count on every lane
+    uint64_t mask = 0;
+    if (live) {
+      const int m = n6 - b6 < 64 ? n6 - b6 : 64;
+      for (int k = 0; k < m; ++k) {
+        v.r[5] = subinterval(r6, b6 + k, n6);
+        const cell7 c = rsbc::tighten(v);
+        if (!(rsbc::is_empty(c) || empty(rsbc::remaining_weight(c)))) mask |= 1ull << k;
+      }
+    }
+    // one atomic per wave: an exclusive scan of the lanes' counts
+    const uint32_t mine = (uint32_t)__popcll(mask);
+    uint32_t x = mine;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const uint32_t y = __shfl_up(x, d);
+      if (lane >= d) x += y;
+    }
+    const uint32_t total = __shfl(x, 63);
+    unsigned long long at = 0;
+    if (lane == 63 && total) at = atomicAdd(count_list, (unsigned long long)total);
+    at = __shfl(at, 63) + (x - mine);
+    while (mask) {
+      const int k = __builtin_ctzll(mask);
+      mask &= mask - 1;
+      leaf_ref ref;
+      ref.rec = rec;
+      ref.i4 = (uint8_t)i4;
+      ref.i5 = (uint8_t)i5;
+      ref.i6 = (uint8_t)(b6 + k);
+      ref.pad = 0;
+      list[at++] = ref;
+    }
+  }
+}
+
+// Stage A over the *count_list listed leaves, grid-stride.  listb has room
+// for all of them.
+extern "C" __global__ void __launch_bounds__(128) ccrd_rsb_enum_leaf_a(const rsb_rec* __restrict__ cells,
+                                                                        const rsb_s3* __restrict__ in,
+                                                                        const leaf_ref* __restrict__ list,
+                                                                        const unsigned long long* __restrict__ count_list,
+                                                                        int n4, int n5, int n6,
+                                                                        const Konst* __restrict__ kp,
+                                                                        leaf_ref* __restrict__ listb,
+                                                                        unsigned long long* __restrict__ count_b,
+                                                                        int* __restrict__ trap) {
+  const uint64_t n = *count_list;
+  const Konst K = *kp;
+  for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x; base < n; base += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t i = base + threadIdx.x;
+    bool keep = false;
+    leaf_ref ref{};
+    if (i < n) {
+      ref = list[i];
+      const cell7 v = leaf_of(cells, in[ref.rec], ref.i4, n4, ref.i5, n5, ref.i6, n6);
+      Trap tp{0};
+      keep = rsb_chain_before_c14(K, v, tp) == -2;
+      if (tp.flag) atomicOr(trap, 1);
+    }
+    const uint64_t slot = wave_append(keep, count_b);
+    if (keep) listb[slot] = ref;
+  }
+}
+
+// Stage B: C14 on the *count_b listed leaves, grid-stride.
+extern "C" __global__ void __launch_bounds__(128) ccrd_rsb_enum_leaf_b(const rsb_rec* __restrict__ cells,
+                                                                        const rsb_s3* __restrict__ in,
+                                                                        const leaf_ref* __restrict__ listb,
+                                                                        const unsigned long long* __restrict__ count_b,
+                                                                        int n4, int n5, int n6,
+                                                                        const Konst* __restrict__ kp,
+                                                                        rsb_rec* __restrict__ leaves, uint64_t cap,
+                                                                        unsigned long long* __restrict__ count,
+                                                                        int* __restrict__ trap) {
+  const uint64_t nb = *count_b;
+  const Konst K = *kp;
+  for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x; base < nb; base += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t i = base + threadIdx.x;
+    bool keep = false;
+    cell7 v{};
+    if (i < nb) {
+      const leaf_ref ref = listb[i];
+      v = leaf_of(cells, in[ref.rec], ref.i4, n4, ref.i5, n5, ref.i6, n6);
+      Trap tp{0};
+      keep = rsb_verdict_c14(K, v, tp) < 0;
+      if (tp.flag) atomicOr(trap, 1);
+    }
+    const uint64_t slot = wave_append(keep, count);
+    if (keep && slot < cap) store_cell7(&leaves[slot], v);
+  }
+}

@@ -101,6 +101,8 @@ lib.cc_ranker_train.argtypes = [_vp, _vp, _sz, _sz, _vp, _i, _i, ctypes.c_float,
 lib.cc_ranker_gradient.argtypes = [_vp, _vp, _sz, _sz, _vp, _vp, _i, _vp, _vp, _vp]
 lib.cc_ranker_rank.argtypes = [_vp, _vp, _sz, _sz, _vp, _vp, _vp]
 lib.cc_c2_eval_first.argtypes = [_vp, _vp, _sz, _sz, _vp, _vp, _vp, _vp, _vp]
+lib.cc_format_interval.argtypes = [_vp, ctypes.c_char_p, _sz, _P(_sz)]
+lib.cc_parse_interval.argtypes = [ctypes.c_char_p, _sz, _vp, _P(_sz)]
 lib.cc_c2_eval_first_ex.argtypes = [_vp, _vp, _sz, _sz, _vp, _i, _vp, _vp, _vp, _vp]
 lib.cc_c2_eval_first_f32.argtypes = [_vp, _vp, _sz, _sz, _vp, _vp, _vp, _vp, _vp, _vp]
 lib.cc_c2_eval_first_f32_ex.argtypes = [_vp, _vp, _sz, _sz, _i, _vp, _vp, _vp, _vp, _vp, _vp]

@@ -266,6 +266,21 @@ int cc_c2_eval_first(cc_ctx* ctx, const double* soa, size_t n, size_t ld, const 
 int cc_c2_eval_first_ex(cc_ctx* ctx, const double* soa, size_t n, size_t ld, const cc_ranker* model, int cascade,
                         uint8_t* first, uint8_t* evals, uint8_t* trap, cc_first_stats* stats);
 
+/* ---- interval text (host only, no context) ----
+ * The reference's interval writer and reader, so that a report can be read
+ * back and compared bit for bit. */
+/* Writes "[lb, ub]" as the reference's operator<< does
+ * (Prover/algcuda/cuda-utils/cpp/include/algcuda/interval.hpp:294-303:
+ * precision 19, each bound right-aligned in 26 columns) with a terminating
+ * NUL; *len = its length.  CC_E_OVERFLOW (with *len set) if cap <= *len. */
+int cc_format_interval(const cc_iv* x, char* buf, size_t cap, size_t* len);
+/* Reads one interval from text[0..n) as the reference's operator>> does
+ * (interval.hpp:305-351): after white space, "[a, b]" or "[a; b]", or a
+ * scalar x read as [x, x]; numbers in the decimal form an iostream reads
+ * (no inf, nan or hex).  *consumed (may be NULL) = characters used.
+ * CC_E_ARG where the reference's stream would fail. */
+int cc_parse_interval(const char* text, size_t n, cc_iv* out, size_t* consumed);
+
 /* ---- whole proofs (replace find_critical_intervals(), see also the C++
  *      declarations at the end of this header) ---- */
 

@@ -10,6 +10,7 @@ import pytest
 
 import oracle
 from circlecover_amd import TR_LAMBDA_MAX, tr_initial_cells
+from circlecover_amd.report import compare_reports, read_report
 from report_format import begin_case, end_case, iv, merge_critical, rsb_report, tr_report
 
 EXE = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "circlecover_amd",
@@ -51,6 +52,7 @@ def test_gpu_tr_prove_report_matches_oracle():
     expected = tr_report(np.array(crit))
     got = run_prover(["--only", "tr", "--tr-la-range", "0,2", "--tr-generations", "0"])
     assert got == expected
+    assert compare_reports(got, expected) == [] and read_report(got)[0].lines
 
 
 @pytest.mark.gpu
@@ -69,6 +71,7 @@ def test_gpu_rsb_prove_report_matches_oracle():
     expected = rsb_report(merge_critical(np.array(leaves), 0))
     got = run_prover(["--only", "rsb", "--rsb-grid", "256,8,8", "--rsb-la-range", "40,1", "--rsb-generations", "0"])
     assert got == expected
+    assert compare_reports(got, expected) == [] and read_report(got)[0].lines
 
 
 @pytest.mark.gpu
