@@ -236,6 +236,11 @@ def extended(p, soa: np.ndarray, mask: np.ndarray) -> dict:
     return out
 
 
+def log(msg: str) -> None:
+    """A progress line on stderr (the JSON result alone goes to stdout)."""
+    print(f"bench.py: {msg}", file=sys.stderr, flush=True)
+
+
 def full_proof() -> dict:
     """Both of the reference's proofs, end to end, through the drop-in
     executable (reference constants: tight_rectangle 256 lambda launches of
@@ -249,8 +254,9 @@ def full_proof() -> dict:
     exe = os.path.join(ROOT, "circlecover_amd", "circlecover_prover")
     out = {}
     for proof in ("tr", "rsb"):
+        log(f"full proof {proof} (the prover's progress follows on stderr)")
         t = time.perf_counter()
-        r = subprocess.run([exe, "--only", proof], capture_output=True, text=True, timeout=600)
+        r = subprocess.run([exe, "--only", proof], stdout=subprocess.PIPE, text=True, timeout=600)
         wall = time.perf_counter() - t
         lines = [ln for ln in r.stdout.splitlines() if ln.strip()]
         rows = max(0, len(lines) - 2) if r.returncode == 0 else None
@@ -286,6 +292,7 @@ def main() -> int:
         d_mask = p.alloc(4 * n)
         d_first = p.alloc(n)
         d_soa.upload(soa)
+        log(f"batch of {n} boxes: {args.warmup} warm-up and {args.steps} timed steps")
         for _ in range(args.warmup):
             p.eval_batch_async(d_soa, n, n, d_mask, d_first)
         p.synchronize()
@@ -299,12 +306,16 @@ def main() -> int:
         mask = d_mask.download(np.empty(n, dtype=np.uint32))
         for b in (d_soa, d_mask, d_first):
             b.free()
+        if not args.core_only:
+            log("other configurations")
         workloads = None if args.core_only else extended(p, soa, mask)
 
     ms_per_step = 1e3 * wall / args.steps
     value = n * N_CRIT / (wall / args.steps)
     launch_ms = ev_ms / args.steps
 
+    if not args.no_traffic:
+        log("HBM counter runs")
     traffic = None if args.no_traffic else measured_traffic()
 
     sample = soa[:, : min(n, 1 << 18)]
@@ -341,6 +352,7 @@ def main() -> int:
     if not args.no_cpu:
         threads = len(os.sched_getaffinity(0))
         threads = max(1, min(threads, 16))
+        log(f"CPU oracle baseline on {threads} threads")
         out["cpu_baseline"] = cpu_baseline(soa, threads)
     print(json.dumps(out), flush=True)
     return 0

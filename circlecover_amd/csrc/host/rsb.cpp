@@ -74,8 +74,9 @@ int read_counts(cc_ctx* ctx, int k) {
 // scratch().leaves.  Counter slots: 0 level A, 1 leaves, 2 trap (int), 3
 // listed non-empty leaves, 4 leaves listed for C14 (both lists 8 bytes per
 // leaf, in scratch().enum_list / enum_b).  Level B runs over the level-A
-// records in chunks whose leaves fit the lists even if none were empty, so
-// the lists cannot overflow and the counts stay on the device until the end.
+// records in chunks whose leaves fit the lists even if none were empty (up
+// to 2^28 leaves a chunk), so the lists cannot overflow and the counts stay
+// on the device until the end.
 int enumerate_dev(cc_ctx* ctx, const void* d_cells, size_t ncells, const int n[4], size_t* n_leaves, int* trap) {
   Scratch& S = scratch(ctx);
   unsigned long long* cnt = static_cast<unsigned long long*>(ctx->d_counts);
@@ -100,10 +101,10 @@ int enumerate_dev(cc_ctx* ctx, const void* d_cells, size_t ncells, const int n[4
   const uint64_t ca = ctx->h_counts[0];
   if (ca > 0xffffffffull) return fail(ctx, CC_E_ARG, "enumeration too large");
   const uint64_t per = (uint64_t)n[1] * (uint64_t)n[2] * (uint64_t)n[3];  // leaves per level-A record
-  constexpr uint64_t kListLeaves = 1ull << 25;                           // list room: 256 MiB each
+  constexpr uint64_t kListLeaves = 1ull << 28;                           // list room: up to 2 GiB each
   constexpr size_t kRef = 8;                                              // bytes per listed leaf
   const uint64_t chunk = per >= kListLeaves ? 1 : kListLeaves / per;     // records per chunk
-  const uint64_t room = chunk * per;
+  const uint64_t room = (ca < chunk ? ca : chunk) * per;
   uint64_t cb = 0;
   if (ca) {
     CC_HIP(ctx, S.enum_list.ensure(room * kRef));
