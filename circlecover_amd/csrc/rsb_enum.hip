@@ -105,12 +105,19 @@ extern "C" __global__ void __launch_bounds__(128) ccrd_rsb_enum_rows(const rsb_r
     v.r[4] = subinterval(c.r[4], i5, n5);
     r6 = c.r[5];
   }
+  // the r6 pieces as subinterval(r6, i, n6) computes them: one quotient for
+  // the row, and each piece's upper bound is the next piece's lower bound
+  const double d6 = live ? rn_span_div(r6.hi, r6.lo, (double)n6) : 0.0;
   for (int b6 = 0; b6 < n6; b6 += 64) {  // the same trip count on every lane
     uint64_t mask = 0;
     if (live) {
       const int m = n6 - b6 < 64 ? n6 - b6 : 64;
+      double l = rn_fma(d6, (double)b6, r6.lo);
       for (int k = 0; k < m; ++k) {
-        v.r[5] = subinterval(r6, b6 + k, n6);
+        const int i = b6 + k;
+        const double u = i + 1 == n6 ? r6.hi : rn_fma(d6, (double)(i + 1), r6.lo);
+        v.r[5] = iv{l, u};
+        l = u;
         const cell7 c = rsbc::tighten(v);
         if (!(rsbc::is_empty(c) || empty(rsbc::remaining_weight(c)))) mask |= 1ull << k;
       }
@@ -142,8 +149,10 @@ extern "C" __global__ void __launch_bounds__(128) ccrd_rsb_enum_rows(const rsb_r
 }
 
 // Stage A over the *count_list listed leaves, grid-stride.  listb has room
-// for all of them.
-extern "C" __global__ void __launch_bounds__(128) ccrd_rsb_enum_leaf_a(const rsb_rec* __restrict__ cells,
+// for all of them.  Four waves per SIMD (128 VGPRs, the rest of the chain's
+// state in scratch): the chain's long FP64 dependency chains need the
+// latency hiding more than the registers (3 or 2 waves were slower).
+extern "C" __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(4, 4))) ccrd_rsb_enum_leaf_a(const rsb_rec* __restrict__ cells,
                                                                         const rsb_s3* __restrict__ in,
                                                                         const leaf_ref* __restrict__ list,
                                                                         const unsigned long long* __restrict__ count_list,
