@@ -181,6 +181,7 @@ def extended(p, soa: np.ndarray, mask: np.ndarray) -> dict:
     # FP32 pre-filter vs FP64 first-success, per-box and cascade forms
     p.eval_first(soa)
     f64_first, _, _, st64 = p.eval_first(soa)
+    p.eval_first(soa, cascade=True)
     _, _, _, st64c = p.eval_first(soa, cascade=True)
     _, _, _, _, w, _ = p.eval_first_f32(soa, with_widths=True)  # widths for the inflation report, untimed
     first, evals, trap, dec, _, st32 = p.eval_first_f32(soa)

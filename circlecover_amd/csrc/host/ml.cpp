@@ -289,6 +289,24 @@ static int first_success(cc_ctx* ctx, const double* soa, size_t n, size_t ld, co
   CC_HIP(ctx, M.evals.ensure(n + 1));
   CC_HIP(ctx, M.trap.ensure(n + 1));
   CC_HIP(ctx, M.bstart.ensure(16 * sizeof(uint64_t)));
+  // every buffer is in place before the timed region starts
+  constexpr uint32_t kChunk = 2048;  // boxes per ccrd_c2_first0 block
+  const uint64_t nb = (n + kChunk - 1) / kChunk;
+  if (cascade && n) {
+    CC_HIP(ctx, M.pend.ensure(2 * (nb + 1) * sizeof(uint32_t)));
+    CC_HIP(ctx, M.list.ensure(n * sizeof(uint32_t) + 4));
+    CC_HIP(ctx, M.stage.ensure(8 * n * sizeof(double)));
+    CC_HIP(ctx, M.stage_id.ensure(n * sizeof(uint32_t)));
+    CC_HIP(ctx, M.packed.ensure(8 * n * sizeof(double)));
+    if (stage1 == 2) {
+      CC_HIP(ctx, M.dec.ensure(n + 1));
+      if (widths) CC_HIP(ctx, M.widths.ensure(4 * n * sizeof(float)));
+    }
+  }
+  if (model && n) {
+    CC_HIP(ctx, M.top1.ensure(n + 1));
+    CC_HIP(ctx, M.counts.ensure(2 * (size_t)1024 * kOut * sizeof(unsigned long long)));
+  }
   hipEvent_t e[4];
   for (auto& x : e) CC_HIP(ctx, hipEventCreate(&x));
   struct Cleanup {
@@ -308,13 +326,6 @@ static int first_success(cc_ctx* ctx, const double* soa, size_t n, size_t ld, co
   const uint32_t* list = nullptr;   // box of each position (null: the position)
   const uint32_t* m_dev = nullptr;  // null: n positions
   if (cascade && n) {
-    constexpr uint32_t kChunk = 2048;  // boxes per ccrd_c2_first0 block
-    const uint64_t nb = (n + kChunk - 1) / kChunk;
-    CC_HIP(ctx, M.pend.ensure(2 * (nb + 1) * sizeof(uint32_t)));
-    CC_HIP(ctx, M.list.ensure(n * sizeof(uint32_t) + 4));
-    CC_HIP(ctx, M.stage.ensure(8 * n * sizeof(double)));
-    CC_HIP(ctx, M.stage_id.ensure(n * sizeof(uint32_t)));
-    CC_HIP(ctx, M.packed.ensure(8 * n * sizeof(double)));
     uint32_t* pend = static_cast<uint32_t*>(M.pend.p);
     uint32_t* offs = pend + nb + 1;
     hipFunction_t f0, fs, fl;
@@ -323,8 +334,6 @@ static int first_success(cc_ctx* ctx, const double* soa, size_t n, size_t ld, co
     CC_HIP(ctx, kernel(ctx, "ccrd_c2_first0_list", &fl));
     const uint32_t nb32 = (uint32_t)nb;
     if (stage1 == 2) {
-      CC_HIP(ctx, M.dec.ensure(n + 1));
-      if (widths) CC_HIP(ctx, M.widths.ensure(4 * n * sizeof(float)));
       CC_HIP(ctx, launch(f0, dim3(nb32), dim3(256), 0, ctx->stream, M.soa.p, ld_dev, n64, ctx->d_konst, M.first.p,
                          M.evals.p, M.trap.p, M.dec.p, widths ? M.widths.p : nullptr, pend, M.stage.p, M.stage_id.p));
     } else {

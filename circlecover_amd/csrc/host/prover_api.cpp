@@ -10,6 +10,9 @@
 // launch exactly as the reference does, so the records are the same.
 #include <algorithm>
 #include <cfloat>
+#include <chrono>
+#include <functional>
+#include <string>
 #include <cmath>
 #include <csignal>
 #include <cstdio>
@@ -151,6 +154,99 @@ struct Interruptible {  // start/stop_interruptible_computation (interrupt.cpp:3
   }
 };
 
+// ------------------------------------------------------------- checkpoint
+// A proof in progress (no counterpart in the reference, whose only resume
+// point is a lambda offset): the proof's parameters, the phase (0 the
+// initial launches, g >= 1 generation g), how many launches or cells of the
+// phase are done, the records the phase started from and those it has found
+// so far.  Written to <path>.tmp, then renamed over <path>.
+template <class Rec>
+struct Checkpoint {
+  std::string path;
+  char kind[8] = {0};
+  int32_t params[8] = {0};
+  int32_t phase = 0;
+  uint64_t pos = 0;
+  std::vector<Rec> current, next;
+
+  // CC_OK and loaded = false if there is no file; CC_E_ARG if the file is
+  // not a checkpoint of this proof with these parameters.
+  int load(cc_ctx* ctx, bool& loaded) {
+    loaded = false;
+    FILE* f = std::fopen(path.c_str(), "rb");
+    if (!f) return CC_OK;
+    char magic[8], k[8];
+    int32_t pr[8], ph = 0, pad = 0;
+    uint64_t ps = 0, nc = 0, nn = 0;
+    bool ok = std::fread(magic, 1, 8, f) == 8 && !std::memcmp(magic, "CCCKPT01", 8) && std::fread(k, 1, 8, f) == 8 &&
+              !std::memcmp(k, kind, 8) && std::fread(pr, sizeof pr, 1, f) == 1 && !std::memcmp(pr, params, sizeof pr) &&
+              std::fread(&ph, 4, 1, f) == 1 && std::fread(&pad, 4, 1, f) == 1 && std::fread(&ps, 8, 1, f) == 1 &&
+              std::fread(&nc, 8, 1, f) == 1 && std::fread(&nn, 8, 1, f) == 1 && nc < (1ull << 40) && nn < (1ull << 40);
+    if (ok) {
+      current.resize(nc);
+      next.resize(nn);
+      ok = (nc == 0 || std::fread(current.data(), sizeof(Rec), nc, f) == nc) &&
+           (nn == 0 || std::fread(next.data(), sizeof(Rec), nn, f) == nn);
+    }
+    std::fclose(f);
+    if (!ok) return fail(ctx, CC_E_ARG, "checkpoint file does not match this proof");
+    phase = ph;
+    pos = ps;
+    loaded = true;
+    return CC_OK;
+  }
+  int save(cc_ctx* ctx) const {
+    const std::string tmp = path + ".tmp";
+    FILE* f = std::fopen(tmp.c_str(), "wb");
+    if (!f) return fail(ctx, CC_E_ARG, "cannot write the checkpoint file");
+    const int32_t pad = 0;
+    const uint64_t nc = current.size(), nn = next.size();
+    bool ok = std::fwrite("CCCKPT01", 1, 8, f) == 8 && std::fwrite(kind, 1, 8, f) == 8 &&
+              std::fwrite(params, sizeof params, 1, f) == 1 && std::fwrite(&phase, 4, 1, f) == 1 &&
+              std::fwrite(&pad, 4, 1, f) == 1 && std::fwrite(&pos, 8, 1, f) == 1 && std::fwrite(&nc, 8, 1, f) == 1 &&
+              std::fwrite(&nn, 8, 1, f) == 1 &&
+              (nc == 0 || std::fwrite(current.data(), sizeof(Rec), nc, f) == nc) &&
+              (nn == 0 || std::fwrite(next.data(), sizeof(Rec), nn, f) == nn);
+    ok = (std::fclose(f) == 0) && ok;
+    if (!ok || std::rename(tmp.c_str(), path.c_str()) != 0) return fail(ctx, CC_E_ARG, "cannot write the checkpoint file");
+    return CC_OK;
+  }
+};
+
+// Saves a proof's progress after its batches (every 2 s at most, and
+// always before it stops) and stops it after max_batches batches of this
+// call (0: no limit).
+template <class Rec>
+struct Progress {
+  Checkpoint<Rec>* ck = nullptr;
+  int max_batches = 0, batches = 0;
+  bool stopped = false;  // max_batches reached
+  std::chrono::steady_clock::time_point last = std::chrono::steady_clock::now();
+
+  // The phase's state after `done` of its launches or cells: CC_OK to go
+  // on, CC_E_INTERRUPTED to stop (the checkpoint then holds this state).
+  int after_batch(cc_ctx* ctx, int phase, uint64_t done, const std::vector<Rec>& current,
+                  const std::vector<Rec>& next) {
+    ++batches;
+    const bool stop = max_batches > 0 && batches >= max_batches;
+    stopped = stop;
+    const auto now = std::chrono::steady_clock::now();
+    if (ck && (stop || now - last >= std::chrono::seconds(2))) {
+      if (int rc = save(ctx, phase, done, current, next)) return rc;
+      last = now;
+    }
+    return stop ? CC_E_INTERRUPTED : CC_OK;
+  }
+  int save(cc_ctx* ctx, int phase, uint64_t done, const std::vector<Rec>& current, const std::vector<Rec>& next) {
+    if (!ck) return CC_OK;
+    ck->phase = phase;
+    ck->pos = done;
+    ck->current = current;
+    ck->next = next;
+    return ck->save(ctx);
+  }
+};
+
 // Host mirror of the device tr_maxdist block (tr_enum.hip).
 struct MaxDist {
   int have_2d, have_3d;
@@ -194,10 +290,12 @@ constexpr size_t kTrBatch = 16;
 // find_critical_intervals_in (tight_rectangle.cu:300-320) for the cells
 // `launches` (one reference launch each), appending each launch's merged
 // leaves in launch order.
+// Launches [start, ...) of `launches`; after each batch, after(done) may
+// stop the run (CC_E_INTERRUPTED) once the batch's records are appended.
 int tr_launches(cc_ctx* ctx, const std::vector<cc_tr_cell>& launches, bool init, std::vector<cc_tr_cell>& append_to,
-                std::ostream& err) {
+                std::ostream& err, size_t start, const std::function<int(size_t)>& after) {
   static const int kInit[4] = {128, 128, 128, 128}, kSub[4] = {128, 128, 128, 64};
-  for (size_t b = 0; b < launches.size(); b += kTrBatch) {
+  for (size_t b = start; b < launches.size(); b += kTrBatch) {
     if (g_interrupted || ctx->interrupt) return CC_E_INTERRUPTED;
     const size_t m = std::min(kTrBatch, launches.size() - b);
     std::vector<cc_tr_cell> leaves;
@@ -214,6 +312,7 @@ int tr_launches(cc_ctx* ctx, const std::vector<cc_tr_cell>& launches, bool init,
     }
     err << "launch " << (b + m) << " of " << launches.size() << " done (" << append_to.size()
         << " critical intervals found)                \r" << std::flush;
+    if (int rc = after(b + m)) return rc;
   }
   return CC_OK;
 }
@@ -229,7 +328,12 @@ int cc_interrupt(cc_ctx* ctx) {
 }
 
 int cc_tr_prove(cc_ctx* ctx, int la_init, int la_first, int la_count, int generations, FILE* out) {
-  if (!ctx) return CC_E_ARG;
+  return cc_tr_prove_ex(ctx, la_init, la_first, la_count, generations, nullptr, 0, out);
+}
+
+int cc_tr_prove_ex(cc_ctx* ctx, int la_init, int la_first, int la_count, int generations, const char* checkpoint,
+                   int max_batches, FILE* out) {
+  if (!ctx || max_batches < 0) return CC_E_ARG;
   static const int kSubdivSchedule[11] = {1, 2, 2, 2, 2, 1, 2, 1, 2, 1, 2};  // tight_rectangle.cu:50-53
   if (la_init <= 0) la_init = 256;
   if (la_first < 0 || la_first >= la_init) la_first = 0;
@@ -240,20 +344,50 @@ int cc_tr_prove(cc_ctx* ctx, int la_init, int la_first, int la_count, int genera
   begin_case(o, "Critical intervals for rectangles (tight)");
   const cc_tr_cell initial{{1.0, 2.0898841580413818342}, {{0.0, DBL_MAX}, {0.0, DBL_MAX}, {0.0, DBL_MAX}, {0.0, DBL_MAX}}};
   std::vector<cc_tr_cell> current, next, launches;
-  err << "Initializing..." << std::endl;
-  for (int a = la_first; a < la_first + la_count; ++a) {
-    cc_tr_cell s = initial;
-    s.la = piece(initial.la, a, la_init);
-    launches.push_back(s);
+  Checkpoint<cc_tr_cell> ck;
+  Progress<cc_tr_cell> pr;
+  bool resumed = false;
+  int rc = CC_OK;
+  if (checkpoint) {
+    ck.path = checkpoint;
+    std::memcpy(ck.kind, "TR", 2);
+    const int32_t p[4] = {la_init, la_first, la_count, generations};
+    std::memcpy(ck.params, p, sizeof p);
+    if ((rc = ck.load(ctx, resumed))) return rc;
+    pr.ck = &ck;
   }
-  int rc = tr_launches(ctx, launches, true, next, err);
-  if (rc) return rc;
-  current.swap(next);
-  next.clear();
+  pr.max_batches = max_batches;
+  const int phase0 = resumed ? ck.phase : 0;
+  size_t pos0 = resumed ? (size_t)ck.pos : 0, last_done = 0;
+  if (resumed) {
+    current = ck.current;
+    next = ck.next;
+    err << "Resuming from phase " << phase0 << " after " << pos0 << " launches" << std::endl;
+  }
+  if (phase0 == 0) {
+    err << "Initializing..." << std::endl;
+    for (int a = la_first; a < la_first + la_count; ++a) {
+      cc_tr_cell s = initial;
+      s.la = piece(initial.la, a, la_init);
+      launches.push_back(s);
+    }
+    rc = tr_launches(ctx, launches, true, next, err, pos0, [&](size_t done) {
+      last_done = done;
+      return pr.after_batch(ctx, 0, done, current, next);
+    });
+    if (rc == CC_E_INTERRUPTED && !pr.stopped) {
+      const int r2 = pr.save(ctx, 0, last_done > pos0 ? last_done : pos0, current, next);
+      if (r2) return r2;
+    }
+    if (rc) return rc;
+    current.swap(next);
+    next.clear();
+  }
   err << "Starting refinement - press Ctrl-C to interrupt and use best result achieved so far..." << std::endl;
+  bool completed = true;
   {
     Interruptible guard;
-    for (int g = 1; g <= generations; ++g) {
+    for (int g = phase0 > 1 ? phase0 : 1; g <= generations; ++g) {
       err << "Running generation " << g << " on " << current.size() << " intervals..." << std::endl;
       launches.clear();
       const int nla = kSubdivSchedule[g - 1];
@@ -263,8 +397,18 @@ int cc_tr_prove(cc_ctx* ctx, int la_init, int la_first, int la_count, int genera
           c.la = piece(s.la, a, nla);
           launches.push_back(c);
         }
-      rc = tr_launches(ctx, launches, false, next, err);
-      if (rc == CC_E_INTERRUPTED) break;  // keep the best complete generation
+      const size_t start = g == phase0 ? pos0 : 0;
+      last_done = start;
+      rc = tr_launches(ctx, launches, false, next, err, start, [&](size_t done) {
+        last_done = done;
+        return pr.after_batch(ctx, g, done, current, next);
+      });
+      if (rc == CC_E_INTERRUPTED && pr.stopped) return rc;  // to be resumed: no report yet
+      if (rc == CC_E_INTERRUPTED) {                        // keep the best complete generation
+        if ((rc = pr.save(ctx, g, last_done, current, next))) return rc;
+        completed = false;
+        break;
+      }
       if (rc) return rc;
       current.swap(next);
       next.clear();
@@ -329,12 +473,18 @@ int cc_tr_prove(cc_ctx* ctx, int la_init, int la_first, int la_count, int genera
   const std::string s = o.str();
   std::fwrite(s.data(), 1, s.size(), out ? out : stdout);
   std::fflush(out ? out : stdout);
+  if (checkpoint && completed) std::remove(checkpoint);  // the proof is done
   return CC_OK;
 }
 
 int cc_rsb_prove(cc_ctx* ctx, int la_init, int la_first, int la_count, int r1_init, int r2_init, int generations,
                  FILE* out) {
-  if (!ctx) return CC_E_ARG;
+  return cc_rsb_prove_ex(ctx, la_init, la_first, la_count, r1_init, r2_init, generations, nullptr, 0, out);
+}
+
+int cc_rsb_prove_ex(cc_ctx* ctx, int la_init, int la_first, int la_count, int r1_init, int r2_init, int generations,
+                    const char* checkpoint, int max_batches, FILE* out) {
+  if (!ctx || max_batches < 0) return CC_E_ARG;
   // rectangle_size_bound.cu:36-62 and values.hpp:28-35
   if (la_init <= 0) la_init = 256;
   if (la_first < 0 || la_first >= la_init) la_first = 0;
@@ -350,9 +500,11 @@ int cc_rsb_prove(cc_ctx* ctx, int la_init, int la_first, int la_count, int r1_in
   err << "Size bound: r <= " << 3.0 / 8.0 << ", critical ratio: " << 0.61 << std::endl;
   const cc_rsb_cell initial{{1.0, 2.5}, {{0.0, w}, {0.0, w}, {0.0, w}, {0.0, w}, {0.0, w}, {0.0, w}}};
   // find_critical_intervals_in (rectangle_size_bound.cu:276-310) over the
-  // lambda pieces [first, first + count) of `outer`
+  // lambda pieces [first, first + count) of `outer`, from cell `start` of
+  // that grid on; after(done) may stop the run after a batch of cells
   auto run = [&](const cc_rsb_cell& outer, int nla, int first, int count, int nr1, int nr2, const int* counts,
-                 std::vector<cc_rsb_cell>& append_to) -> int {
+                 std::vector<cc_rsb_cell>& append_to, size_t start,
+                 const std::function<int(size_t)>& after) -> int {
     std::vector<cc_rsb_cell> cells;
     for (int a = first; a < first + count; ++a) {
       cc_rsb_cell sub = outer;
@@ -365,7 +517,7 @@ int cc_rsb_prove(cc_ctx* ctx, int la_init, int la_first, int la_count, int r1_in
     }
     const size_t m = cells.size();
     const size_t chunk = 4096;
-    for (size_t b = 0; b < m; b += chunk) {
+    for (size_t b = start; b < m; b += chunk) {
       if (g_interrupted || ctx->interrupt) return CC_E_INTERRUPTED;
       const size_t k = std::min(chunk, m - b);
       std::vector<cc_rsb_cell> leaves(1 << 16);
@@ -379,36 +531,77 @@ int cc_rsb_prove(cc_ctx* ctx, int la_init, int la_first, int la_count, int r1_in
       append_to.insert(append_to.end(), leaves.begin(), leaves.begin() + n);
       err << "cells " << (b + k) << " of " << m << " done (" << append_to.size()
           << " critical intervals found)                \r" << std::flush;
+      if ((rc = after(b + k))) return rc;
     }
     return CC_OK;
   };
   std::vector<cc_rsb_cell> current, next;
-  err << "Initializing..." << std::endl;
-  int rc = run(initial, la_init, la_first, la_count, r1_init, r2_init, kInit, next);
-  if (rc) return rc;
-  current = merge_critical(next, 0);
-  next.clear();
+  Checkpoint<cc_rsb_cell> ck;
+  Progress<cc_rsb_cell> pr;
+  bool resumed = false;
+  int rc = CC_OK;
+  if (checkpoint) {
+    ck.path = checkpoint;
+    std::memcpy(ck.kind, "RSB", 3);
+    const int32_t p[6] = {la_init, la_first, la_count, r1_init, r2_init, generations};
+    std::memcpy(ck.params, p, sizeof p);
+    if ((rc = ck.load(ctx, resumed))) return rc;
+    pr.ck = &ck;
+  }
+  pr.max_batches = max_batches;
+  const int phase0 = resumed ? ck.phase : 0;
+  const size_t pos0 = resumed ? (size_t)ck.pos : 0;
+  size_t last_done = pos0;
+  if (resumed) {
+    current = ck.current;
+    next = ck.next;
+    err << "Resuming from phase " << phase0 << " at position " << pos0 << std::endl;
+  }
+  if (phase0 == 0) {
+    err << "Initializing..." << std::endl;
+    rc = run(initial, la_init, la_first, la_count, r1_init, r2_init, kInit, next, pos0, [&](size_t done) {
+      last_done = done;
+      return pr.after_batch(ctx, 0, done, current, next);
+    });
+    if (rc == CC_E_INTERRUPTED && !pr.stopped) {
+      const int r2 = pr.save(ctx, 0, last_done, current, next);
+      if (r2) return r2;
+    }
+    if (rc) return rc;
+    current = merge_critical(next, 0);
+    next.clear();
+  }
   if (current.empty()) {
     err << "No critical regions after initialization - done!" << std::endl;
     end_case(o);
     const std::string s = o.str();
     std::fwrite(s.data(), 1, s.size(), out ? out : stdout);
+    if (checkpoint) std::remove(checkpoint);  // the proof is done
     return CC_OK;
   }
+  bool completed = true;
   {
     Interruptible guard;
-    for (int g = 1; g <= generations; ++g) {
+    const auto none = [](size_t) { return CC_OK; };
+    for (int g = phase0 > 1 ? phase0 : 1; g <= generations; ++g) {
       err << "Running generation " << g << " on " << current.size() << " intervals..." << std::endl;
       bool stop = false;
-      for (const cc_rsb_cell& s : current) {
-        rc = run(s, 64, 0, 64, 32, 32, kSub, next);  // rectangle_size_bound.cu:50-52
+      // position in a generation: the cells of `current` done
+      for (size_t j = g == phase0 ? pos0 : 0; j < current.size(); ++j) {
+        rc = run(current[j], 64, 0, 64, 32, 32, kSub, next, 0, none);  // rectangle_size_bound.cu:50-52
         if (rc == CC_E_INTERRUPTED) {
+          if ((rc = pr.save(ctx, g, j, current, next))) return rc;
           stop = true;
           break;
         }
         if (rc) return rc;
+        if ((rc = pr.after_batch(ctx, g, j + 1, current, next)) == CC_E_INTERRUPTED) return rc;  // to be resumed
+        if (rc) return rc;
       }
-      if (stop) break;
+      if (stop) {
+        completed = false;
+        break;
+      }
       current = merge_critical(next, g % 3 == 0 ? 3 : g % 3 == 1 ? 2 : 1);
       next.clear();
       if (current.empty()) break;
@@ -423,6 +616,7 @@ int cc_rsb_prove(cc_ctx* ctx, int la_init, int la_first, int la_count, int r1_in
   const std::string s = o.str();
   std::fwrite(s.data(), 1, s.size(), out ? out : stdout);
   std::fflush(out ? out : stdout);
+  if (checkpoint && completed) std::remove(checkpoint);  // the proof is done
   return CC_OK;
 }
 

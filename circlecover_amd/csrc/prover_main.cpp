@@ -9,18 +9,23 @@
 //   --tr-generations G   refinement generations of tight_rectangle (11)
 //   --rsb-grid A,B,C     initial lambda x r1 x r2 grid (256,128,128)
 //   --rsb-generations G  refinement generations of rectangle_size_bound (1)
+//   --checkpoint P       save progress to P.tr / P.rsb and resume from them
+//   --max-batches N      stop each proof after N batches, exit status 3
+//                        (resume later with the same --checkpoint)
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <iostream>
 #include <stdexcept>
+#include <string>
 
 #include "circlecover_amd.h"
 
 int main(int argc, const char* argv[]) {
   bool tr = true, rsb = true, custom = false;
   int tr_la = 0, tr_gen = -1, rsb_la = 0, rsb_r1 = 0, rsb_r2 = 0, rsb_gen = -1;
-  int tr_first = 0, tr_count = 0, rsb_first = 0, rsb_count = 0;
+  int tr_first = 0, tr_count = 0, rsb_first = 0, rsb_count = 0, max_batches = 0;
+  const char* ckpt = nullptr;
   for (int i = 1; i < argc; ++i) {
     const char* a = argv[i];
     const char* v = i + 1 < argc ? argv[i + 1] : nullptr;
@@ -46,6 +51,10 @@ int main(int argc, const char* argv[]) {
       custom = true, ++i;
     } else if (!std::strcmp(a, "--rsb-generations") && v) {
       rsb_gen = std::atoi(v), custom = true, ++i;
+    } else if (!std::strcmp(a, "--checkpoint") && v) {
+      ckpt = v, custom = true, ++i;
+    } else if (!std::strcmp(a, "--max-batches") && v) {
+      max_batches = std::atoi(v), custom = true, ++i;
     } else {
       std::cerr << "unknown argument " << a << std::endl;
       return 2;
@@ -62,9 +71,13 @@ int main(int argc, const char* argv[]) {
     return 1;
   }
   int rc = CC_OK;
-  if (tr) rc = cc_tr_prove(ctx, tr_la, tr_first, tr_count, tr_gen, stdout);
-  if (rc == CC_OK && rsb) rc = cc_rsb_prove(ctx, rsb_la, rsb_first, rsb_count, rsb_r1, rsb_r2, rsb_gen, stdout);
+  const std::string ck_tr = ckpt ? std::string(ckpt) + ".tr" : "", ck_rsb = ckpt ? std::string(ckpt) + ".rsb" : "";
+  if (tr) rc = cc_tr_prove_ex(ctx, tr_la, tr_first, tr_count, tr_gen, ckpt ? ck_tr.c_str() : nullptr, max_batches, stdout);
+  if (rc == CC_OK && rsb)
+    rc = cc_rsb_prove_ex(ctx, rsb_la, rsb_first, rsb_count, rsb_r1, rsb_r2, rsb_gen, ckpt ? ck_rsb.c_str() : nullptr,
+                         max_batches, stdout);
   if (rc != CC_OK && rc != CC_E_INTERRUPTED) std::cerr << "error " << rc << ": " << cc_last_error(ctx) << std::endl;
   cc_destroy(ctx);
+  if (rc == CC_E_INTERRUPTED && max_batches > 0) return 3;  // stopped after --max-batches: resume later
   return rc == CC_OK || rc == CC_E_INTERRUPTED ? 0 : 1;
 }

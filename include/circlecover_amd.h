@@ -303,6 +303,21 @@ int cc_tr_prove(cc_ctx* ctx, int la_init, int la_first, int la_count, int genera
  * "Critical: " line per remaining cell. */
 int cc_rsb_prove(cc_ctx* ctx, int la_init, int la_first, int la_count, int r1_init, int r2_init, int generations,
                  FILE* out);
+/* Checkpointed forms of the two proofs (no counterpart in the reference,
+ * whose only resume point is a lambda offset; for runs longer than a job's
+ * time limit).  checkpoint (may be NULL) names a file: a run starts from it
+ * if it exists (CC_E_ARG if it belongs to another proof or other
+ * parameters), saves its progress there after its batches (every 2 s at
+ * most: 16 launches of tight_rectangle, 4096 cells of rectangle_size_bound,
+ * or one critical cell of a generation) and when it is interrupted, and
+ * removes it when the proof completes.  max_batches > 0 stops the run after
+ * that many batches with CC_E_INTERRUPTED and no report: a later call with
+ * the same checkpoint continues it, and the final report is the one an
+ * uninterrupted run writes. */
+int cc_tr_prove_ex(cc_ctx* ctx, int la_init, int la_first, int la_count, int generations, const char* checkpoint,
+                   int max_batches, FILE* out);
+int cc_rsb_prove_ex(cc_ctx* ctx, int la_init, int la_first, int la_count, int r1_init, int r2_init, int generations,
+                    const char* checkpoint, int max_batches, FILE* out);
 /* Asks a running proof or branch-and-bound to stop at its next level
  * boundary (returns CC_E_INTERRUPTED there; reference: was_interrupted,
  * Prover/interrupt.cpp:30-46). */

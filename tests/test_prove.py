@@ -82,3 +82,40 @@ def test_gpu_rsb_prove_trap_fails_the_run():
     r = subprocess.run([EXE, "--only", "rsb", "--rsb-grid", "256,8,8", "--rsb-la-range", "0,1",
                         "--rsb-generations", "0"], capture_output=True, text=True, timeout=600)
     assert r.returncode == 1 and "error 3" in r.stderr
+
+
+def _run_stop(args):
+    r = subprocess.run([EXE, *args], capture_output=True, text=True, timeout=600)
+    return r.returncode, r.stdout
+
+
+@pytest.mark.gpu
+def test_gpu_tr_prove_resumes_from_checkpoint(tmp_path):
+    """A proof stopped after a batch, resumed, stopped again inside a
+    generation and resumed to the end writes the uninterrupted run's report;
+    the checkpoint file is gone once the proof completes."""
+    args = ["--only", "tr", "--tr-la-range", "0,40", "--tr-generations", "2"]
+    full = run_prover(args)
+    ck = str(tmp_path / "ck")
+    code, out = _run_stop([*args, "--checkpoint", ck, "--max-batches", "1"])
+    assert code == 3 and out == "" and os.path.exists(ck + ".tr")
+    code, out = _run_stop([*args, "--checkpoint", ck, "--max-batches", "3"])
+    assert code == 3 and out == "" and os.path.exists(ck + ".tr")
+    assert run_prover([*args, "--checkpoint", ck]) == full
+    assert not os.path.exists(ck + ".tr")
+
+
+@pytest.mark.gpu
+def test_gpu_rsb_prove_resumes_from_checkpoint(tmp_path):
+    args = ["--only", "rsb", "--rsb-grid", "256,8,8", "--rsb-la-range", "40,70", "--rsb-generations", "0"]
+    full = run_prover(args)
+    ck = str(tmp_path / "ck")
+    code, out = _run_stop([*args, "--checkpoint", ck, "--max-batches", "1"])
+    assert code == 3 and out == "" and os.path.exists(ck + ".rsb")
+    assert run_prover([*args, "--checkpoint", ck]) == full
+    assert not os.path.exists(ck + ".rsb")
+    # a checkpoint of other parameters is refused
+    _run_stop([*args, "--checkpoint", ck, "--max-batches", "1"])
+    other = ["--only", "rsb", "--rsb-grid", "256,8,8", "--rsb-la-range", "40,71", "--rsb-generations", "0"]
+    r = subprocess.run([EXE, *other, "--checkpoint", ck], capture_output=True, text=True, timeout=600)
+    assert r.returncode == 1 and "checkpoint" in r.stderr
