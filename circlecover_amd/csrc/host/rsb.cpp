@@ -118,7 +118,7 @@ int enumerate_dev(cc_ctx* ctx, const void* d_cells, size_t ncells, const int n[4
         const uint64_t nr = ca - r0 < chunk ? ca - r0 : chunk;
         const uint64_t rows = nr * (uint64_t)n[1] * (uint64_t)n[2];
         CC_HIP(ctx, hipMemsetAsync(cnt + 3, 0, 2 * sizeof(unsigned long long), ctx->stream));
-        CC_HIP(ctx, launch(frows, dim3(blocks_for(rows, 128)), dim3(128), 0, ctx->stream, d_cells, S.s1.p, r0, nr,
+        CC_HIP(ctx, launch(frows, dim3(blocks_for(rows, 1024)), dim3(1024), 0, ctx->stream, d_cells, S.s1.p, r0, nr,
                            n[1], n[2], n[3], S.enum_list.p, cnt + 3));
         CC_HIP(ctx, launch(fla, dim3(grid), dim3(128), 0, ctx->stream, d_cells, S.s1.p, S.enum_list.p,
                            (const unsigned long long*)(cnt + 3), n[1], n[2], n[3], ctx->d_konst, S.enum_b.p, cnt + 4,

@@ -74,9 +74,14 @@ CCD cell7 leaf_of(const rsb_rec* __restrict__ cells, const rsb_s3& s, int i4, in
   return rsbc::tighten(v);
 }
 
-// Rows of the level-A records [rec0, rec0 + nrec) (n4, n5, n6 <= 256).  The
-// list has room for every leaf of those records.
-extern "C" __global__ void __launch_bounds__(128) ccrd_rsb_enum_rows(const rsb_rec* __restrict__ cells,
+// Rows of the level-A records [rec0, rec0 + nrec) (n4, n5, n6 <= 256), in
+// workgroups of 1024 lanes.  The list has room for every leaf of those
+// records.  One atomic per workgroup claims its part of the list: nearly
+// every wave lists some leaves, and one atomic per wave on the one counter
+// cost more than the emptiness tests themselves.
+static constexpr int kRowsBlock = 1024;
+
+extern "C" __global__ void __launch_bounds__(kRowsBlock) ccrd_rsb_enum_rows(const rsb_rec* __restrict__ cells,
                                                                       const rsb_s3* __restrict__ in, uint64_t rec0,
                                                                       uint64_t nrec, int n4, int n5, int n6,
                                                                       leaf_ref* __restrict__ list,
@@ -122,7 +127,11 @@ extern "C" __global__ void __launch_bounds__(128) ccrd_rsb_enum_rows(const rsb_r
         if (!(rsbc::is_empty(c) || empty(rsbc::remaining_weight(c)))) mask |= 1ull << k;
       }
     }
-    // one atomic per wave: an exclusive scan of the lanes' counts
+    // an exclusive scan of the lanes' counts in the wave, then of the
+    // waves' totals in the workgroup, and one atomic for the workgroup
+    __shared__ uint32_t wave_total[kRowsBlock / 64];
+    __shared__ unsigned long long wave_base[kRowsBlock / 64];
+    const int wv = threadIdx.x >> 6;
     const uint32_t mine = (uint32_t)__popcll(mask);
     uint32_t x = mine;
 #pragma unroll
@@ -130,10 +139,19 @@ extern "C" __global__ void __launch_bounds__(128) ccrd_rsb_enum_rows(const rsb_r
       const uint32_t y = __shfl_up(x, d);
       if (lane >= d) x += y;
     }
-    const uint32_t total = __shfl(x, 63);
-    unsigned long long at = 0;
-    if (lane == 63 && total) at = atomicAdd(count_list, (unsigned long long)total);
-    at = __shfl(at, 63) + (x - mine);
+    if (lane == 63) wave_total[wv] = x;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      unsigned long long sum = 0;
+      for (int w = 0; w < kRowsBlock / 64; ++w) sum += wave_total[w];
+      unsigned long long base = sum ? atomicAdd(count_list, sum) : 0ull;
+      for (int w = 0; w < kRowsBlock / 64; ++w) {
+        wave_base[w] = base;
+        base += wave_total[w];
+      }
+    }
+    __syncthreads();
+    unsigned long long at = wave_base[wv] + (x - mine);
     while (mask) {
       const int k = __builtin_ctzll(mask);
       mask &= mask - 1;
