@@ -113,9 +113,39 @@ extern "C" __global__ void __launch_bounds__(kRowsBlock) ccrd_rsb_enum_rows(cons
   // the r6 pieces as subinterval(r6, i, n6) computes them: one quotient for
   // the row, and each piece's upper bound is the next piece's lower bound
   const double d6 = live ? rn_span_div(r6.hi, r6.lo, (double)n6) : 0.0;
+  // The open (not empty) leaves of a row are a prefix of its r6 pieces
+  // when the pieces' bounds strictly increase, which a quotient above two
+  // ulps of r6.hi ensures: with l6, u6 the piece's bounds, the tightened
+  // leaf is empty exactly when the row's part is, or l6 >= u6, or l6 >= the
+  // least upper bound of r2..r5 (tightening raises their lower bounds to
+  // l6); and its remaining weight R (lower bound <= upper bound for a
+  // non-empty leaf) is empty exactly when R.hi < 0, where R.hi is the
+  // upward-rounded la * 0.61 minus the tightened lower bounds, none of which
+  // falls as l6 grows.  So a binary search with the full test finds the
+  // prefix; rows with narrower pieces test every piece.
+  int open_prefix = -1;
+  if (live && d6 > 2.0 * (next_up(r6.hi) - r6.hi)) {
+    auto open_at = [&](int i) {
+      const double l6 = rn_fma(d6, (double)i, r6.lo);
+      const double u6 = i + 1 == n6 ? r6.hi : rn_fma(d6, (double)(i + 1), r6.lo);
+      v.r[5] = iv{l6, u6};
+      const cell7 c = rsbc::tighten(v);
+      return !(rsbc::is_empty(c) || empty(rsbc::remaining_weight(c)));
+    };
+    int a = 0, b = n6;  // open on [0, a), closed on [b, n6)
+    while (a < b) {
+      const int mid = (a + b) >> 1;
+      if (open_at(mid)) a = mid + 1;
+      else b = mid;
+    }
+    open_prefix = a;
+  }
   for (int b6 = 0; b6 < n6; b6 += 64) {  // the same trip count on every lane
     uint64_t mask = 0;
-    if (live) {
+    if (live && open_prefix >= 0) {
+      const int m = open_prefix - b6 < 64 ? open_prefix - b6 : 64;
+      if (m > 0) mask = m == 64 ? ~0ull : (1ull << m) - 1ull;
+    } else if (live) {
       const int m = n6 - b6 < 64 ? n6 - b6 : 64;
       double l = rn_fma(d6, (double)b6, r6.lo);
       for (int k = 0; k < m; ++k) {
