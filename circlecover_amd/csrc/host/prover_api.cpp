@@ -11,8 +11,6 @@
 #include <algorithm>
 #include <cfloat>
 #include <chrono>
-#include <functional>
-#include <string>
 #include <cmath>
 #include <csignal>
 #include <cstdio>
@@ -21,7 +19,10 @@
 #include <iostream>
 #include <sstream>
 #include <stdexcept>
+#include <functional>
+#include <string>
 #include <system_error>
+#include <thread>
 #include <vector>
 
 #include "runtime.hpp"
@@ -87,24 +88,59 @@ void do_join(cc_iv& a, const cc_iv& b) {
 bool iv_less(const cc_iv& a, const cc_iv& b) { return a.lb < b.lb || (a.lb == b.lb && a.ub < b.ub); }
 bool iv_eq(const cc_iv& a, const cc_iv& b) { return a.lb == b.lb && a.ub == b.ub; }
 
-// merge_adjacent_r1 (tight_rectangle.cu:121-159).  Records with equal r1
+// merge_adjacent_r1 (tight_rectangle.cu:121-159) applied to each launch's
+// leaves: sort by r1, then join (hull of every radius) each record whose r1
+// touches the running record's.  leaves[i] belongs to launch from[i] < m;
+// the merged records are appended launch by launch.  The leaves are not
+// moved: each launch's (r1, index) keys are sorted, on up to 16 threads,
+// and the sweep reads the records through them.  Records with equal r1
 // keys are all joined into one running record whatever their order, so the
 // result does not depend on the sort's treatment of ties.
-std::vector<cc_tr_cell> merge_adjacent_r1(std::vector<cc_tr_cell> v) {
-  std::vector<cc_tr_cell> res;
-  if (v.empty()) return res;
-  std::sort(v.begin(), v.end(), [](const cc_tr_cell& a, const cc_tr_cell& b) { return iv_less(a.r[0], b.r[0]); });
-  res.push_back(v.front());
-  for (const cc_tr_cell& c : v) {
-    cc_tr_cell& p = res.back();
-    const bool touch = p.r[0].lb <= c.r[0].lb ? p.r[0].ub >= c.r[0].lb : c.r[0].ub >= p.r[0].lb;
-    if (touch) {
-      for (int k = 0; k < 4; ++k) do_join(p.r[k], c.r[k]);
-      continue;
-    }
-    res.push_back(c);
+void merge_per_launch(const std::vector<cc_tr_cell>& leaves, const std::vector<uint32_t>& from, size_t m,
+                      std::vector<cc_tr_cell>& append_to) {
+  struct Key {
+    double lb, ub;
+    uint32_t i;
+  };
+  std::vector<size_t> start(m + 1, 0);
+  for (uint32_t f : from) ++start[f + 1];
+  for (size_t c = 0; c < m; ++c) start[c + 1] += start[c];
+  std::vector<Key> keys(leaves.size());
+  {
+    std::vector<size_t> at(start.begin(), start.end() - 1);
+    for (size_t i = 0; i < leaves.size(); ++i) keys[at[from[i]]++] = Key{leaves[i].r[0].lb, leaves[i].r[0].ub, (uint32_t)i};
   }
-  return res;
+  std::vector<std::vector<cc_tr_cell>> merged(m);
+  auto one = [&](size_t c) {
+    Key* b = keys.data() + start[c];
+    Key* e = keys.data() + start[c + 1];
+    if (b == e) return;
+    std::sort(b, e, [](const Key& x, const Key& y) { return x.lb < y.lb || (x.lb == y.lb && x.ub < y.ub); });
+    std::vector<cc_tr_cell>& res = merged[c];
+    res.push_back(leaves[b->i]);
+    for (const Key* k = b; k != e; ++k) {
+      const cc_tr_cell& r = leaves[k->i];
+      cc_tr_cell& p = res.back();
+      const bool touch = p.r[0].lb <= r.r[0].lb ? p.r[0].ub >= r.r[0].lb : r.r[0].ub >= p.r[0].lb;
+      if (touch) {
+        for (int q = 0; q < 4; ++q) do_join(p.r[q], r.r[q]);
+        continue;
+      }
+      res.push_back(r);
+    }
+  };
+  const size_t nt = std::min<size_t>(m, 16);
+  if (nt <= 1 || leaves.size() < (1u << 16)) {
+    for (size_t c = 0; c < m; ++c) one(c);
+  } else {
+    std::vector<std::thread> pool;
+    for (size_t t = 0; t < nt; ++t)
+      pool.emplace_back([&, t] {
+        for (size_t c = t; c < m; c += nt) one(c);
+      });
+    for (auto& th : pool) th.join();
+  }
+  for (size_t c = 0; c < m; ++c) append_to.insert(append_to.end(), merged[c].begin(), merged[c].end());
 }
 
 // merge_critical_intervals<N> (rectangle_size_bound.cu:103-130): sort by
@@ -304,12 +340,7 @@ int tr_launches(cc_ctx* ctx, const std::vector<cc_tr_cell>& launches, bool init,
     int rc = tr_enumerate_host(ctx, launches.data() + b, m, init ? kInit : kSub, leaves, from, &trap);
     if (rc) return rc;
     if (trap) return fail(ctx, CC_E_TRAP, "a cell reached a sqrt of a possibly negative interval");
-    std::vector<std::vector<cc_tr_cell>> per(m);
-    for (size_t i = 0; i < leaves.size(); ++i) per[from[i]].push_back(leaves[i]);
-    for (size_t c = 0; c < m; ++c) {
-      std::vector<cc_tr_cell> merged = merge_adjacent_r1(per[c]);
-      append_to.insert(append_to.end(), merged.begin(), merged.end());
-    }
+    merge_per_launch(leaves, from, m, append_to);
     err << "launch " << (b + m) << " of " << launches.size() << " done (" << append_to.size()
         << " critical intervals found)                \r" << std::flush;
     if (int rc = after(b + m)) return rc;
