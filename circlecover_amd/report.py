@@ -2,7 +2,7 @@
 writer and reader (Prover/algcuda/cuda-utils/cpp/include/algcuda/interval.hpp:
 294-351, through the C ABI's cc_format_interval / cc_parse_interval) and the
 output_case layout (Prover/output_case.hpp:35-54), so that two reports, for
-instance this build's and an archived run of the reference, can be compared
+instance this library's and an archived run of the reference, can be compared
 interval by interval, bit for bit."""
 from __future__ import annotations
 
