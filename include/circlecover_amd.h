@@ -160,7 +160,8 @@ size_t cc_rsb_grid(const cc_rsb_cell* outer, int nla, int nr1, int nr2, cc_rsb_c
  * relaunch protocol (:256-269) expects. */
 int cc_rsb_find_critical_in(cc_ctx* ctx, const cc_rsb_cell* cell, int is_init, cc_rsb_cell* out, size_t cap,
                             size_t* n_out);
-/* The same enumeration over a batch of cells with counts[4] pieces for r3..r6. */
+/* The same enumeration over a batch of cells with counts[4] pieces for r3..r6
+ * (r4..r6: 1..256 each; CC_E_ARG otherwise). */
 int cc_rsb_enumerate(cc_ctx* ctx, const cc_rsb_cell* cells, size_t ncells, const int counts[4], cc_rsb_cell* out,
                      size_t cap, size_t* n_out);
 
