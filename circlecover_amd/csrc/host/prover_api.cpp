@@ -22,7 +22,6 @@
 #include <functional>
 #include <string>
 #include <system_error>
-#include <thread>
 #include <vector>
 
 #include "runtime.hpp"
@@ -30,8 +29,8 @@
 using namespace cchost;
 
 namespace cchost {
-int tr_enumerate_host(cc_ctx* ctx, const cc_tr_cell* cells, size_t ncells, const int counts[4],
-                      std::vector<cc_tr_cell>& leaves, std::vector<uint32_t>& leaf_cell, int* trap);
+int tr_enumerate_merged(cc_ctx* ctx, const cc_tr_cell* cells, size_t ncells, const int counts[4],
+                        std::vector<cc_tr_cell>& append_to, int* trap);
 }
 
 namespace {
@@ -87,61 +86,6 @@ void do_join(cc_iv& a, const cc_iv& b) {
 }
 bool iv_less(const cc_iv& a, const cc_iv& b) { return a.lb < b.lb || (a.lb == b.lb && a.ub < b.ub); }
 bool iv_eq(const cc_iv& a, const cc_iv& b) { return a.lb == b.lb && a.ub == b.ub; }
-
-// merge_adjacent_r1 (tight_rectangle.cu:121-159) applied to each launch's
-// leaves: sort by r1, then join (hull of every radius) each record whose r1
-// touches the running record's.  leaves[i] belongs to launch from[i] < m;
-// the merged records are appended launch by launch.  The leaves are not
-// moved: each launch's (r1, index) keys are sorted, on up to 16 threads,
-// and the sweep reads the records through them.  Records with equal r1
-// keys are all joined into one running record whatever their order, so the
-// result does not depend on the sort's treatment of ties.
-void merge_per_launch(const std::vector<cc_tr_cell>& leaves, const std::vector<uint32_t>& from, size_t m,
-                      std::vector<cc_tr_cell>& append_to) {
-  struct Key {
-    double lb, ub;
-    uint32_t i;
-  };
-  std::vector<size_t> start(m + 1, 0);
-  for (uint32_t f : from) ++start[f + 1];
-  for (size_t c = 0; c < m; ++c) start[c + 1] += start[c];
-  std::vector<Key> keys(leaves.size());
-  {
-    std::vector<size_t> at(start.begin(), start.end() - 1);
-    for (size_t i = 0; i < leaves.size(); ++i) keys[at[from[i]]++] = Key{leaves[i].r[0].lb, leaves[i].r[0].ub, (uint32_t)i};
-  }
-  std::vector<std::vector<cc_tr_cell>> merged(m);
-  auto one = [&](size_t c) {
-    Key* b = keys.data() + start[c];
-    Key* e = keys.data() + start[c + 1];
-    if (b == e) return;
-    std::sort(b, e, [](const Key& x, const Key& y) { return x.lb < y.lb || (x.lb == y.lb && x.ub < y.ub); });
-    std::vector<cc_tr_cell>& res = merged[c];
-    res.push_back(leaves[b->i]);
-    for (const Key* k = b; k != e; ++k) {
-      const cc_tr_cell& r = leaves[k->i];
-      cc_tr_cell& p = res.back();
-      const bool touch = p.r[0].lb <= r.r[0].lb ? p.r[0].ub >= r.r[0].lb : r.r[0].ub >= p.r[0].lb;
-      if (touch) {
-        for (int q = 0; q < 4; ++q) do_join(p.r[q], r.r[q]);
-        continue;
-      }
-      res.push_back(r);
-    }
-  };
-  const size_t nt = std::min<size_t>(m, 16);
-  if (nt <= 1 || leaves.size() < (1u << 16)) {
-    for (size_t c = 0; c < m; ++c) one(c);
-  } else {
-    std::vector<std::thread> pool;
-    for (size_t t = 0; t < nt; ++t)
-      pool.emplace_back([&, t] {
-        for (size_t c = t; c < m; c += nt) one(c);
-      });
-    for (auto& th : pool) th.join();
-  }
-  for (size_t c = 0; c < m; ++c) append_to.insert(append_to.end(), merged[c].begin(), merged[c].end());
-}
 
 // merge_critical_intervals<N> (rectangle_size_bound.cu:103-130): sort by
 // (lambda, radii) lexicographically, then join radii[N..6) of consecutive
@@ -334,13 +278,12 @@ int tr_launches(cc_ctx* ctx, const std::vector<cc_tr_cell>& launches, bool init,
   for (size_t b = start; b < launches.size(); b += kTrBatch) {
     if (g_interrupted || ctx->interrupt) return CC_E_INTERRUPTED;
     const size_t m = std::min(kTrBatch, launches.size() - b);
-    std::vector<cc_tr_cell> leaves;
-    std::vector<uint32_t> from;
     int trap = 0;
-    int rc = tr_enumerate_host(ctx, launches.data() + b, m, init ? kInit : kSub, leaves, from, &trap);
+    const size_t before = append_to.size();
+    int rc = tr_enumerate_merged(ctx, launches.data() + b, m, init ? kInit : kSub, append_to, &trap);
+    if (rc || trap) append_to.resize(before);
     if (rc) return rc;
     if (trap) return fail(ctx, CC_E_TRAP, "a cell reached a sqrt of a possibly negative interval");
-    merge_per_launch(leaves, from, m, append_to);
     err << "launch " << (b + m) << " of " << launches.size() << " done (" << append_to.size()
         << " critical intervals found)                \r" << std::flush;
     if (int rc = after(b + m)) return rc;

@@ -107,6 +107,7 @@ hipError_t launch(hipFunction_t f, dim3 grid, dim3 block, size_t shmem, hipStrea
 // Level buffers shared by the enumeration and branch-and-bound drivers.
 struct Scratch {
   DevBuf cells, aux, s1, s2, s3, leaves, next, verdict;
+  DevBuf tr_piece, tr_buckets;          // TR leaves' r1 piece (4 B each); per-(cell, piece) hulls
   DevBuf c2_defer;                      // batch kernel: deferred-box counter (first 256 B), data, list
   DevBuf enum_list, enum_b;             // RSB enumeration: non-empty leaves, leaves for C14 (8 B each)
   DevBuf c2_r1;                         // batch kernel: per-wave staging of the boxes criterion 0's

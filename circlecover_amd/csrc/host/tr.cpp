@@ -2,6 +2,7 @@
 // (find_critical_intervals_in_call_kernel, Prover/tight_rectangle/
 // tight_rectangle.cu:264-298) generalised to a batch of cells, and the small
 // level-synchronous branch-and-bound on top of it.
+#include <algorithm>
 #include <cstring>
 #include <vector>
 
@@ -82,8 +83,9 @@ int enumerate_dev(cc_ctx* ctx, const void* d_cells, size_t ncells, const int n[4
       const uint64_t cap = S.leaves.cap / kCell;
       CC_HIP(ctx, hipMemsetAsync(cnt + 3, 0, sizeof(unsigned long long), ctx->stream));
       CC_HIP(ctx, S.verdict.ensure((S.leaves.cap / kCell) * sizeof(uint32_t)));
+      CC_HIP(ctx, S.tr_piece.ensure((S.leaves.cap / kCell) * sizeof(uint32_t)));
       CC_HIP(ctx, launch(f4, dim3(blocks_for(t4, 128)), dim3(128), 0, ctx->stream, d_cells, S.s3.p, c3, n[3],
-                         ctx->d_konst, S.aux.p, S.leaves.p, cap, S.verdict.p, cnt + 3, d_trap));
+                         ctx->d_konst, S.aux.p, S.leaves.p, cap, S.verdict.p, S.tr_piece.p, cnt + 3, d_trap));
       if ((rc = read_counts(ctx, 5))) return rc;
       c4 = ctx->h_counts[3];
       if (c4 <= cap) break;
@@ -120,22 +122,81 @@ int upload_cells(cc_ctx* ctx, const cc_tr_cell* cells, size_t ncells) {
 
 namespace cchost {
 
-// Leaves of a batch of cells together with the index of the cell each came
-// from (for per-launch merging of batched launches).
-int tr_enumerate_host(cc_ctx* ctx, const cc_tr_cell* cells, size_t ncells, const int counts[4],
-                      std::vector<cc_tr_cell>& leaves, std::vector<uint32_t>& leaf_cell, int* trap) {
+// The leaves of a batch of cells, merged per cell as merge_adjacent_r1
+// (tight_rectangle.cu:129-159) merges one launch's leaves, appended in cell
+// order.  The merge result is the connected components of the union of the
+// leaves' r1 intervals, each with the hull of all four radii and the cell's
+// lambda, in order of their least r1 lower bound: whatever the order of
+// equal keys in the reference's sort, since the radii are never negative
+// zero or NaN and the hull is then the same bits in any order.  Every leaf
+// of a (cell, r1 piece) bucket holds the piece's r1 upper bound, so the
+// device forms each bucket's hull (cc_tr_bucket_hull) and only the buckets
+// come back: per cell, they are sorted by their least r1 lower bound and
+// joined where that bound is at most the running r1 upper bound, as the
+// reference's sweep joins leaves.
+int tr_enumerate_merged(cc_ctx* ctx, const cc_tr_cell* cells, size_t ncells, const int counts[4],
+                        std::vector<cc_tr_cell>& append_to, int* trap) {
   CC_HIP(ctx, hipSetDevice(ctx->device));
   int rc = upload_cells(ctx, cells, ncells);
   if (rc) return rc;
   size_t n = 0;
   if ((rc = enumerate_dev(ctx, scratch(ctx).cells.p, ncells, counts, &n, trap))) return rc;
-  leaves.resize(n);
-  leaf_cell.resize(n);
-  if (n) {
-    CC_HIP(ctx, hipMemcpyAsync(leaves.data(), scratch(ctx).leaves.p, n * kCell, hipMemcpyDeviceToHost, ctx->stream));
-    CC_HIP(ctx, hipMemcpyAsync(leaf_cell.data(), scratch(ctx).verdict.p, n * sizeof(uint32_t), hipMemcpyDeviceToHost,
-                               ctx->stream));
-    CC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  if (n == 0) return CC_OK;
+  Scratch& S = scratch(ctx);
+  const size_t n1 = (size_t)counts[0], nb = ncells * n1;
+  constexpr size_t kWords = 10;
+  std::vector<unsigned long long> h(nb * kWords + 1);
+  for (size_t k = 0; k < nb; ++k) {
+    unsigned long long* b = h.data() + kWords * k;
+    for (int q = 0; q < 4; ++q) b[q] = ~0ull, b[4 + q] = 0ull;
+    b[8] = ~0ull;
+    b[9] = 0ull;
+  }
+  h[nb * kWords] = 0;  // the flag of a leaf with r1.lb > r1.ub
+  CC_HIP(ctx, S.tr_buckets.ensure(h.size() * sizeof(unsigned long long)));
+  auto* d_b = static_cast<unsigned long long*>(S.tr_buckets.p);
+  CC_HIP(ctx, hipMemcpyAsync(d_b, h.data(), h.size() * sizeof(unsigned long long), hipMemcpyHostToDevice,
+                             ctx->stream));
+  hipFunction_t fh;
+  CC_HIP(ctx, kernel(ctx, "cc_tr_bucket_hull", &fh));
+  CC_HIP(ctx, launch(fh, dim3(blocks_for(n, 256)), dim3(256), 0, ctx->stream, S.leaves.p, S.verdict.p, S.tr_piece.p,
+                     (uint64_t)n, (int)n1, d_b, reinterpret_cast<int*>(d_b + nb * kWords)));
+  CC_HIP(ctx, hipMemcpyAsync(h.data(), d_b, h.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                             ctx->stream));
+  CC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  if (h[nb * kWords] & 0xffffffffull) return fail(ctx, CC_E_TRAP, "a leaf's r1 lower bound exceeds its upper bound");
+  auto value = [](unsigned long long k) {
+    const unsigned long long b = (k >> 63) ? k ^ (1ull << 63) : ~k;
+    double x;
+    std::memcpy(&x, &b, sizeof x);
+    return x;
+  };
+  std::vector<cc_tr_cell> hulls;
+  for (size_t c = 0; c < ncells; ++c) {
+    hulls.clear();
+    for (size_t p = 0; p < n1; ++p) {
+      const unsigned long long* b = h.data() + kWords * (c * n1 + p);
+      if (b[9] == 0) continue;
+      if (b[8] != b[4]) return fail(ctx, CC_E_TRAP, "leaves of one r1 piece differ in r1's upper bound");
+      cc_tr_cell v;
+      v.la = cells[c].la;
+      for (int q = 0; q < 4; ++q) v.r[q] = cc_iv{value(b[q]), value(b[4 + q])};
+      hulls.push_back(v);
+    }
+    std::stable_sort(hulls.begin(), hulls.end(),
+                     [](const cc_tr_cell& x, const cc_tr_cell& y) { return x.r[0].lb < y.r[0].lb; });
+    for (size_t k = 0; k < hulls.size(); ++k) {
+      const cc_tr_cell& v = hulls[k];
+      if (k > 0 && v.r[0].lb <= append_to.back().r[0].ub) {
+        cc_tr_cell& p = append_to.back();
+        for (int q = 0; q < 4; ++q) {
+          if (v.r[q].lb < p.r[q].lb) p.r[q].lb = v.r[q].lb;
+          if (v.r[q].ub > p.r[q].ub) p.r[q].ub = v.r[q].ub;
+        }
+      } else {
+        append_to.push_back(v);
+      }
+    }
   }
   return CC_OK;
 }

@@ -72,7 +72,7 @@ extern "C" __global__ void __launch_bounds__(256) ccrd_tr_level1(const double* _
   if (!empty(a.r1r)) {
     const cell5 c = load_cell(cells + 10 * (uint64_t)ci);
     rec.cell = ci;
-    rec.pad = 0;
+    rec.pad = (uint32_t)i1;  // the r1 piece, carried down to the leaves
     rec.r1 = subinterval(a.r1r, i1, n1);
     rec.r2r = intersect(r2_range(a.W, rec.r1), c.r[1]);
     keep = !empty(rec.r2r) && !r1_strategies(K, c.la, a.W, rec.r1, rec.r2r.hi);
@@ -99,7 +99,7 @@ extern "C" __global__ void __launch_bounds__(256) ccrd_tr_level2(const double* _
   Trap tp{0};
   tr_s2 rec;
   rec.cell = s.cell;
-  rec.pad = 0;
+  rec.pad = s.pad;
   rec.r1 = s.r1;
   rec.r2 = subinterval(s.r2r, i2, n2);
   bool keep = !r1_strategies(K, c.la, W, s.r1, rec.r2.hi);
@@ -130,7 +130,7 @@ extern "C" __global__ void __launch_bounds__(256) ccrd_tr_level3(const double* _
   Trap tp{0};
   tr_s3 rec;
   rec.cell = s.cell;
-  rec.pad = 0;
+  rec.pad = s.pad;
   rec.r1 = s.r1;
   rec.r2 = s.r2;
   rec.r3 = subinterval(s.r3r, i3, n3);
@@ -158,6 +158,7 @@ extern "C" __global__ void __launch_bounds__(128) ccrd_tr_level4(const double* _
                                                                   const tr_aux* __restrict__ aux,
                                                                   cell5* __restrict__ leaves, uint64_t cap,
                                                                   uint32_t* __restrict__ leaf_cell,
+                                                                  uint32_t* __restrict__ leaf_piece,
                                                                   unsigned long long* __restrict__ count,
                                                                   int* __restrict__ trap) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -189,6 +190,91 @@ extern "C" __global__ void __launch_bounds__(128) ccrd_tr_level4(const double* _
   if (keep && slot < cap) {
     leaves[slot] = v;
     leaf_cell[slot] = s.cell;
+    leaf_piece[slot] = s.pad;
+  }
+}
+
+// The per-launch merge of leaves (merge_adjacent_r1,
+// tight_rectangle.cu:129-159) joins the leaves whose r1 intervals form one
+// connected component of their union, hull over all radii.  Every leaf of
+// one (cell, r1 piece) bucket has the piece's r1 upper bound (tightening
+// only raises r1's lower bound, to at most r2's upper bound, which r2's
+// range keeps below r1's), so a bucket lies in one component and the merge
+// only needs each bucket's hull.  This kernel forms them: per bucket, the
+// least lower bound and greatest upper bound of r1..r4, the least r1 upper
+// bound (equal to the greatest one when the premise holds), and the leaf
+// count, as order-preserving 64-bit keys (10 words per bucket, initialised
+// by the caller to the identities).  A leaf with r1's lower bound above its
+// upper bound sets *bad.  A wave whose live leaves share one bucket (the
+// common case: a wave's leaves come from a few level-3 records) reduces in
+// registers and issues one set of atomics.
+CCD unsigned long long ord_key(double x) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(x);
+  return (b >> 63) ? ~b : b | (1ull << 63);
+}
+
+extern "C" __global__ void __launch_bounds__(256) cc_tr_bucket_hull(const cell5* __restrict__ leaves,
+                                                                    const uint32_t* __restrict__ leaf_cell,
+                                                                    const uint32_t* __restrict__ leaf_piece,
+                                                                    uint64_t n, int n1,
+                                                                    unsigned long long* __restrict__ buckets,
+                                                                    int* __restrict__ bad) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool live = i < n;
+  const uint64_t lives = __ballot(live);
+  if (!lives) return;  // the whole wave
+  uint64_t key = 0;
+  unsigned long long lo[4], hi[4], hmin;
+  if (live) {
+    const cell5 c = leaves[i];
+    key = (uint64_t)leaf_cell[i] * (uint64_t)n1 + leaf_piece[i];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      lo[q] = ord_key(c.r[q].lo);
+      hi[q] = ord_key(c.r[q].hi);
+    }
+    hmin = hi[0];
+    if (!(c.r[0].lo <= c.r[0].hi)) atomicOr(bad, 1);
+  } else {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      lo[q] = ~0ull;
+      hi[q] = 0ull;
+    }
+    hmin = ~0ull;
+  }
+  const int first = __ffsll((unsigned long long)lives) - 1;
+  const uint64_t k0 = __shfl(key, first);
+  if (__ballot(live && key != k0) == 0) {
+    for (int off = 32; off > 0; off >>= 1) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const unsigned long long a = __shfl_xor(lo[q], off), b = __shfl_xor(hi[q], off);
+        lo[q] = a < lo[q] ? a : lo[q];
+        hi[q] = b > hi[q] ? b : hi[q];
+      }
+      const unsigned long long h = __shfl_xor(hmin, off);
+      hmin = h < hmin ? h : hmin;
+    }
+    if (__lane_id() == first) {
+      unsigned long long* b = buckets + 10 * k0;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        atomicMin(b + q, lo[q]);
+        atomicMax(b + 4 + q, hi[q]);
+      }
+      atomicMin(b + 8, hmin);
+      atomicAdd(b + 9, (unsigned long long)__popcll(lives));
+    }
+  } else if (live) {
+    unsigned long long* b = buckets + 10 * key;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      atomicMin(b + q, lo[q]);
+      atomicMax(b + 4 + q, hi[q]);
+    }
+    atomicMin(b + 8, hmin);
+    atomicAdd(b + 9, 1ull);
   }
 }
 

@@ -181,3 +181,61 @@ def test_gpu_tr_overflow_protocol(prover):
 @pytest.mark.gpu
 def test_gpu_tr_empty_batch(prover):
     assert prover.tr_enumerate(np.empty((0, 10)), C1_COUNTS).shape[0] == 0
+
+
+def merge_by_r1_piece(leaves):
+    """One launch's merge as the device path forms it: the hull of the
+    leaves sharing an r1 upper bound (one r1 piece), then the sweep of those
+    hulls in order of their least r1 lower bound, joined where it is at most
+    the running r1 upper bound."""
+    ub = leaves[:, 3]
+    keys, inv = np.unique(ub, return_inverse=True)
+    hulls = np.empty((keys.size, 10))
+    for k in range(keys.size):
+        g = leaves[inv == k]
+        hulls[k, 0:2] = g[0, 0:2]
+        hulls[k, 2::2] = g[:, 2::2].min(axis=0)
+        hulls[k, 3::2] = g[:, 3::2].max(axis=0)
+    hulls = hulls[np.argsort(hulls[:, 2], kind="stable")]
+    out = []
+    for h in hulls:
+        if out and h[2] <= out[-1][3]:
+            out[-1][2::2] = np.minimum(out[-1][2::2], h[2::2])
+            out[-1][3::2] = np.maximum(out[-1][3::2], h[3::2])
+        else:
+            out.append(h.copy())
+    return np.array(out).reshape(-1, 10)
+
+
+@pytest.mark.parametrize("which", ["initial", "initial_fine_r1", "generation"])
+def test_merge_by_r1_piece_is_the_reference_merge(which):
+    """The premise of the device-side merge: every leaf of a launch keeps its
+    r1 piece's upper bound and r1.lb <= r1.ub, so the hulls per piece swept
+    in order give merge_adjacent_r1's records, bit for bit (oracle leaves of
+    several launches; also with every other r1 piece's leaves left out,
+    which opens gaps, so that the merge ends in several records)."""
+    if which == "initial":
+        launches = [ref_init_cell(i) for i in (0, 1, 40, 100)]
+        counts = (32, 32, 32, 32)
+    elif which == "initial_fine_r1":
+        launches = [ref_init_cell(i) for i in (0, 100)]
+        counts = (128, 16, 16, 16)
+    else:
+        first, _ = oracle.tr_find_critical(ref_init_cell(0)[None], (32, 32, 32, 32))
+        launches = list(oracle.tr_merge_adjacent_r1(first))[:4]
+        counts = (16, 16, 16, 8)
+    most = 0
+    for cell in launches:
+        leaves, trap = oracle.tr_find_critical(np.asarray(cell)[None], counts)
+        assert not trap
+        if leaves.shape[0] == 0:
+            continue
+        assert (leaves[:, 2] <= leaves[:, 3]).all()
+        _, piece = np.unique(leaves[:, 3], return_inverse=True)
+        for part in (leaves, leaves[piece % 2 == 0]):
+            ref = oracle.tr_merge_adjacent_r1(part)
+            got = merge_by_r1_piece(part)
+            assert got.shape == ref.shape
+            assert (got.view(np.uint64) == ref.view(np.uint64)).all()
+            most = max(most, ref.shape[0])
+    assert most >= 2
