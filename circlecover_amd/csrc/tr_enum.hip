@@ -205,9 +205,11 @@ extern "C" __global__ void __launch_bounds__(128) ccrd_tr_level4(const double* _
 // bound (equal to the greatest one when the premise holds), and the leaf
 // count, as order-preserving 64-bit keys (10 words per bucket, initialised
 // by the caller to the identities).  A leaf with r1's lower bound above its
-// upper bound sets *bad.  A wave whose live leaves share one bucket (the
-// common case: a wave's leaves come from a few level-3 records) reduces in
-// registers and issues one set of atomics.
+// upper bound sets *bad.  Within a wave, an inclusive segmented scan over
+// runs of equal bucket keys (the leaves of a run come from the same few
+// level-3 records) leaves each run's hull in its last lane, which alone
+// issues the run's atomics: one set per run instead of one per leaf, so
+// the few hot buckets of a launch do not serialise the kernel.
 CCD unsigned long long ord_key(double x) {
   const unsigned long long b = (unsigned long long)__double_as_longlong(x);
   return (b >> 63) ? ~b : b | (1ull << 63);
@@ -243,38 +245,46 @@ extern "C" __global__ void __launch_bounds__(256) cc_tr_bucket_hull(const cell5*
     }
     hmin = ~0ull;
   }
-  const int first = __ffsll((unsigned long long)lives) - 1;
-  const uint64_t k0 = __shfl(key, first);
-  if (__ballot(live && key != k0) == 0) {
-    for (int off = 32; off > 0; off >>= 1) {
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const unsigned long long a = __shfl_xor(lo[q], off), b = __shfl_xor(hi[q], off);
-        lo[q] = a < lo[q] ? a : lo[q];
-        hi[q] = b > hi[q] ? b : hi[q];
-      }
-      const unsigned long long h = __shfl_xor(hmin, off);
-      hmin = h < hmin ? h : hmin;
-    }
-    if (__lane_id() == first) {
-      unsigned long long* b = buckets + 10 * k0;
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        atomicMin(b + q, lo[q]);
-        atomicMax(b + 4 + q, hi[q]);
-      }
-      atomicMin(b + 8, hmin);
-      atomicAdd(b + 9, (unsigned long long)__popcll(lives));
-    }
-  } else if (live) {
-    unsigned long long* b = buckets + 10 * key;
+  const int lane = __lane_id();
+  // (head, hull) pairs combined left to right; a head starts a new run
+  const uint64_t prev_key = __shfl_up(key, 1);
+  bool head = lane == 0 || prev_key != key || !live;
+  unsigned cnt = live ? 1u : 0u;
+  for (int off = 1; off < 64; off <<= 1) {
+    const bool ok = lane >= off;
+    const bool h = __shfl_up((int)head, off) != 0;
+    const unsigned c = __shfl_up(cnt, off);
+    unsigned long long l[4], u[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-      atomicMin(b + q, lo[q]);
-      atomicMax(b + 4 + q, hi[q]);
+      l[q] = __shfl_up(lo[q], off);
+      u[q] = __shfl_up(hi[q], off);
     }
-    atomicMin(b + 8, hmin);
-    atomicAdd(b + 9, 1ull);
+    const unsigned long long m = __shfl_up(hmin, off);
+    if (ok && !head) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        lo[q] = l[q] < lo[q] ? l[q] : lo[q];
+        hi[q] = u[q] > hi[q] ? u[q] : hi[q];
+      }
+      hmin = m < hmin ? m : hmin;
+      cnt += c;
+    }
+    if (ok) head = head || h;
+  }
+  // the last lane of a run: the next lane starts another run or is dead
+  const uint64_t next_key = __shfl_down(key, 1);
+  const bool next_live = __shfl_down((int)live, 1) != 0;
+  const bool last = live && (lane == 63 || !next_live || next_key != key);
+  if (last) {
+    unsigned long long* bk = buckets + 10 * key;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      atomicMin(bk + q, lo[q]);
+      atomicMax(bk + 4 + q, hi[q]);
+    }
+    atomicMin(bk + 8, hmin);
+    atomicAdd(bk + 9, (unsigned long long)cnt);
   }
 }
 
