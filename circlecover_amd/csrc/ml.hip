@@ -230,7 +230,7 @@ __device__ __forceinline__ bf16x4 bf16_quad(float a, float b, float c, float d) 
 // ties); logits (may be null) receives the 10 logits of each box.  list
 // (may be null): position i ranks box list[i] (outputs at position i); the
 // positions are [0, n), n = *n_dev when n_dev is not null.
-extern "C" __global__ void __launch_bounds__(256) ccml_rank(const double* __restrict__ soa, uint64_t n, uint64_t ld,
+extern "C" __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) ccml_rank(const double* __restrict__ soa, uint64_t n, uint64_t ld,
                                                              const uint32_t* __restrict__ list,
                                                              const uint32_t* __restrict__ n_dev,
                                                              const float* __restrict__ norm,
@@ -277,9 +277,11 @@ extern "C" __global__ void __launch_bounds__(256) ccml_rank(const double* __rest
   const uint64_t tiles = (n + 31) / 32;
   const uint64_t stride = (uint64_t)gridDim.x * 4;
   uint64_t tile = (uint64_t)blockIdx.x * 4 + wv;
-  // this lane's two dimensions (2g, 2g + 1) of its box, loaded one tile ahead
-  double v[4] = {0.0, 0.0, 0.0, 0.0};
-  auto load = [&](uint64_t t) {
+  // this lane's two dimensions (2g, 2g + 1) of its box, loaded two of the
+  // wave's tiles ahead into two alternating buffers: a tile's work is far
+  // shorter than the memory latency under load
+  double v0[4] = {0.0, 0.0, 0.0, 0.0}, v1[4] = {0.0, 0.0, 0.0, 0.0};
+  auto load = [&](double (&v)[4], uint64_t t) {
     const uint64_t i = t * 32 + col;
     if (t < tiles && i < n) {
       const uint64_t box = list ? list[i] : i;
@@ -287,8 +289,7 @@ extern "C" __global__ void __launch_bounds__(256) ccml_rank(const double* __rest
       for (int e = 0; e < 4; ++e) v[e] = soa[(4 * g + e) * ld + box];
     }
   };
-  load(tile);
-  for (; tile < tiles; tile += stride) {
+  auto step = [&](double (&v)[4], uint64_t tile) {
     const uint64_t i = tile * 32 + col;
     const bool live = i < n;
     // layer 1: B = features 4g..4g+3 of box i (centre and log2 width of
@@ -305,7 +306,7 @@ extern "C" __global__ void __launch_bounds__(256) ccml_rank(const double* __rest
       f[2 * d + 1] = live ? (lw - norm[k + 1]) / norm[kIn + k + 1] : 0.f;
     }
     const bf16x4 x = bf16_quad<false>(f[0], f[1], f[2], f[3]);
-    load(tile + stride);
+    load(v, tile + 2 * stride);
     f32x16 h1[2];
 #pragma unroll
     for (int t = 0; t < 2; ++t) {
@@ -360,6 +361,12 @@ extern "C" __global__ void __launch_bounds__(256) ccml_rank(const double* __rest
         }
       }
     }
+  };
+  load(v0, tile);
+  load(v1, tile + stride);
+  for (; tile < tiles; tile += 2 * stride) {  // tile and stride are the same on every lane of the wave
+    step(v0, tile);
+    if (tile + stride < tiles) step(v1, tile + stride);
   }
 }
 
