@@ -196,6 +196,9 @@ typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef float f32x2 __attribute__((ext_vector_type(2)));
 typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
 typedef int i32x2 __attribute__((ext_vector_type(2)));
+typedef int i32x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
 
 // Two floats to a packed BF16 pair, rounding to nearest even (one
 // v_cvt_pk_bf16_f32 on gfx950); with relu, negative inputs become 0 first.
@@ -212,6 +215,25 @@ template <bool relu>
 __device__ __forceinline__ bf16x4 bf16_quad(float a, float b, float c, float d) {
   const i32x2 w = {bf16_pair<relu>(a, b), bf16_pair<relu>(c, d)};
   return __builtin_bit_cast(bf16x4, w);
+}
+// Accumulator elements r0..r0+7 of h as the BF16 octet of an MFMA operand.
+template <bool relu>
+__device__ __forceinline__ bf16x8 bf16_oct(const f32x16& h, int r0) {
+  const i32x4 w = {bf16_pair<relu>(h[r0], h[r0 + 1]), bf16_pair<relu>(h[r0 + 2], h[r0 + 3]),
+                   bf16_pair<relu>(h[r0 + 4], h[r0 + 5]), bf16_pair<relu>(h[r0 + 6], h[r0 + 7])};
+  return __builtin_bit_cast(bf16x8, w);
+}
+// The hidden unit that K index k (0..63) of layers 2 and 3 reads.  Those
+// layers use v_mfma_f32_32x32x16_bf16, whose B operand gives lane l the K
+// indices 16 s + 8 (l / 32) + (0..7) at step s; a lane holds the previous
+// layer's units 32 t + 8 q + 4 (l / 32) + (0..3) as accumulator elements
+// 4 q + (0..3) of block t.  Step s takes elements 8 (s % 2) .. + 7 of block
+// s / 2 (q = 2 (s % 2) + (0, 1)), so K index 16 s + 8 g + 4 b + e is unit
+// 32 (s / 2) + 16 (s % 2) + 8 b + 4 g + e, and the weight rows are read in
+// that order: no activation moves between lanes.
+__device__ __forceinline__ int unit_of_k(int k) {
+  const int s = k >> 4, g = (k >> 3) & 1, b = (k >> 2) & 1, e = k & 3;
+  return 32 * (s >> 1) + 16 * (s & 1) + 8 * b + 4 * g + e;
 }
 
 // Batched forward on MFMA, with activations kept in registers.  Operand
@@ -248,19 +270,25 @@ extern "C" __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_pe
   const uint16_t* W1 = wb;
   const uint16_t* W2 = wb + kIn * kH;
   const uint16_t* W3 = W2 + kH * kH;
-  bf16x4 a1[2], a2[8][2], a3[8];
+  bf16x4 a1[2];
+  bf16x8 a2[4][2], a3[4];
 #pragma unroll
   for (int t = 0; t < 2; ++t)
 #pragma unroll
     for (int e = 0; e < 4; ++e) a1[t][e] = (short)W1[(4 * g + e) * kH + 32 * t + col];
 #pragma unroll
-  for (int s = 0; s < 8; ++s) {
+  for (int s = 0; s < 4; ++s) {
+    u16x8 w2[2], w3;
 #pragma unroll
-    for (int t = 0; t < 2; ++t)
+    for (int e = 0; e < 8; ++e) {
+      const int u = unit_of_k(16 * s + 8 * g + e);
 #pragma unroll
-      for (int e = 0; e < 4; ++e) a2[s][t][e] = (short)W2[(8 * s + 4 * g + e) * kH + 32 * t + col];
+      for (int t = 0; t < 2; ++t) w2[t][e] = W2[u * kH + 32 * t + col];
+      w3[e] = col < kOut ? W3[u * kOut + col] : (unsigned short)0;
+    }
 #pragma unroll
-    for (int e = 0; e < 4; ++e) a3[s][e] = (short)(col < kOut ? W3[(8 * s + 4 * g + e) * kOut + col] : 0);
+    for (int t = 0; t < 2; ++t) a2[s][t] = __builtin_bit_cast(bf16x8, w2[t]);
+    a3[s] = __builtin_bit_cast(bf16x8, w3);
   }
   // accumulator start r..r+3 of the units 8q + 4g + (0..3), q = r / 4
   auto start = [&](int off, f32x16& c) {
@@ -313,28 +341,23 @@ extern "C" __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_pe
       start(32 * t, h1[t]);
       h1[t] = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a1[t], x, h1[t], 0, 0, 0);
     }
-    // layer 2: step s takes units 8s + 4g + e, held in h1[s / 4][4 (s % 4) + e]
+    // layer 2: step s takes accumulator elements 8 (s % 2) .. + 7 of h1[s / 2]
+    // (units unit_of_k(16 s + 8 g + 0..7))
     f32x16 h2[2];
     start(kH, h2[0]);
     start(kH + 32, h2[1]);
 #pragma unroll
-    for (int s = 0; s < 8; ++s) {
-      const f32x16& h = h1[s >> 2];
-      const int r = 4 * (s & 3);
-      const bf16x4 b = bf16_quad<true>(h[r], h[r + 1], h[r + 2], h[r + 3]);
+    for (int s = 0; s < 4; ++s) {
+      const bf16x8 b = bf16_oct<true>(h1[s >> 1], 8 * (s & 1));
 #pragma unroll
-      for (int t = 0; t < 2; ++t) h2[t] = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a2[s][t], b, h2[t], 0, 0, 0);
+      for (int t = 0; t < 2; ++t) h2[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a2[s][t], b, h2[t], 0, 0, 0);
     }
     // layer 3: logits of classes 8 (r / 4) + 4g + (r % 4) of box i
     f32x16 z3;
     start(2 * kH, z3);
 #pragma unroll
-    for (int s = 0; s < 8; ++s) {
-      const f32x16& h = h2[s >> 2];
-      const int r = 4 * (s & 3);
-      z3 = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a3[s], bf16_quad<true>(h[r], h[r + 1], h[r + 2], h[r + 3]), z3, 0,
-                                                  0, 0);
-    }
+    for (int s = 0; s < 4; ++s)
+      z3 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a3[s], bf16_oct<true>(h2[s >> 1], 8 * (s & 1)), z3, 0, 0, 0);
     // classes here: g = 0 holds 0..3 and 8..10; g = 1 holds 4..7.  Best of
     // each lane in increasing class order, then the pair of lanes l, l ^ 32
     // combined (larger logit; on equal logits the lower class).
