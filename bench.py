@@ -61,7 +61,47 @@ def cpu_model() -> str:
     return "unknown"
 
 
+def host_cpu() -> dict:
+    """The host as lscpu describes it, and the CPUs this process may use."""
+    import subprocess
+
+    info = {}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            info[k.strip()] = v.strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+
+    def num(key):
+        try:
+            return int(info[key])
+        except (KeyError, ValueError):
+            return None
+
+    return {"model": info.get("Model name", cpu_model()), "sockets": num("Socket(s)"),
+            "cores_per_socket": num("Core(s) per socket"), "threads_per_core": num("Thread(s) per core"),
+            "cpus_online": num("CPU(s)"), "cpus_in_affinity": len(os.sched_getaffinity(0))}
+
+
+C3_SAMPLE = 32768  # cells of the 256x64x64 grid in the CPU leg of the branch-and-bound
+
+
+def c3_sample_cells():
+    import circlecover_amd as cc
+
+    g = cc.rsb_grid(1.0, 8.0, 256, 64, 64)
+    return g[np.sort(np.random.default_rng(3).choice(g.shape[0], C3_SAMPLE, replace=False))]
+
+
 def cpu_baseline(soa: np.ndarray, threads: int) -> dict:
+    """The CPU oracle on this host's cores, on bounded samples of three
+    workloads: the ten-criterion batch (all boxes on `threads` threads, and
+    2^16 boxes on one thread), the small tight_rectangle branch-and-bound
+    (all of it) and the rectangle_size_bound branch-and-bound on a fixed
+    sample of its grid's cells."""
+    import circlecover_amd as cc
     import oracle
 
     n = soa.shape[1]
@@ -72,12 +112,30 @@ def cpu_baseline(soa: np.ndarray, threads: int) -> dict:
     t0 = time.perf_counter()
     oracle.c2_eval(one, threads=1)
     dt1 = time.perf_counter() - t0
+    # tight_rectangle small branch-and-bound: 64 lambda cells, 64/32/32/32
+    # pieces, three bisection levels
+    t0 = time.perf_counter()
+    leaves, _ = oracle.tr_find_critical(cc.tr_initial_cells(1.0, 1.5, 64), (64, 32, 32, 32), threads=threads)
+    for _ in range(3):
+        leaves, _ = oracle.tr_find_critical(oracle.tr_bisect(leaves), (1, 1, 1, 1), threads=threads)
+    c1_s = time.perf_counter() - t0
+    cells = c3_sample_cells()
+    t0 = time.perf_counter()
+    unproven, per, _, _ = oracle.rsb_bnb(cells, 24, threads=threads)
+    c3_s = time.perf_counter() - t0
     return {"value": n * N_CRIT / dt, "unit": "cell-evals/s", "cores": threads, "kind": "port",
             "sample": f"first {n} of the {N_BOXES} seeded boxes, all ten criteria, CPU oracle "
                       f"(x86 MXCSR directed rounding), {threads} threads, {dt:.2f} s",
             "single_thread_value": one.shape[1] * N_CRIT / dt1,
             "single_thread_sample": f"first {one.shape[1]} boxes, 1 thread, {dt1:.2f} s",
-            "cpu_model": cpu_model()}
+            "c1_s": c1_s, "c1_unproven": int(leaves.shape[0]),
+            "c1_sample": "the whole small tight_rectangle branch-and-bound (64 lambda cells of [1, 1.5], "
+                         f"64/32/32/32 pieces, 3 bisection levels), {threads} threads",
+            "c3_sample_s": c3_s, "c3_sample_cells_evaluated": int(per.sum()),
+            "c3_sample_unproven": int(unproven.shape[0]),
+            "c3_sample": f"{C3_SAMPLE} seeded cells of the 256x64x64 rectangle_size_bound grid, depth 24, "
+                         f"{threads} threads",
+            "host": host_cpu(), "cpu_model": cpu_model()}
 
 
 def ops_per_box(soa: np.ndarray) -> float:
@@ -171,6 +229,14 @@ def extended(p, soa: np.ndarray, mask: np.ndarray) -> dict:
                       "cells_evaluated": int(sum(sb.cells_per_depth[: sb.depth_reached + 1])),
                       "cell_evals": int(sb.cell_evals), "cell_evals_per_s_batched": sb.cell_evals / t_b,
                       "cell_evals_per_s_plain": sp.cell_evals / t_p, "trap_cells": int(sb.trap_cells)}
+    # the CPU leg's sample of the same grid, on the GPU (batched form)
+    sample = c3_sample_cells()
+    p.rsb_bnb_batched(sample, 24)
+    t = time.perf_counter()
+    us, ss = p.rsb_bnb_batched(sample, 24)
+    out["rsb_bnb"]["sample"] = {"cells": int(sample.shape[0]), "batched_s": time.perf_counter() - t,
+                                "unproven": int(us.shape[0]),
+                                "cells_evaluated": int(sum(ss.cells_per_depth[: ss.depth_reached + 1]))}
     # tight_rectangle small branch-and-bound
     tc = cc.tr_initial_cells(1.0, 1.5, 64)
     p.tr_bnb(tc, (64, 32, 32, 32), 3)
@@ -351,8 +417,9 @@ def main() -> int:
             workloads["full_proof"] = full_proof()
         out["workloads"] = workloads
     if not args.no_cpu:
-        threads = len(os.sched_getaffinity(0))
-        threads = max(1, min(threads, 16))
+        # every CPU this process may use, up to the 16 a one-GPU job is
+        # granted on the measurement host (more threads would only share them)
+        threads = max(1, min(len(os.sched_getaffinity(0)), 16))
         log(f"CPU oracle baseline on {threads} threads")
         out["cpu_baseline"] = cpu_baseline(soa, threads)
     print(json.dumps(out), flush=True)

@@ -5,6 +5,7 @@
 #define CC_EXACT_SHORTCUTS 1
 #include "dev/queue.hpp"
 #include "dev/rsb.hpp"
+#include "dev/f32.hpp"
 
 using namespace ccd;
 
@@ -34,16 +35,19 @@ static constexpr int kStripCap = 4;
 // r1_stage (5 rows of stride ns = the launch's lane count) and r1_id, and
 // their number in r1_cnt[wave]: the second pass reads them densely, with no
 // atomics on either side.
-extern "C" __global__ void __launch_bounds__(256) ccrd_c2_eval(const double* __restrict__ soa, uint64_t n,
-                                                                uint64_t ld, const Konst* __restrict__ kp,
-                                                                uint32_t* __restrict__ mask,
-                                                                uint8_t* __restrict__ first,
-                                                                uint32_t* __restrict__ deferred,
-                                                                unsigned long long* __restrict__ n_deferred,
-                                                                double* __restrict__ d_stage,
-                                                                double* __restrict__ r1_stage,
-                                                                uint32_t* __restrict__ r1_id,
-                                                                uint32_t* __restrict__ r1_cnt, uint64_t ns) {
+//
+// kF32Pre: criterion 0 is first tried as the FP32 shadow of its FP64
+// computation (dev/f32.hpp), which proves a box only when the FP64 criterion
+// proves it too; the FP64 criterion runs on the boxes the shadow leaves
+// undecided, so the outputs are those of the FP64 form.
+template <bool kF32Pre>
+__device__ __forceinline__ void c2_eval_body(const double* __restrict__ soa, uint64_t n, uint64_t ld,
+                                             const Konst* __restrict__ kp, uint32_t* __restrict__ mask,
+                                             uint8_t* __restrict__ first, uint32_t* __restrict__ deferred,
+                                             unsigned long long* __restrict__ n_deferred,
+                                             double* __restrict__ d_stage, double* __restrict__ r1_stage,
+                                             uint32_t* __restrict__ r1_id, uint32_t* __restrict__ r1_cnt,
+                                             uint64_t ns) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   // Lanes past n stay to the wave-wide append below and store nothing.
   const bool live = i < n;
@@ -60,7 +64,15 @@ extern "C" __global__ void __launch_bounds__(256) ccrd_c2_eval(const double* __r
 
   uint32_t m = 0;
   Trap tp{0};
-  if (trc::r1_strip(K, la, W, r1, r2.hi)) m |= 1u << 0;
+  bool c0 = false;
+  if constexpr (kF32Pre) {
+    const f32::KonstF KF = f32::konst(K);
+    const f32::siv las = f32::sv(la), r1s = f32::sv(r1);
+    f32::Sh st{true};
+    const f32::siv Ws = f32::crit_ratio(KF, las, st) * las;  // required_weight_for(la)
+    c0 = st.ok && f32::r1_strategies(KF, las, Ws, r1s, f32::shd(r2.hi), st) && st.ok;
+  }
+  if (c0 || trc::r1_strip(K, la, W, r1, r2.hi)) m |= 1u << 0;
   if (trc::r1_r2_simple(K, la, W, r1, r2, r3.hi, tp)) m |= 1u << 1;
   if (tp.flag) m |= 1u << 17;
   if (trc::r1_r2_stacked(K, la, W, r1, r2, r3.hi)) m |= 1u << 2;
@@ -101,6 +113,32 @@ extern "C" __global__ void __launch_bounds__(256) ccrd_c2_eval(const double* __r
   mask[i] = m;
   const uint32_t v = m & 0x3ffu;
   first[i] = v ? (uint8_t)__builtin_ctz(v) : (uint8_t)255;
+}
+
+extern "C" __global__ void __launch_bounds__(256) ccrd_c2_eval(const double* __restrict__ soa, uint64_t n,
+                                                                uint64_t ld, const Konst* __restrict__ kp,
+                                                                uint32_t* __restrict__ mask,
+                                                                uint8_t* __restrict__ first,
+                                                                uint32_t* __restrict__ deferred,
+                                                                unsigned long long* __restrict__ n_deferred,
+                                                                double* __restrict__ d_stage,
+                                                                double* __restrict__ r1_stage,
+                                                                uint32_t* __restrict__ r1_id,
+                                                                uint32_t* __restrict__ r1_cnt, uint64_t ns) {
+  c2_eval_body<false>(soa, n, ld, kp, mask, first, deferred, n_deferred, d_stage, r1_stage, r1_id, r1_cnt, ns);
+}
+
+extern "C" __global__ void __launch_bounds__(256) ccrd_c2_eval_f32pre(const double* __restrict__ soa, uint64_t n,
+                                                                       uint64_t ld, const Konst* __restrict__ kp,
+                                                                       uint32_t* __restrict__ mask,
+                                                                       uint8_t* __restrict__ first,
+                                                                       uint32_t* __restrict__ deferred,
+                                                                       unsigned long long* __restrict__ n_deferred,
+                                                                       double* __restrict__ d_stage,
+                                                                       double* __restrict__ r1_stage,
+                                                                       uint32_t* __restrict__ r1_id,
+                                                                       uint32_t* __restrict__ r1_cnt, uint64_t ns) {
+  c2_eval_body<true>(soa, n, ld, kp, mask, first, deferred, n_deferred, d_stage, r1_stage, r1_id, r1_cnt, ns);
 }
 
 // Second pass over the boxes ccrd_c2_eval deferred (their data read densely

@@ -525,44 +525,35 @@ CCD bool strip_search(double q1, double q2, double q3, double& lo, double& hi, d
 #endif
 
 // r1_r2_r3_strategy_cover_strip: r1_r2_r3_strategies.cu:69-94
-// `trace` (diagnostics only, null in production): per iteration lo, hi, mid.
-CCD bool r123_cover_strip(const Konst& K, iv la, iv W, iv r1, iv r2, iv r3, double ub_r4,
-                          double* trace = nullptr, int trace_cap = 0) {
+CCD bool r123_cover_strip(const Konst& K, iv la, iv W, iv r1, iv r2, iv r3, double ub_r4) {
   const double q1 = mul_rd(4.0, r1.lo), q2 = mul_rd(4.0, r2.lo), q3 = mul_rd(4.0, r3.lo);
   double lo = 0.3, hi = la.hi;
 #if CC_EXACT_SHORTCUTS
   // The bisection keeps lo accepted and hi rejected (or the initial bound)
   // and stops when no double lies strictly between them; on a range where
   // the test is true-then-false it therefore ends at lo = the last accepted
-  // double below hi, which strip_search finds with a handful of tests.  The
-  // diagnostic trace keeps the full loop.
-  if (!trace) {
-    double s_lo;
-    if (!strip_eval(q1, q2, q3, lo, s_lo)) return false;
-    if (!strip_search(q1, q2, q3, lo, hi, s_lo, kBisectCap)) {
-      for (int it = 0; it < kBisectCap; ++it) {
-        double mid = rn_mid(lo, hi);
-        if (mid <= lo || mid >= hi) break;
-        if (strip_width_ok(q1, q2, q3, mid)) lo = mid;
-        else hi = mid;
-      }
+  // double below hi, which strip_search finds with a handful of tests.
+  double s_lo;
+  if (!strip_eval(q1, q2, q3, lo, s_lo)) return false;
+  if (!strip_search(q1, q2, q3, lo, hi, s_lo, kBisectCap)) {
+    for (int it = 0; it < kBisectCap; ++it) {
+      double mid = rn_mid(lo, hi);
+      if (mid <= lo || mid >= hi) break;
+      if (strip_width_ok(q1, q2, q3, mid)) lo = mid;
+      else hi = mid;
     }
-    return can_recurse(K, add_ru(la.hi, -lo), 1.0, ub_r4, (W - r1 - r2 - r3).lo);
   }
-#endif
+  return can_recurse(K, add_ru(la.hi, -lo), 1.0, ub_r4, (W - r1 - r2 - r3).lo);
+#else
   if (!strip_width_ok(q1, q2, q3, lo)) return false;
   for (int it = 0; it < kBisectCap; ++it) {
     double mid = rn_mid(lo, hi);
-    if (trace && it < trace_cap) {
-      trace[3 * it + 0] = lo;
-      trace[3 * it + 1] = hi;
-      trace[3 * it + 2] = mid;
-    }
     if (mid <= lo || mid >= hi) break;
     if (strip_width_ok(q1, q2, q3, mid)) lo = mid;
     else hi = mid;
   }
   return can_recurse(K, add_ru(la.hi, -lo), 1.0, ub_r4, (W - r1 - r2 - r3).lo);
+#endif
 }
 
 #if CC_EXACT_SHORTCUTS

@@ -135,6 +135,7 @@ int cc_ranker_train(cc_ctx* ctx, const double* soa, size_t n, size_t ld, const u
                     int batch, float lr, uint64_t seed, cc_ranker* model, float* epoch_loss) {
   if (!ctx || !soa || !label || !model || ld < n || epochs < 0 || batch < 1 || batch > (1 << 20)) return CC_E_ARG;
   CC_HIP(ctx, hipSetDevice(ctx->device));
+  clear_interrupt(ctx);
   MlState& M = ml(ctx);
   std::vector<uint32_t> lab;  // boxes some criterion proves
   for (size_t i = 0; i < n; ++i)
@@ -181,7 +182,7 @@ int cc_ranker_train(cc_ctx* ctx, const double* soa, size_t n, size_t ld, const u
   std::vector<uint32_t> perm = lab;
   std::vector<float> losses(lab.size());
   for (int e = 0; e < epochs; ++e) {
-    if (ctx->interrupt) return fail(ctx, CC_E_INTERRUPTED, "interrupted");
+    if (interrupted(ctx)) return fail(ctx, CC_E_INTERRUPTED, "interrupted");
     for (size_t i = perm.size() - 1; i > 0; --i) {  // Fisher-Yates
       const size_t j = (size_t)(rng.next() % (i + 1));
       const uint32_t tmp = perm[i];

@@ -17,6 +17,7 @@
 #include <cstring>
 #include <iomanip>
 #include <iostream>
+#include <new>
 #include <sstream>
 #include <stdexcept>
 #include <functional>
@@ -144,7 +145,7 @@ template <class Rec>
 struct Checkpoint {
   std::string path;
   char kind[8] = {0};
-  int32_t params[8] = {0};
+  int32_t params[24] = {0};
   int32_t phase = 0;
   uint64_t pos = 0;
   std::vector<Rec> current, next;
@@ -156,19 +157,38 @@ struct Checkpoint {
     FILE* f = std::fopen(path.c_str(), "rb");
     if (!f) return CC_OK;
     char magic[8], k[8];
-    int32_t pr[8], ph = 0, pad = 0;
+    int32_t pr[24], ph = 0, pad = 0;
     uint64_t ps = 0, nc = 0, nn = 0;
-    bool ok = std::fread(magic, 1, 8, f) == 8 && !std::memcmp(magic, "CCCKPT01", 8) && std::fread(k, 1, 8, f) == 8 &&
+    bool ok = std::fread(magic, 1, 8, f) == 8 && !std::memcmp(magic, "CCCKPT02", 8) && std::fread(k, 1, 8, f) == 8 &&
               !std::memcmp(k, kind, 8) && std::fread(pr, sizeof pr, 1, f) == 1 && !std::memcmp(pr, params, sizeof pr) &&
               std::fread(&ph, 4, 1, f) == 1 && std::fread(&pad, 4, 1, f) == 1 && std::fread(&ps, 8, 1, f) == 1 &&
-              std::fread(&nc, 8, 1, f) == 1 && std::fread(&nn, 8, 1, f) == 1 && nc < (1ull << 40) && nn < (1ull << 40);
+              std::fread(&nc, 8, 1, f) == 1 && std::fread(&nn, 8, 1, f) == 1;
+    // The record counts must account for exactly the rest of the file, so
+    // that a truncated or corrupt file is refused before anything is sized
+    // from it.
     if (ok) {
-      current.resize(nc);
-      next.resize(nn);
-      ok = (nc == 0 || std::fread(current.data(), sizeof(Rec), nc, f) == nc) &&
+      const long here = std::ftell(f);
+      ok = here >= 0 && std::fseek(f, 0, SEEK_END) == 0;
+      const long end = ok ? std::ftell(f) : -1;
+      ok = ok && end >= here && std::fseek(f, here, SEEK_SET) == 0;
+      const uint64_t rest = ok ? (uint64_t)(end - here) : 0, per = sizeof(Rec);
+      ok = ok && nc <= rest / per && nn <= rest / per && (nc + nn) * per == rest;
+    }
+    if (ok) {
+      try {
+        current.resize(nc);
+        next.resize(nn);
+      } catch (const std::bad_alloc&) {
+        ok = false;
+      }
+      ok = ok && (nc == 0 || std::fread(current.data(), sizeof(Rec), nc, f) == nc) &&
            (nn == 0 || std::fread(next.data(), sizeof(Rec), nn, f) == nn);
     }
     std::fclose(f);
+    if (!ok) {
+      current.clear();
+      next.clear();
+    }
     if (!ok) return fail(ctx, CC_E_ARG, "checkpoint file does not match this proof");
     phase = ph;
     pos = ps;
@@ -181,7 +201,7 @@ struct Checkpoint {
     if (!f) return fail(ctx, CC_E_ARG, "cannot write the checkpoint file");
     const int32_t pad = 0;
     const uint64_t nc = current.size(), nn = next.size();
-    bool ok = std::fwrite("CCCKPT01", 1, 8, f) == 8 && std::fwrite(kind, 1, 8, f) == 8 &&
+    bool ok = std::fwrite("CCCKPT02", 1, 8, f) == 8 && std::fwrite(kind, 1, 8, f) == 8 &&
               std::fwrite(params, sizeof params, 1, f) == 1 && std::fwrite(&phase, 4, 1, f) == 1 &&
               std::fwrite(&pad, 4, 1, f) == 1 && std::fwrite(&pos, 8, 1, f) == 1 && std::fwrite(&nc, 8, 1, f) == 1 &&
               std::fwrite(&nn, 8, 1, f) == 1 &&
@@ -274,13 +294,13 @@ constexpr size_t kTrBatch = 16;
 // stop the run (CC_E_INTERRUPTED) once the batch's records are appended.
 int tr_launches(cc_ctx* ctx, const std::vector<cc_tr_cell>& launches, bool init, std::vector<cc_tr_cell>& append_to,
                 std::ostream& err, size_t start, const std::function<int(size_t)>& after) {
-  static const int kInit[4] = {128, 128, 128, 128}, kSub[4] = {128, 128, 128, 64};
   for (size_t b = start; b < launches.size(); b += kTrBatch) {
-    if (g_interrupted || ctx->interrupt) return CC_E_INTERRUPTED;
+    if (g_interrupted || interrupted(ctx)) return CC_E_INTERRUPTED;
     const size_t m = std::min(kTrBatch, launches.size() - b);
     int trap = 0;
     const size_t before = append_to.size();
-    int rc = tr_enumerate_merged(ctx, launches.data() + b, m, init ? kInit : kSub, append_to, &trap);
+    int rc = tr_enumerate_merged(ctx, launches.data() + b, m, init ? ctx->tr_init_counts : ctx->tr_gen_counts,
+                                 append_to, &trap);
     if (rc || trap) append_to.resize(before);
     if (rc) return rc;
     if (trap) return fail(ctx, CC_E_TRAP, "a cell reached a sqrt of a possibly negative interval");
@@ -295,9 +315,38 @@ int tr_launches(cc_ctx* ctx, const std::vector<cc_tr_cell>& launches, bool init,
 
 extern "C" {
 
+int cc_tr_prove_shape(cc_ctx* ctx, const int init_counts[4], const int gen_counts[4]) {
+  if (!ctx) return CC_E_ARG;
+  static const int kInit[4] = {128, 128, 128, 128}, kGen[4] = {128, 128, 128, 64};
+  const int* a = init_counts ? init_counts : kInit;
+  const int* b = gen_counts ? gen_counts : kGen;
+  for (int k = 0; k < 4; ++k)
+    if (a[k] < 1 || b[k] < 1 || a[k] > 4096 || b[k] > 4096) return fail(ctx, CC_E_ARG, "piece counts must be 1..4096");
+  std::memcpy(ctx->tr_init_counts, a, sizeof ctx->tr_init_counts);
+  std::memcpy(ctx->tr_gen_counts, b, sizeof ctx->tr_gen_counts);
+  return CC_OK;
+}
+
+int cc_rsb_prove_shape(cc_ctx* ctx, const int init_counts[4], const int gen_grid[3], const int gen_counts[4]) {
+  if (!ctx) return CC_E_ARG;
+  static const int kInit[4] = {64, 64, 64, 64}, kGen[4] = {16, 16, 16, 32}, kGrid[3] = {64, 32, 32};
+  const int* a = init_counts ? init_counts : kInit;
+  const int* b = gen_counts ? gen_counts : kGen;
+  const int* g = gen_grid ? gen_grid : kGrid;
+  for (int k = 0; k < 4; ++k)  // r4..r6 pieces of the enumeration: 1..256
+    if (a[k] < 1 || b[k] < 1 || (k > 0 && (a[k] > 256 || b[k] > 256)) || a[k] > 4096 || b[k] > 4096)
+      return fail(ctx, CC_E_ARG, "piece counts must be 1..4096 for r3 and 1..256 for r4..r6");
+  for (int k = 0; k < 3; ++k)
+    if (g[k] < 1 || g[k] > 4096) return fail(ctx, CC_E_ARG, "generation grid sizes must be 1..4096");
+  std::memcpy(ctx->rsb_init_counts, a, sizeof ctx->rsb_init_counts);
+  std::memcpy(ctx->rsb_gen_counts, b, sizeof ctx->rsb_gen_counts);
+  std::memcpy(ctx->rsb_gen_grid, g, sizeof ctx->rsb_gen_grid);
+  return CC_OK;
+}
+
 int cc_interrupt(cc_ctx* ctx) {
   if (!ctx) return CC_E_ARG;
-  ctx->interrupt = 1;
+  ctx->interrupt.store(1, std::memory_order_relaxed);
   return CC_OK;
 }
 
@@ -308,6 +357,7 @@ int cc_tr_prove(cc_ctx* ctx, int la_init, int la_first, int la_count, int genera
 int cc_tr_prove_ex(cc_ctx* ctx, int la_init, int la_first, int la_count, int generations, const char* checkpoint,
                    int max_batches, FILE* out) {
   if (!ctx || max_batches < 0) return CC_E_ARG;
+  clear_interrupt(ctx);
   static const int kSubdivSchedule[11] = {1, 2, 2, 2, 2, 1, 2, 1, 2, 1, 2};  // tight_rectangle.cu:50-53
   if (la_init <= 0) la_init = 256;
   if (la_first < 0 || la_first >= la_init) la_first = 0;
@@ -327,6 +377,8 @@ int cc_tr_prove_ex(cc_ctx* ctx, int la_init, int la_first, int la_count, int gen
     std::memcpy(ck.kind, "TR", 2);
     const int32_t p[4] = {la_init, la_first, la_count, generations};
     std::memcpy(ck.params, p, sizeof p);
+    std::memcpy(ck.params + 4, ctx->tr_init_counts, sizeof ctx->tr_init_counts);
+    std::memcpy(ck.params + 8, ctx->tr_gen_counts, sizeof ctx->tr_gen_counts);
     if ((rc = ck.load(ctx, resumed))) return rc;
     pr.ck = &ck;
   }
@@ -379,6 +431,7 @@ int cc_tr_prove_ex(cc_ctx* ctx, int la_init, int la_first, int la_count, int gen
       });
       if (rc == CC_E_INTERRUPTED && pr.stopped) return rc;  // to be resumed: no report yet
       if (rc == CC_E_INTERRUPTED) {                        // keep the best complete generation
+        err << "Interrupted during generation " << g << ": reporting generation " << (g - 1) << std::endl;
         if ((rc = pr.save(ctx, g, last_done, current, next))) return rc;
         completed = false;
         break;
@@ -459,6 +512,7 @@ int cc_rsb_prove(cc_ctx* ctx, int la_init, int la_first, int la_count, int r1_in
 int cc_rsb_prove_ex(cc_ctx* ctx, int la_init, int la_first, int la_count, int r1_init, int r2_init, int generations,
                     const char* checkpoint, int max_batches, FILE* out) {
   if (!ctx || max_batches < 0) return CC_E_ARG;
+  clear_interrupt(ctx);
   // rectangle_size_bound.cu:36-62 and values.hpp:28-35
   if (la_init <= 0) la_init = 256;
   if (la_first < 0 || la_first >= la_init) la_first = 0;
@@ -466,7 +520,9 @@ int cc_rsb_prove_ex(cc_ctx* ctx, int la_init, int la_first, int la_count, int r1
   if (r1_init <= 0) r1_init = 128;
   if (r2_init <= 0) r2_init = 128;
   if (generations < 0) generations = 1;
-  static const int kInit[4] = {64, 64, 64, 64}, kSub[4] = {16, 16, 16, 32};
+  const int* kInit = ctx->rsb_init_counts;
+  const int* kSub = ctx->rsb_gen_counts;
+  const int* kGrid = ctx->rsb_gen_grid;
   const double w = 9.0 / 64.0;
   std::ostringstream o;
   std::ostream& err = std::cerr;
@@ -492,7 +548,7 @@ int cc_rsb_prove_ex(cc_ctx* ctx, int la_init, int la_first, int la_count, int r1
     const size_t m = cells.size();
     const size_t chunk = 4096;
     for (size_t b = start; b < m; b += chunk) {
-      if (g_interrupted || ctx->interrupt) return CC_E_INTERRUPTED;
+      if (g_interrupted || interrupted(ctx)) return CC_E_INTERRUPTED;
       const size_t k = std::min(chunk, m - b);
       std::vector<cc_rsb_cell> leaves(1 << 16);
       size_t n = 0;
@@ -519,6 +575,9 @@ int cc_rsb_prove_ex(cc_ctx* ctx, int la_init, int la_first, int la_count, int r1
     std::memcpy(ck.kind, "RSB", 3);
     const int32_t p[6] = {la_init, la_first, la_count, r1_init, r2_init, generations};
     std::memcpy(ck.params, p, sizeof p);
+    std::memcpy(ck.params + 6, ctx->rsb_init_counts, sizeof ctx->rsb_init_counts);
+    std::memcpy(ck.params + 10, ctx->rsb_gen_counts, sizeof ctx->rsb_gen_counts);
+    std::memcpy(ck.params + 14, ctx->rsb_gen_grid, sizeof ctx->rsb_gen_grid);
     if ((rc = ck.load(ctx, resumed))) return rc;
     pr.ck = &ck;
   }
@@ -562,8 +621,13 @@ int cc_rsb_prove_ex(cc_ctx* ctx, int la_init, int la_first, int la_count, int r1
       bool stop = false;
       // position in a generation: the cells of `current` done
       for (size_t j = g == phase0 ? pos0 : 0; j < current.size(); ++j) {
-        rc = run(current[j], 64, 0, 64, 32, 32, kSub, next, 0, none);  // rectangle_size_bound.cu:50-52
+        const size_t mark = next.size();  // the leaves of cells 0..j-1
+        rc = run(current[j], kGrid[0], 0, kGrid[0], kGrid[1], kGrid[2], kSub, next, 0,
+                 none);  // rectangle_size_bound.cu:50-52
         if (rc == CC_E_INTERRUPTED) {
+          // cell j is redone from its start on resume: drop its partial leaves
+          next.resize(mark);
+          err << "Interrupted during generation " << g << ": reporting generation " << (g - 1) << std::endl;
           if ((rc = pr.save(ctx, g, j, current, next))) return rc;
           stop = true;
           break;

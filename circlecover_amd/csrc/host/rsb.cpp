@@ -214,6 +214,7 @@ int cc_rsb_bnb(cc_ctx* ctx, const cc_rsb_cell* cells, size_t ncells, int max_dep
   if (!ctx || !n_out || (ncells && !cells) || (cap && !out) || max_depth < 0 || max_depth >= CC_BNB_MAX_DEPTH)
     return CC_E_ARG;
   CC_HIP(ctx, hipSetDevice(ctx->device));
+  clear_interrupt(ctx);
   const auto t0 = std::chrono::steady_clock::now();
   Scratch& S = scratch(ctx);
   hipFunction_t f;
@@ -229,7 +230,7 @@ int cc_rsb_bnb(cc_ctx* ctx, const cc_rsb_cell* cells, size_t ncells, int max_dep
   uint64_t n = ncells, unproven = 0;
   int d = 0;
   for (; d <= max_depth && n; ++d) {
-    if (ctx->interrupt) return fail(ctx, CC_E_INTERRUPTED, "interrupted");
+    if (interrupted(ctx)) return fail(ctx, CC_E_INTERRUPTED, "interrupted");
     st.cells_per_depth[d] = n;
     const int at_limit = d == max_depth;
     CC_HIP(ctx, S.next.ensure(at_limit ? kRec : 2 * n * kRec));

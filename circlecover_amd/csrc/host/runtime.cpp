@@ -38,6 +38,42 @@ void release_scratch(cc_ctx* ctx) {
   ctx->scratch = nullptr;
 }
 
+int c2_batch_check(cc_ctx* ctx, int prover, int precision, const double* d_soa, size_t n, size_t ld,
+                   const uint32_t* d_mask, const uint8_t* d_first) {
+  if (!ctx) return CC_E_ARG;
+  if (prover != CC_BATCH_C2) return fail(ctx, CC_E_ARG, "unknown criterion set");
+  if (precision != CC_PREC_FP64 && precision != CC_PREC_FP32_PRE) return fail(ctx, CC_E_ARG, "unknown precision");
+  if (ld < n) return fail(ctx, CC_E_ARG, "ld < n");
+  if (n == 0) return CC_OK;
+  if (!d_soa || !d_mask || !d_first) return fail(ctx, CC_E_ARG, "null buffer");
+  if (n > 0xffffffffull) return fail(ctx, CC_E_ARG, "batch too large");
+  return CC_OK;
+}
+
+int c2_batch_launch(cc_ctx* ctx, int prover, int precision, const double* d_soa, size_t n, size_t ld,
+                    uint32_t* d_mask, uint8_t* d_first, const C2Scratch& s) {
+  if (int rc = c2_batch_check(ctx, prover, precision, d_soa, n, ld, d_mask, d_first)) return rc;
+  if (n == 0) return CC_OK;
+  const unsigned block = 256;
+  const uint64_t ns = c2_lanes(n), blocks = ns / block, nwaves = ns / 64;
+  const uint64_t n64 = n, ld64 = ld, one = 1;
+  hipFunction_t f_eval = precision == CC_PREC_FP32_PRE ? ctx->f_c2_eval_f32pre : ctx->f_c2_eval;
+  CC_HIP(ctx, launch(ctx->f_zero_u64, dim3(1), dim3(64), 0, ctx->stream, s.cnt, one));
+  CC_HIP(ctx, launch(f_eval, dim3((unsigned)blocks), dim3(block), 0, ctx->stream, d_soa, n64, ld64, ctx->d_konst,
+                     d_mask, d_first, s.list, s.cnt, s.dstage, s.r1_stage, s.r1_id, s.r1_cnt, ns));
+  // The deferred boxes are few (well under 1% of a typical batch); 1024
+  // blocks of 16 groups of 16 lanes cover 16384 of them in one sweep and
+  // stride over more.
+  CC_HIP(ctx, launch(ctx->f_c2_deferred, dim3(1024), dim3(block), 0, ctx->stream, (const double*)s.dstage, n64,
+                     ctx->d_konst, (const uint32_t*)s.list, (const unsigned long long*)s.cnt, d_mask, d_first));
+  // Criterion 0's remaining strategies: the staging slots of 32 waves per block.
+  const uint64_t r1_blocks = (nwaves + 31) / 32;
+  CC_HIP(ctx, launch(ctx->f_c2_r1, dim3((unsigned)r1_blocks), dim3(block), 0, ctx->stream,
+                     (const double*)s.r1_stage, (const uint32_t*)s.r1_id, (const uint32_t*)s.r1_cnt, nwaves, ns,
+                     ctx->d_konst, d_mask, d_first));
+  return CC_OK;
+}
+
 }  // namespace cchost
 
 using namespace cchost;
@@ -73,9 +109,12 @@ int cc_init(int device, cc_ctx** out) {
     const char* name;
   } fns[] = {{&ctx->f_init_konst, "ccrd_init_konst"}, {&ctx->f_c2_eval, "ccrd_c2_eval"}, {&ctx->f_prim, "ccrd_prim"},
                 {&ctx->f_prim_rn, "ccrd_prim_rn"},
-                {&ctx->f_trace_strip, "ccrd_c2_trace_strip"},
                 {&ctx->f_rsb_verdict, "ccrd_rsb_verdict"},
-                {&ctx->f_rsb_masks, "ccrd_rsb_masks"}};
+                {&ctx->f_rsb_masks, "ccrd_rsb_masks"},
+                {&ctx->f_c2_eval_f32pre, "ccrd_c2_eval_f32pre"},
+                {&ctx->f_zero_u64, "cc_zero_u64"},
+                {&ctx->f_c2_deferred, "ccrd_c2_strip_deferred"},
+                {&ctx->f_c2_r1, "ccrd_c2_r1_pass"}};
   for (auto& fn : fns)
     if ((e = hipModuleGetFunction(fn.f, ctx->mod, fn.name)) != hipSuccess) return bail(e, fn.name);
   if ((e = hipEventCreate(&ctx->ev0)) != hipSuccess) return bail(e, "hipEventCreate");
@@ -163,59 +202,17 @@ int cc_timer_stop(cc_ctx* ctx, float* ms) {
 
 int cc_eval_batch_async(cc_ctx* ctx, int prover, int precision, const double* d_soa, size_t n, size_t ld,
                         uint32_t* d_mask, uint8_t* d_first) {
-  if (!ctx) return CC_E_ARG;
-  if (prover != CC_BATCH_C2) return fail(ctx, CC_E_ARG, "unknown criterion set");
-  if (precision != CC_PREC_FP64) return fail(ctx, CC_E_ARG, "unsupported precision");
-  if (ld < n) return fail(ctx, CC_E_ARG, "ld < n");
+  if (int rc = c2_batch_check(ctx, prover, precision, d_soa, n, ld, d_mask, d_first)) return rc;
   if (n == 0) return CC_OK;
-  if (!d_soa || !d_mask || !d_first) return fail(ctx, CC_E_ARG, "null buffer");
-  const unsigned block = 256;
-  const uint64_t blocks = (n + block - 1) / block;
-  if (blocks > 0x7fffffffull || n > 0xffffffffull) return fail(ctx, CC_E_ARG, "batch too large");
-  uint64_t n64 = n, ld64 = ld;
-  // Boxes whose cover-strip search the first pass defers: a counter, a list
-  // of up to n box indices, and their data (8 rows of stride n).
   DevBuf& db = scratch(ctx).c2_defer;
-  CC_HIP(ctx, db.ensure(256 + 4 * n + 8 * 8 * n));
-  unsigned long long* d_cnt = db.as<unsigned long long>();
-  double* d_dstage = reinterpret_cast<double*>(db.as<char>() + 256);
-  uint32_t* d_list = reinterpret_cast<uint32_t*>(d_dstage + 8 * n);
-  hipFunction_t f_zero, f_rest;
-  CC_HIP(ctx, kernel(ctx, "cc_zero_u64", &f_zero));
-  CC_HIP(ctx, kernel(ctx, "ccrd_c2_strip_deferred", &f_rest));
-  uint64_t one = 1;
-  // Per-wave staging for criterion 0's second pass: 5 FP64 rows and the box
-  // indices over all ns lanes of the launch, then one count per wave.
-  const uint64_t ns = blocks * block, nwaves = ns / 64;
-  DevBuf& rb = scratch(ctx).c2_r1;
-  CC_HIP(ctx, rb.ensure(ns * (5 * sizeof(double) + sizeof(uint32_t)) + nwaves * sizeof(uint32_t)));
-  double* d_r1_stage = rb.as<double>();
-  uint32_t* d_r1_id = reinterpret_cast<uint32_t*>(d_r1_stage + 5 * ns);
-  uint32_t* d_r1_cnt = d_r1_id + ns;
-  CC_HIP(ctx, launch(f_zero, dim3(1), dim3(64), 0, ctx->stream, d_cnt, one));
-  CC_HIP(ctx, launch(ctx->f_c2_eval, dim3((unsigned)blocks), dim3(block), 0, ctx->stream, d_soa, n64, ld64,
-                     ctx->d_konst, d_mask, d_first, d_list, d_cnt, d_dstage, d_r1_stage, d_r1_id, d_r1_cnt, ns));
-  // The deferred boxes are few (well under 1% of a typical batch); 1024
-  // blocks of 16 groups of 16 lanes cover 16384 of them in one sweep and
-  // stride over more.
-  CC_HIP(ctx, launch(f_rest, dim3(1024), dim3(block), 0, ctx->stream, (const double*)d_dstage, n64, ctx->d_konst,
-                     (const uint32_t*)d_list, (const unsigned long long*)d_cnt, d_mask, d_first));
-  // Criterion 0's remaining strategies: the staging slots of 32 waves per block.
-  hipFunction_t f_r1;
-  CC_HIP(ctx, kernel(ctx, "ccrd_c2_r1_pass", &f_r1));
-  const uint64_t r1_blocks = (nwaves + 31) / 32;
-  CC_HIP(ctx, launch(f_r1, dim3((unsigned)r1_blocks), dim3(block), 0, ctx->stream, (const double*)d_r1_stage,
-                     (const uint32_t*)d_r1_id, (const uint32_t*)d_r1_cnt, nwaves, ns, ctx->d_konst, d_mask,
-                     d_first));
-  return CC_OK;
+  CC_HIP(ctx, db.ensure(c2_scratch_bytes(n)));
+  return c2_batch_launch(ctx, prover, precision, d_soa, n, ld, d_mask, d_first, c2_scratch_at(db.p, n));
 }
 
 int cc_eval_batch(cc_ctx* ctx, int prover, int precision, const double* soa, size_t n, size_t ld,
                   uint32_t* mask, uint8_t* first) {
-  if (!ctx) return CC_E_ARG;
-  if (ld < n) return fail(ctx, CC_E_ARG, "ld < n");
+  if (int rc = c2_batch_check(ctx, prover, precision, soa, n, ld, mask, first)) return rc;
   if (n == 0) return CC_OK;
-  if (!soa || !mask || !first) return fail(ctx, CC_E_ARG, "null buffer");
   CC_HIP(ctx, hipSetDevice(ctx->device));
   double* d_soa = nullptr;
   uint32_t* d_mask = nullptr;
@@ -355,26 +352,6 @@ int cc_prim_rn_selftest(cc_ctx* ctx, const double* a, const double* b, const dou
   (void)hipStreamSynchronize(ctx->stream);
   for (double* p : {da, db, dc, dout})
     if (p) (void)hipFree(p);
-  return rc;
-}
-
-int cc_debug_trace_strip(cc_ctx* ctx, const double* box, double* trace, int cap) {
-  if (!ctx || !box || !trace || cap <= 0 || cap > 8192) return CC_E_ARG;
-  CC_HIP(ctx, hipSetDevice(ctx->device));
-  double *dbox = nullptr, *dtr = nullptr;
-  int rc = CC_OK;
-  size_t tb = (3 * (size_t)cap + 1) * sizeof(double);
-  hipError_t e = hipMalloc(&dbox, 64);
-  if (e == hipSuccess) e = hipMalloc(&dtr, tb);
-  if (e == hipSuccess) e = hipMemcpyAsync(dbox, box, 64, hipMemcpyHostToDevice, ctx->stream);
-  if (e == hipSuccess)
-    e = launch(ctx->f_trace_strip, dim3(1), dim3(64), 0, ctx->stream, (const double*)dbox, ctx->d_konst, dtr, cap);
-  if (e == hipSuccess) e = hipMemcpyAsync(trace, dtr, tb, hipMemcpyDeviceToHost, ctx->stream);
-  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-  if (e != hipSuccess) rc = hip_fail(ctx, e, "debug_trace_strip");
-  (void)hipStreamSynchronize(ctx->stream);
-  if (dbox) (void)hipFree(dbox);
-  if (dtr) (void)hipFree(dtr);
   return rc;
 }
 

@@ -235,6 +235,7 @@ int cc_tr_bnb(cc_ctx* ctx, const cc_tr_cell* cells, size_t ncells, const int cou
   for (int k = 0; k < 4; ++k)
     if (counts[k] < 1) return fail(ctx, CC_E_ARG, "piece counts must be >= 1");
   CC_HIP(ctx, hipSetDevice(ctx->device));
+  clear_interrupt(ctx);
   Scratch& S = scratch(ctx);
   hipFunction_t fb;
   CC_HIP(ctx, kernel(ctx, "ccrd_tr_bisect", &fb));
@@ -247,7 +248,7 @@ int cc_tr_bnb(cc_ctx* ctx, const cc_tr_cell* cells, size_t ncells, const int cou
   if (per_level) per_level[0] = n;
   static const int kOne[4] = {1, 1, 1, 1};
   for (int l = 1; l <= levels && n; ++l) {
-    if (ctx->interrupt) return fail(ctx, CC_E_INTERRUPTED, "interrupted");
+    if (interrupted(ctx)) return fail(ctx, CC_E_INTERRUPTED, "interrupted");
     const uint64_t nn = n;
     CC_HIP(ctx, S.next.ensure(2 * nn * kCell));
     CC_HIP(ctx, launch(fb, dim3(blocks_for(2 * nn, 256)), dim3(256), 0, ctx->stream, S.leaves.p, nn, S.next.p));

@@ -58,20 +58,6 @@ extern "C" __global__ void ccrd_prim_rn(const double* __restrict__ a, const doub
   out[3 * i + 2] = rn_span_div(a[i], b[i], c[i]);
 }
 
-// Diagnostics: the cover-strip bisection of one box, lo/hi/mid per iteration
-// (trace holds 3*cap doubles), plus the verdict in trace[3*cap].
-extern "C" __global__ void ccrd_c2_trace_strip(const double* __restrict__ box, const Konst* __restrict__ kp,
-                                               double* __restrict__ trace, int cap) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  const Konst K = *kp;
-  const iv la{box[0], box[1]}, r1{box[2], box[3]}, r2{box[4], box[5]}, r3{box[6], box[7]};
-  const iv W = trc::weight_for(K, la);
-  const double rem4 = (W - r1 - r2 - r3).hi;
-  const double ub_r4 = r3.hi < rem4 ? r3.hi : rem4;
-  for (int k = 0; k < 3 * cap; ++k) trace[k] = 0.0;
-  trace[3 * cap] = trc::r123_cover_strip(K, la, W, r1, r2, r3, ub_r4, trace, cap) ? 1.0 : 0.0;
-}
-
 extern "C" __global__ void __launch_bounds__(128) ccrd_rsb_verdict(const double* __restrict__ cells, uint64_t n,
                                                                     const Konst* __restrict__ kp,
                                                                     int* __restrict__ out,

@@ -63,7 +63,6 @@ lib.cc_gen_c2_boxes.argtypes = [_u64, _sz, _vp]
 lib.cc_gen_c2_boxes.restype = None
 lib.cc_prim_selftest.argtypes = [_vp, _vp, _vp, _sz, _vp]
 lib.cc_prim_rn_selftest.argtypes = [_vp, _vp, _vp, _vp, _sz, _vp]
-lib.cc_debug_trace_strip.argtypes = [_vp, _vp, _vp, _i]
 lib.cc_rsb_eval_cells.argtypes = [_vp, _vp, _sz, _vp, _vp]
 lib.cc_rsb_eval_cells_async.argtypes = [_vp, _vp, _sz, _vp, _vp]
 lib.cc_rsb_eval_masks.argtypes = [_vp, _vp, _sz, _vp]
@@ -80,6 +79,10 @@ class FastParams(ctypes.Structure):
 
 
 lib.cc_rsb_bnb_batched.argtypes = [_vp, _vp, _sz, _i, _vp, _vp, _sz, _P(_sz), _vp]
+lib.cc_run_bnb.argtypes = [_vp, _vp, _P(_vp), _P(_sz), _vp]
+lib.cc_free.argtypes = [_vp]
+lib.cc_free.restype = None
+lib.cc_interrupt.argtypes = [_vp]
 lib.cc_rsb_autotune.argtypes = [_vp, _vp, _sz, _i, _vp, _P(ctypes.c_double)]
 RANKER_PARAMS = 5451
 RANKER_CLASSES = 11  # criteria 0..9 first proving a box, 10: none proves it
@@ -155,6 +158,15 @@ class BnbStats(ctypes.Structure):
         return {"cells_per_depth": list(self.cells_per_depth)[: d + 1], "verdict_hist": list(self.verdict_hist),
                 "trap_cells": self.trap_cells, "cell_evals": self.cell_evals, "depth_reached": d,
                 "device_ms": self.device_ms, "wall_ms": self.wall_ms}
+
+
+CC_BNB_PLAIN, CC_BNB_BATCHED = 0, 1
+
+
+class BnbConfig(ctypes.Structure):
+    _fields_ = [("cells", ctypes.c_void_p), ("ncells", ctypes.c_size_t), ("la_lb", ctypes.c_double),
+                ("la_ub", ctypes.c_double), ("nla", ctypes.c_int), ("nr1", ctypes.c_int), ("nr2", ctypes.c_int),
+                ("max_depth", ctypes.c_int), ("form", ctypes.c_int), ("params", ctypes.c_void_p)]
 
 
 def gen_c2_boxes(seed: int, n: int) -> np.ndarray:
@@ -319,6 +331,35 @@ class Prover:
                                                         ctypes.byref(n), ctypes.byref(st)), cap, 14)
         return out, st
 
+    def run_bnb(self, max_depth: int, form: int = CC_BNB_BATCHED, cells: np.ndarray | None = None,
+                grid=None, params=None):
+        """cc_run_bnb over `cells` ((m, 14)) or grid = (la_lb, la_ub, nla, nr1, nr2).
+        Returns (unproven cells in lexicographic order, BnbStats)."""
+        cfg = BnbConfig()
+        keep = []
+        if cells is not None:
+            cells = np.ascontiguousarray(cells, dtype=np.float64).reshape(-1, 14)
+            keep.append(cells)
+            cfg.cells, cfg.ncells = _ptr(cells), cells.shape[0]
+        else:
+            cfg.la_lb, cfg.la_ub, cfg.nla, cfg.nr1, cfg.nr2 = grid
+        cfg.max_depth, cfg.form = max_depth, form
+        fp = FastParams(*params) if params is not None else None
+        cfg.params = ctypes.cast(ctypes.byref(fp), ctypes.c_void_p) if fp is not None else None
+        out, n, st = ctypes.c_void_p(), ctypes.c_size_t(), BnbStats()
+        self._check(lib.cc_run_bnb(self.ctx, ctypes.byref(cfg), ctypes.byref(out), ctypes.byref(n), ctypes.byref(st)))
+        try:
+            res = np.empty((n.value, 14))
+            if n.value:
+                ctypes.memmove(res.ctypes.data, out.value, res.nbytes)
+        finally:
+            lib.cc_free(out)
+        return res, st
+
+    def interrupt(self) -> None:
+        """cc_interrupt: may be called from another thread while a call runs."""
+        self._check(lib.cc_interrupt(self.ctx))
+
     def rsb_bnb_batched(self, cells: np.ndarray, max_depth: int, params=None, cap: int = 1 << 16):
         """Batched branch-and-bound; params = (cpw_a, cpw_b, blocks_per_cu,
         use_graph, waves_per_simd) or None for the tuned default.  Returns
@@ -409,12 +450,6 @@ class Prover:
         out = np.empty((a.size, 3), dtype=np.float64)
         self._check(lib.cc_prim_rn_selftest(self.ctx, _ptr(a), _ptr(b), _ptr(c), a.size, _ptr(out)))
         return out
-
-    def debug_trace_strip(self, box: np.ndarray, cap: int = 4096):
-        box = np.ascontiguousarray(box, dtype=np.float64)
-        tr = np.empty(3 * cap + 1, dtype=np.float64)
-        self._check(lib.cc_debug_trace_strip(self.ctx, _ptr(box), _ptr(tr), cap))
-        return tr[:-1].reshape(cap, 3), bool(tr[-1])
 
     def prim_selftest(self, a: np.ndarray, b: np.ndarray) -> np.ndarray:
         a = np.ascontiguousarray(a, dtype=np.float64)

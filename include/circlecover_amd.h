@@ -6,7 +6,8 @@
  * returning int returns CC_OK (0) or one of the CC_E_* codes; the message of
  * the last failure on a context is available through cc_last_error().
  *
- * Threading: a context is used by one host thread at a time.  Functions
+ * Threading: a context is used by one host thread at a time (cc_interrupt
+ * excepted, which any thread may call).  Functions
  * ending in _async enqueue work on the context's HIP stream and take DEVICE
  * pointers; all other functions are synchronous and take HOST pointers.
  */
@@ -76,7 +77,10 @@ int cc_timer_stop(cc_ctx* ctx, float* ms);
  *      TR/tight_rectangle.cu:169-262, RSB/rectangle_size_bound.cu:161-236) ---- */
 #define CC_BATCH_C2 0      /* the ten-criterion 4-D box set (see cc_eval_batch) */
 #define CC_PREC_FP64 0     /* outward-rounded FP64 intervals */
-#define CC_PREC_FP32_PRE 1 /* FP32 outward pre-filter, undecided boxes re-run in FP64 */
+#define CC_PREC_FP32_PRE 1 /* FP32 pre-filter: criterion 0 is first tried as the FP32 shadow of its
+                              FP64 computation, which proves a box only when the FP64 criterion
+                              does; boxes it leaves undecided, and criteria 1..9, run in FP64.
+                              Outputs equal CC_PREC_FP64's. */
 
 /* Evaluates criterion set `prover` on n boxes stored as 8 SoA FP64 arrays of
  * stride ld >= n: (la.lb, la.ub, r1.lb, r1.ub, r2.lb, r2.ub, r3.lb, r3.ub).
@@ -212,6 +216,30 @@ int cc_rsb_bnb_batched(cc_ctx* ctx, const cc_rsb_cell* cells, size_t ncells, int
 int cc_rsb_autotune(cc_ctx* ctx, const cc_rsb_cell* cells, size_t ncells, int max_depth, cc_fast_params* best,
                     double* best_ms);
 
+/* ---- the branch-and-bound in one call (no reference counterpart; the
+ *      reference subdivides with fixed piece counts instead) ---- */
+#define CC_BNB_PLAIN 0   /* level-synchronous: cc_rsb_bnb */
+#define CC_BNB_BATCHED 1 /* persistent stage kernels and one hipGraph: cc_rsb_bnb_batched */
+typedef struct cc_bnb_config {
+  const cc_rsb_cell* cells; /* initial cells (host memory); NULL: the grid below */
+  size_t ncells;
+  double la_lb, la_ub;      /* grid: lambda in [la_lb, la_ub] split into nla pieces, squared radii in */
+  int nla, nr1, nr2;        /* [0, 9/64] with r1 in nr1 and r2 (<= r1) in nr2 pieces, built as cc_rsb_grid */
+  int max_depth;            /* 0 .. CC_BNB_MAX_DEPTH-1 */
+  int form;                 /* CC_BNB_PLAIN or CC_BNB_BATCHED */
+  const cc_fast_params* params; /* CC_BNB_BATCHED launch shape; NULL: autotuned or default */
+} cc_bnb_config;
+/* Runs the widest-dimension branch-and-bound of cc_rsb_bnb in the chosen
+ * form and returns the unproven cells in lexicographic (lambda, r1..r6)
+ * order, the order of the reference's Variables::operator<
+ * (rectangle_size_bound/variables.hpp:82-94), in an array the library
+ * allocates: *unproven (NULL when *n == 0), released with cc_free.  Both
+ * forms give the same cells and statistics.  Interruptible (cc_interrupt)
+ * in the plain form. */
+int cc_run_bnb(cc_ctx* ctx, const cc_bnb_config* cfg, cc_rsb_cell** unproven, size_t* n, cc_bnb_stats* stats);
+/* Releases an array returned by cc_run_bnb. */
+void cc_free(void* p);
+
 /* ---- learned criterion ranker (no reference counterpart) ----
  * An 8 -> 64 -> 64 -> 11 MLP (ReLU; BF16 weights, FP32 accumulate) that
  * predicts from a box (centre and log2 width of lambda, r1, r2, r3) which of
@@ -319,9 +347,34 @@ int cc_tr_prove_ex(cc_ctx* ctx, int la_init, int la_first, int la_count, int gen
                    int max_batches, FILE* out);
 int cc_rsb_prove_ex(cc_ctx* ctx, int la_init, int la_first, int la_count, int r1_init, int r2_init, int generations,
                     const char* checkpoint, int max_batches, FILE* out);
-/* Asks a running proof or branch-and-bound to stop at its next level
- * boundary (returns CC_E_INTERRUPTED there; reference: was_interrupted,
- * Prover/interrupt.cpp:30-46). */
+/* Piece counts of the two proofs on this context, for shortened runs of the
+ * same procedure (NULL: the reference's constants, which are also the
+ * defaults): tight_rectangle's r1..r4 counts of the initial launches
+ * (128^4) and of the generation launches (128,128,128,64;
+ * TR/tight_rectangle.cu:44-62); rectangle_size_bound's r3..r6 counts of the
+ * initial launches (64^4) and of the generation launches (16,16,16,32), and
+ * the lambda x r1 x r2 host grid of a generation (64x32x32;
+ * RSB/rectangle_size_bound.cu:36-62).  A checkpoint records the shape it was
+ * made with. */
+int cc_tr_prove_shape(cc_ctx* ctx, const int init_counts[4], const int gen_counts[4]);
+int cc_rsb_prove_shape(cc_ctx* ctx, const int init_counts[4], const int gen_grid[3], const int gen_counts[4]);
+/* Asks the call running on ctx to stop at its next launch or level boundary
+ * (reference: was_interrupted, Prover/interrupt.cpp:30-46, polled at
+ * TR/tight_rectangle.cu:481-489 and RSB/rectangle_size_bound.cu:342-349).
+ * The one function that may be called from another thread while a call runs
+ * on ctx.  Polled by cc_tr_prove[_ex], cc_rsb_prove[_ex], cc_tr_bnb,
+ * cc_rsb_bnb, cc_run_bnb (level-synchronous form) and cc_ranker_train, each
+ * of which clears the flag when it starts, as the reference's
+ * start_interruptible_computation does (interrupt.cpp:38-41): an interrupt
+ * never stops a later call.  Results:
+ *   - a proof interrupted during its refinement generations writes the
+ *     report of the last complete generation and returns CC_OK (the
+ *     reference's "best result achieved so far");
+ *   - a proof interrupted during its initial launches returns
+ *     CC_E_INTERRUPTED without a report (the reference cannot be
+ *     interrupted there; its SIGINT ends the process), after saving its
+ *     checkpoint if it has one;
+ *   - the other calls return CC_E_INTERRUPTED and their outputs are unset. */
 int cc_interrupt(cc_ctx* ctx);
 
 /* FP32 pre-filter in plain first-success order: each box is converted
@@ -352,9 +405,6 @@ void cc_subintervals(double lb, double ub, int n, double* out);
 int cc_prim_selftest(cc_ctx* ctx, const double* a, const double* b, size_t n, double* out);
 /* Round-to-nearest helpers self-test: out[3*i+k] = 0.5*(a+b), fma(a,b,c), (a-b)/c. */
 int cc_prim_rn_selftest(cc_ctx* ctx, const double* a, const double* b, const double* c, size_t n, double* out);
-/* Diagnostics: lo/hi/mid of every iteration (3*cap doubles) of the cover-strip
- * bisection (criterion 4) for one box given as 8 doubles; trace[3*cap] = verdict. */
-int cc_debug_trace_strip(cc_ctx* ctx, const double* box, double* trace, int cap);
 
 #ifdef __cplusplus
 }

@@ -38,7 +38,6 @@ def _load(name: str) -> ctypes.CDLL:
     L.or_iv_op.restype = None
     L.or_rn.argtypes = [_d, _d, _d, _vp]
     L.or_rn.restype = None
-    L.or_trace_strip.argtypes = [_vp, _vp, _i]
     L.or_rsb_verdicts.argtypes = [_vp, _sz, _vp, _vp, _i]
     L.or_rsb_masks.argtypes = [_vp, _sz, _vp, _i]
     L.or_tr_find_critical.argtypes = [_vp, _sz, _vp, _vp, _sz, _vp, _vp, _i]
@@ -233,13 +232,6 @@ def rn_helpers(a: float, b: float, c: float):
     out = (ctypes.c_double * 3)()
     lib.or_rn(a, b, c, ctypes.addressof(out))
     return tuple(out)
-
-
-def trace_strip(box, cap: int = 4096):
-    box = np.ascontiguousarray(box, dtype=np.float64)
-    tr = np.zeros(3 * cap, dtype=np.float64)
-    ok = lib.or_trace_strip(box.ctypes.data, tr.ctypes.data, cap)
-    return tr.reshape(cap, 3), bool(ok)
 
 
 def tr_critical_ratio(lb: float, ub: float):

@@ -234,14 +234,6 @@ void or_rn(double a, double b, double c, double* out) {
   out[2] = rn_div(rn_sub(a, b), c);
 }
 
-// Diagnostics twin of the device trace: cover-strip bisection of one box.
-int or_trace_strip(const double* box, double* trace, int cap) {
-  IV la(box[0], box[1]), r1(box[2], box[3]), r2(box[4], box[5]), r3(box[6], box[7]);
-  double a = r3.ub, b = (tr::required_weight_for(la) - r1 - r2 - r3).ub;
-  for (int k = 0; k < 3 * cap; ++k) trace[k] = 0.0;
-  return tr::r1_r2_r3_strategy_cover_strip(la, r1, r2, r3, a < b ? a : b, trace, cap);
-}
-
 // Verdict of n 7-D rectangle_size_bound cells (14 doubles each: la, r1..r6
 // as lb,ub pairs): out[i] = first succeeding step (see rsb_chain.hpp) or -1;
 // trap[i] = 1 when a reference trap site was reached.

@@ -36,8 +36,7 @@ bool r1_r2_strategies_simple(const IV& la, const IV& r1, const IV& r2, double ub
 bool r1_r2_strategies_stacked(const IV& la, const IV& r1, const IV& r2, double ub_r3);
 bool r1_r2_strategies(const IV& la, const IV& r1, const IV& r2, double ub_r3);
 bool r1_r2_r3_strategy_horizontal(const IV& la, const IV& r1, const IV& r2, const IV& r3, double ub_r4);
-bool r1_r2_r3_strategy_cover_strip(const IV& la, const IV& r1, const IV& r2, const IV& r3, double ub_r4,
-                                   double* trace = nullptr, int trace_cap = 0);
+bool r1_r2_r3_strategy_cover_strip(const IV& la, const IV& r1, const IV& r2, const IV& r3, double ub_r4);
 bool r1_r2_r3_strategies(const IV& la, const IV& r1, const IV& r2, const IV& r3, double ub_r4);
 bool r1_r2_r3_strategies(const Cell& c, const IV& R4);
 bool two_by_two_strategies(const Cell& c, const IV& R);

@@ -237,17 +237,11 @@ static bool can_realize_strip_width(double r1, double r2, double r3, double w) {
 }
 
 // r1_r2_r3_strategies.cu:69-94
-bool r1_r2_r3_strategy_cover_strip(const IV& la, const IV& r1, const IV& r2, const IV& r3, double ub_r4,
-                                   double* trace, int trace_cap) {
+bool r1_r2_r3_strategy_cover_strip(const IV& la, const IV& r1, const IV& r2, const IV& r3, double ub_r4) {
   double lo = 0.3, hi = la.ub;
   if (!can_realize_strip_width(r1.lb, r2.lb, r3.lb, lo)) return false;
   for (int it = 0; it < kBisectCap; ++it) {
     double mid = rn_mid(lo, hi);
-    if (trace && it < trace_cap) {
-      trace[3 * it + 0] = lo;
-      trace[3 * it + 1] = hi;
-      trace[3 * it + 2] = mid;
-    }
     if (mid <= lo || mid >= hi) break;
     if (can_realize_strip_width(r1.lb, r2.lb, r3.lb, mid)) lo = mid;
     else hi = mid;

@@ -1,0 +1,387 @@
+"""The ten criteria of the 4-D box set, decided in exact real arithmetic
+(test support, independent of oracle/ and of the kernels).
+
+Each function restates the mathematics of one reference criterion on a
+point cell (lambda, squared radii r1 >= r2 >= r3): the same placements,
+comparisons and constants (the doubles the reference uses, e.g. its 0.61,
+which is below 61/100), and the reference's own quirks, but no rounding:
+every quantity is an mpmath number at 60 significant digits.  A Dec object
+records the smallest relative slack |a - b| / max(|a|, |b|) of every
+comparison the evaluation decides, so a caller knows how far the verdict is
+from flipping.  Where that slack is large, the outward-rounded FP64
+criterion (a few hundred operations, each off by at most one ulp) must reach
+the same verdict; a dropped term, a swapped operand or a wrong constant in a
+transcription moves a quantity by far more than rounding does and flips
+verdicts at clear margins.
+
+Reference files (under Prover/): tight_rectangle/critical_ratio.cu:32-86,
+can_recurse.cu:32-62, can_apply_size_bounded_covering.cu:49-52,
+r1_strategies.cu:27-87, r1_r2_strategies.cu:29-134,
+r1_r2_r3_strategies.cu:28-94; rectangle_size_bound/values.hpp:28-35,
+disk_satisfies_size_bound.cu:25-37, maximize_covered.cu:106-130,
+can_recurse.cu:26-88, 221-235, even_split_recursion.cu:42-57,
+uneven_split_recursion.cu:78-125, r1_r2_opposite_corners_recursion.cu:28-71,
+r1_r2_vertical_strip.cu:28-41.
+"""
+from __future__ import annotations
+
+import mpmath
+from mpmath import mpf
+
+mpmath.mp.dps = 60  # 60 significant digits
+
+INF = mpmath.inf
+CR = mpf(0.61)                       # rectangle_size_bound::critical_ratio, the double
+UB_DISK_WEIGHT = mpf(9) / 64         # (3/8)^2
+UB_DISK_RADIUS = mpf(3) / 8
+C_WORST = mpf(195) / 256             # worst-case ratio floor
+TOL_LAMBDA_RSB = mpf(2.0898841580413813901)  # rectangle_size_bound/can_recurse.cu:28 (the double)
+LAMBDA_2 = mpmath.sqrt(mpmath.sqrt(7) / 2 - mpf(1) / 4)  # break-even point of critical_ratio
+
+
+class Dec:
+    """Comparisons of one evaluation, with the smallest relative slack seen."""
+
+    def __init__(self):
+        self.slack = INF
+
+    def cmp(self, a, op: str, b) -> bool:
+        a, b = mpf(a), mpf(b)
+        scale = max(abs(a), abs(b))
+        self.slack = min(self.slack, abs(a - b) / scale if scale else mpf(0))
+        d = a - b
+        return {"<": d < 0, "<=": d <= 0, ">": d > 0, ">=": d >= 0}[op]
+
+
+# ---------------------------------------------------------------- helpers
+
+def tr_critical_ratio(la):
+    """critical_ratio(IV la) on a point (critical_ratio.cu:58-78); the two
+    forms agree at the break-even point, so the branch needs no slack."""
+    if la <= LAMBDA_2:
+        return (mpf(3) / 256) / la * (16 * la * la + 40 + 9 / (la * la))
+    return la / 4 + 1 / (2 * la)
+
+
+def tr_critical_ratio_wh(w, h):
+    """critical_ratio(IV w, IV h) (critical_ratio.cu:32-52): the ratio of the
+    longer side over the shorter, as the reference orients it."""
+    if w <= h:
+        return tr_critical_ratio(h / w)
+    return tr_critical_ratio(w / h)
+
+
+def size_bound_ok(D, r_sq, h):
+    """disk_satisfies_size_bound: r^2 (1/h)^2 <= 9/64 (disk_satisfies_size_bound.cu:25-28)."""
+    if h == 0:
+        return False
+    return D.cmp(r_sq / (h * h), "<=", UB_DISK_WEIGHT)
+
+
+def tr_can_recurse(D, w, h, largest, rem):
+    """tight_rectangle can_recurse(ub_w, ub_h, ub_largest, lb_rem) (can_recurse.cu:32-62):
+    the worst-case ratio of the w x h rectangle, or the corollary's ratio
+    for its largest disk if lower (at least 195/256), or 0.61 when the
+    largest disk, scaled to the shorter side, meets the size bound."""
+    cr = tr_critical_ratio_wh(w, h)
+    s = min(w, h)
+    norm = largest / (s * s)
+    cor = mpmath.sqrt(mpmath.sqrt(norm * norm + 1) + 1) / 2
+    if D.cmp(cor, "<", C_WORST):
+        cor = C_WORST
+    if D.cmp(cr, ">", cor):
+        cr = cor
+    if D.cmp(norm, "<=", UB_DISK_WEIGHT):
+        cr = CR
+    return D.cmp(rem, ">=", cr * w * h)
+
+
+def two_disks_maximize_height(D, r1, r2, w):
+    """Height of the strip of width w that two disks stacked in it cover
+    (maximize_covered.cu:106-130), 0 if a disk cannot span its part."""
+    d = r1 - r2
+    w1 = (w / 2 + (2 / w) * d) ** 2
+    w2 = (w / 2 - (2 / w) * d) ** 2
+    if D.cmp(r1, "<=", w1 / 4) or D.cmp(r2, "<=", w2 / 4):
+        return mpf(0)
+    a, b = 2 * (r1 + r2), w * w / 4 + 4 * d * d / (w * w)
+    if D.cmp(a, "<=", b):
+        return mpf(0)
+    return mpmath.sqrt(a - b)
+
+
+def rsb_can_recurse(D, rem, largest, w, h):
+    """rectangle_size_bound can_recurse (can_recurse.cu:26-88)."""
+    if D.cmp(rem, "<=", 0):
+        if rem < 0:
+            return False
+        return w <= 0 or h <= 0
+    if w <= 0 or h <= 0:
+        return True
+    smin, smax = min(w, h), max(w, h)
+    area = rem / CR
+    side = area / smax
+    if D.cmp(side, "<", smin):
+        return False
+    if D.cmp(side, ">", smax):
+        side = mpmath.sqrt(area)
+    if D.cmp(UB_DISK_WEIGHT * side * side, ">=", largest):
+        return True
+    skew = smax / smin
+    if D.cmp(skew, "<=", TOL_LAMBDA_RSB):
+        return D.cmp(tr_critical_ratio(skew) * w * h, "<=", rem)
+    sig = (largest / (smin * smin)) ** 2
+    cr = mpmath.sqrt(mpmath.sqrt(sig + 1) + 1) / 2
+    if D.cmp(cr, "<", C_WORST):
+        cr = C_WORST
+    if D.cmp(cr * w * h, "<=", rem):
+        return True
+    return D.cmp(smax * smax / 4 + smin * smin / 2, "<=", rem)
+
+
+def bound_worst_case_ratio(D, w, h):
+    """rectangle_size_bound bound_worst_case_ratio (can_recurse.cu:221-235)."""
+    if w <= 0 or h <= 0:
+        return INF
+    skew = max(w / h, h / w)
+    r = skew / 4 + 1 / (2 * skew)
+    return C_WORST if D.cmp(r, "<", C_WORST) else r
+
+
+# ------------------------------------------------------- tight_rectangle
+
+def r1_strategies(D, la, r1, ub_r2):
+    """Criterion 0 (r1_strategies.cu:27-87).  Returns (verdict, strategy):
+    1 full-height strip of r1 plus recursion, 2 r1's square in a corner with
+    the right strip and the top rectangle, 3 the pocket placement."""
+    rem = tr_critical_ratio(la) * la - r1
+    if D.cmp(4 * r1, ">", 1):
+        s = mpmath.sqrt(4 * r1 - 1)
+        if tr_can_recurse(D, la - s, mpf(1), ub_r2, rem):
+            return True, 1
+    side = mpmath.sqrt(2 * r1)
+    if D.cmp(side, "<", 1):
+        wright = la - side
+        if size_bound_ok(D, ub_r2, min(wright, mpf(1))):
+            wgt = wright * CR
+        else:
+            wgt = tr_critical_ratio_wh(wright, mpf(1)) * wright
+        top = rem - (wgt + ub_r2)
+        if D.cmp(top, ">", 0) and tr_can_recurse(D, side, 1 - side, ub_r2, top):
+            return True, 2
+    if D.cmp(4 * r1, ">", 1):
+        s1 = mpmath.sqrt(4 * r1 - 1)
+        t = (la - s1 / 2) ** 2
+        if D.cmp(r1, ">", t):
+            h1 = 2 * mpmath.sqrt(r1 - t)
+            ph, pw = (1 - h1) / 2, la - s1
+            ru = (rem - ub_r2) / 2
+            if D.cmp(ru, ">", 0) and tr_can_recurse(D, pw, ph, ub_r2, ru):
+                return True, 3
+    return False, 0
+
+
+def manually_proved_two_pocket(D, la, r1, r2):
+    """r1_r2_strategies.cu:29-44.  A negative pocket width la - S1 becomes
+    the interval [0, la - S1] (tighten_lb raises only the lower bound), whose
+    abs() and square are [0, 0] (interval.hpp:216-232): its square is
+    max(la - S1, 0)^2."""
+    if D.cmp(r2, "<", mpf(0.2400000000000000188)) or D.cmp(r1, "<", mpf(0.4900000000000000466)):
+        return False
+    eps = mpf(0.00999999999999999847)
+    if D.cmp(r1, "<", (la * la + 1) / 4 - eps):
+        return False
+    s1 = 2 * mpmath.sqrt(r1 - mpf(1) / 4)
+    return D.cmp(r2, ">=", mpf(1) / 4 + max(la - s1, mpf(0)) ** 2 - eps)
+
+
+def r1_r2_simple(D, la, r1, r2, ub_r3):
+    """Criterion 1 (r1_r2_strategies.cu:46-72).  Strategies: 1 the manual
+    two-disk pocket, 2 side strips wider than lambda, 3 the recursion on the
+    rest, whose width the reference computes as lambda + width (Q10)."""
+    if manually_proved_two_pocket(D, la, r1, r2):
+        return True, 1
+    w = two_disks_maximize_height(D, r1, r2, mpf(1))
+    if w <= 0:
+        return False, 0
+    if D.cmp(4 * r2, ">", 1):
+        lw = mpmath.sqrt(4 * r1 - 1) + mpmath.sqrt(4 * r2 - 1)
+        if D.cmp(lw, ">", w):
+            w = lw
+            if D.cmp(lw, ">", la):
+                return True, 2
+    rem = tr_critical_ratio(la) * la - r1 - r2
+    return (True, 3) if tr_can_recurse(D, la + w, mpf(1), ub_r3, rem) else (False, 0)
+
+
+def r1_r2_stacked(D, la, r1, r2, ub_r3):
+    """Criterion 2 (r1_r2_strategies.cu:74-130): r1's square in the lower
+    left corner, r2's strip above it; strategy 1 the top rectangle
+    size-bounded, 2 the bottom one size-bounded, 3 both by recursion."""
+    side = mpmath.sqrt(2 * r1)
+    if D.cmp(side, ">", 1):
+        return False, 0
+    hrem = 1 - side
+    if D.cmp(4 * r2, "<", hrem * hrem):
+        return False, 0
+    w2 = mpmath.sqrt(4 * r2 - hrem * hrem)
+    w_top, w_bot = la - w2, la - side
+    rem = tr_critical_ratio(la) * la - r1 - r2
+    if size_bound_ok(D, ub_r3, min(w_top, hrem)):
+        left = rem - (CR * w_top * hrem + ub_r3)
+        ok = D.cmp(left, ">", 0) and tr_can_recurse(D, w_bot, side, ub_r3, left)
+        return (True, 1) if ok else (False, 0)
+    if size_bound_ok(D, ub_r3, min(w_bot, side)):
+        left = rem - (CR * w_bot * side + ub_r3)
+        ok = D.cmp(left, ">", 0) and tr_can_recurse(D, w_top, hrem, ub_r3, left)
+        return (True, 2) if ok else (False, 0)
+    left = rem - (tr_critical_ratio_wh(w_top, hrem) * w_top * hrem + ub_r3)
+    ok = D.cmp(left, ">", 0) and tr_can_recurse(D, w_bot, side, ub_r3, left)
+    return (True, 3) if ok else (False, 0)
+
+
+def r123_horizontal(D, la, r1, r2, r3, ub_4):
+    """Criterion 3 (r1_r2_r3_strategies.cu:28-46): three full-height strips;
+    strategy 1 they span lambda, 2 recursion on the rest."""
+    if D.cmp(4 * r3, "<=", 1):
+        return False, 0
+    wcov = sum(mpmath.sqrt(4 * r - 1) for r in (r1, r2, r3))
+    wrem = la - wcov
+    if D.cmp(la, "<=", wcov):
+        return True, 1
+    rem = tr_critical_ratio(la) * la - r1 - r2 - r3
+    return (True, 2) if tr_can_recurse(D, wrem, mpf(1), ub_4, rem) else (False, 0)
+
+
+def strip_width_ok(r1, r2, r3, w):
+    """can_realize_strip_width (r1_r2_r3_strategies.cu:48-67): r1 spans the
+    strip of width w at height h1, r2 and r3 side by side cover the rest of
+    its height with widths summing to at least w."""
+    q = 4 * r1 - w * w
+    if q <= 0:
+        return False
+    hrem = 1 - mpmath.sqrt(q)
+    a, b = 4 * r2 - hrem * hrem, 4 * r3 - hrem * hrem
+    if b <= 0 or a <= 0:
+        return False
+    return mpmath.sqrt(a) + mpmath.sqrt(b) >= w
+
+
+def cover_strip_width(D, la, r1, r2, r3):
+    """The widest strip width in [0.3, lambda] the bisection of
+    r1_r2_r3_strategy_cover_strip accepts, None if 0.3 is not accepted.  The
+    bisection runs on exact midpoints; the decisions of its first levels,
+    where the bracket is still wide, count towards the slack (later levels
+    close in on the same boundary in either arithmetic)."""
+    if not strip_width_ok(r1, r2, r3, mpf("0.3")):
+        return None
+    lo, hi = mpf("0.3"), la
+    for it in range(140):  # to 2^-140 of the bracket, far below a double's resolution
+        mid = (lo + hi) / 2
+        ok = strip_width_ok(r1, r2, r3, mid)
+        if hi - lo > mpf("1e-6"):
+            _strip_slack(D, r1, r2, r3, mid)
+        if ok:
+            lo = mid
+        else:
+            hi = mid
+    return lo
+
+
+def r123_cover_strip(D, la, r1, r2, r3, ub_4):
+    """Criterion 4 (r1_r2_r3_strategies.cu:69-94): r1, r2, r3 cover a
+    full-height strip of the widest width the bisection finds, recursion
+    covers the rest."""
+    w = cover_strip_width(D, la, r1, r2, r3)
+    if w is None:
+        return False, 0
+    rem = tr_critical_ratio(la) * la - r1 - r2 - r3
+    return (True, 1) if tr_can_recurse(D, la - w, mpf(1), ub_4, rem) else (False, 0)
+
+
+def _strip_slack(D, r1, r2, r3, w):
+    """The comparisons strip_width_ok decides at w, for the slack only."""
+    q = 4 * r1 - w * w
+    if not D.cmp(4 * r1, ">", w * w):
+        return
+    hrem = 1 - mpmath.sqrt(q)
+    if not (D.cmp(4 * r3, ">", hrem * hrem) and D.cmp(4 * r2, ">", hrem * hrem)):
+        return
+    D.cmp(mpmath.sqrt(4 * r2 - hrem * hrem) + mpmath.sqrt(4 * r3 - hrem * hrem), ">=", w)
+
+
+# ---------------------------------------------------- rectangle_size_bound
+
+def s1_even_split(D, la, r1, r2):
+    """Criterion 5, shortcut_even_split_recursion (even_split_recursion.cu:42-57)."""
+    big = la / 2
+    if D.cmp(big, "<", 1) and not size_bound_ok(D, r1, big):
+        return False, 0
+    small = big - r2 / CR
+    return (True, 1) if (D.cmp(small, ">=", 1) or size_bound_ok(D, r2, small)) else (False, 0)
+
+
+def s23_uneven_split(D, la, r1, r2):
+    """Criteria 6 and 7, shortcut_uneven_split_recursion (uneven_split_recursion.cu:
+    78-125); the 4-argument form is this one with r3 in place of r2."""
+    if r1 <= 0:
+        return False, 0
+    need = mpmath.sqrt(r1) / UB_DISK_RADIUS
+    if D.cmp(need, ">=", la):
+        return False, 0
+    rem = la * CR - (r2 + need / CR)
+    if D.cmp(la * CR, "<=", r2 + need / CR):
+        return False, 0
+    wrem = rem / CR
+    return (True, 1) if (D.cmp(wrem, ">=", 1) or size_bound_ok(D, r2, wrem)) else (False, 0)
+
+
+def s4_opposite_corners(D, la, r1, r2, r3):
+    """Criterion 8, shortcut_r1_r2_opposite_corners_recursion
+    (r1_r2_opposite_corners_recursion.cu:28-71): r1's and r2's squares in
+    opposite corners, regions A, B, C around them."""
+    w1, w2 = mpmath.sqrt(2 * r1), mpmath.sqrt(2 * r2)
+    comb = w1 + w2
+    if not D.cmp(comb, "<", 1):
+        return False, 0
+    min_ab = min(la - w2, 1 - w1, la - w1, w1)
+    if not size_bound_ok(D, r3, min_ab):
+        return False, 0
+    wc, hc = w2, 1 - comb
+    budget = (2 * CR - 1) * (r1 + r2)
+    if size_bound_ok(D, r3, min(wc, hc)):
+        return (False, 0) if D.cmp(2 * r3, ">", budget) else (True, 1)
+    extra = (bound_worst_case_ratio(D, wc, hc) - CR) * wc * hc
+    return (False, 0) if D.cmp(budget - 2 * r3, "<", extra) else (True, 2)
+
+
+def s5_vertical_strip(D, la, r1, r2, r3):
+    """Criterion 9, shortcut_r1_r2_vertical_strip (r1_r2_vertical_strip.cu:28-41)."""
+    w = two_disks_maximize_height(D, r1, r2, mpf(1))
+    if w <= 0:
+        return False, 0
+    rem = la * CR - (r1 + r2)
+    wrem = la - w
+    if D.cmp(wrem, ">=", 1) or size_bound_ok(D, r3, wrem):
+        return (True, 1) if D.cmp(rem, ">=", CR * wrem) else (False, 0)
+    return (True, 2) if rsb_can_recurse(D, rem, r3, wrem, mpf(1)) else (False, 0)
+
+
+def c2_criteria(la, r1, r2, r3):
+    """All ten criteria on the point cell (la, r1, r2, r3) as the batch
+    evaluates them: [(verdict, strategy, slack)] for criteria 0..9, with
+    ub_r4 = min(r3, required_weight_for(la) - r1 - r2 - r3)
+    (tight_rectangle.cu:97-102)."""
+    la, r1, r2, r3 = (mpf(x) for x in (la, r1, r2, r3))
+    ub4 = min(r3, tr_critical_ratio(la) * la - r1 - r2 - r3)
+    calls = (lambda D: r1_strategies(D, la, r1, r2), lambda D: r1_r2_simple(D, la, r1, r2, r3),
+             lambda D: r1_r2_stacked(D, la, r1, r2, r3), lambda D: r123_horizontal(D, la, r1, r2, r3, ub4),
+             lambda D: r123_cover_strip(D, la, r1, r2, r3, ub4), lambda D: s1_even_split(D, la, r1, r2),
+             lambda D: s23_uneven_split(D, la, r1, r2), lambda D: s23_uneven_split(D, la, r1, r3),
+             lambda D: s4_opposite_corners(D, la, r1, r2, r3), lambda D: s5_vertical_strip(D, la, r1, r2, r3))
+    out = []
+    for f in calls:
+        D = Dec()
+        v, s = f(D)
+        out.append((bool(v), s, D.slack))
+    return out

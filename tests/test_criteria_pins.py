@@ -1,0 +1,233 @@
+"""Independent pins of the ten criteria of the 4-D box set.
+
+tests/criteria_exact.py states each criterion of the reference in exact
+real arithmetic (mpmath, 60 digits), written from the reference's formulas
+and not from oracle/.  On point cells the oracle's outward-rounded FP64
+verdict must equal the exact verdict wherever every comparison the exact
+evaluation decides has a relative slack of at least 1e-11: a few hundred
+rounded operations move a quantity by about 1e-14 at most, while a
+transcription slip (a dropped term, a swapped operand, a sign or a wrong
+constant) moves it by far more.  This pins both directions: the oracle
+proves no cell the mathematics rejects, and misses none it accepts.
+
+The point sets are aimed at each criterion's regime so that every
+criterion, and every strategy inside it, both succeeds and fails; points
+bisected onto each criterion's exact decision boundary add cells whose
+slack is as small as 1e-8.  A mutation check plants plausible slips in the
+exact statement and requires each to be caught.  (The GPU kernels are
+compared with the oracle bit for bit by tests/test_c2_batch.py.)
+"""
+import mpmath
+import numpy as np
+import pytest
+
+import criteria_exact as ce
+import oracle
+
+CLEAR = 1e-11
+STRATEGIES = {0: {1, 2, 3}, 1: {1, 2, 3}, 2: {1, 2, 3}, 3: {1, 2}, 4: {1}, 5: {1}, 6: {1}, 7: {1}, 8: {1, 2},
+              9: {1, 2}}
+
+
+def _points(rng, n, la_rng, r1_rng, r_rng=None):
+    la = rng.uniform(*la_rng, n)
+    if r_rng is None:
+        r = np.sort(rng.uniform(*r1_rng, (n, 3)), axis=1)[:, ::-1]
+    else:
+        r1 = rng.uniform(*r1_rng, n)
+        r23 = np.sort(rng.uniform(*r_rng, (n, 2)), axis=1)[:, ::-1]
+        r = np.column_stack([r1, np.minimum(r23, r1[:, None])])
+    return np.column_stack([la, r])
+
+
+def point_sets():
+    rng = np.random.default_rng(0xC1C1EC)
+    return np.concatenate([
+        _points(rng, 250, (1.0, 2.5), (0.2, 0.8)),            # tight_rectangle radii
+        _points(rng, 250, (1.0, 2.5), (0.0, 9.0 / 64.0)),     # size-bounded radii
+        _points(rng, 250, (1.0, 8.0), (0.0, 0.3)),            # the wide grid of the branch-and-bound
+        _points(rng, 150, (1.0, 3.0), (0.1, 0.5)),
+        _points(rng, 150, (1.0, 1.4), (0.6, 0.8), (0.0, 0.15)),  # r1's pocket placement
+    ])
+
+
+def exact(points, fns=None):
+    """(n, 10) verdicts, strategies and slacks of the exact statement."""
+    v = np.zeros((len(points), 10), bool)
+    st = np.zeros((len(points), 10), int)
+    sl = np.zeros((len(points), 10))
+    for i, p in enumerate(points):
+        for k, (a, b, c) in enumerate(ce.c2_criteria(*p) if fns is None else _with(fns, p)):
+            v[i, k], st[i, k], sl[i, k] = a, b, float(c)
+    return v, st, sl
+
+
+def _with(fns, p):
+    la, r1, r2, r3 = (mpmath.mpf(x) for x in p)
+    ub4 = min(r3, ce.tr_critical_ratio(la) * la - r1 - r2 - r3)
+    args = ((la, r1, r2), (la, r1, r2, r3), (la, r1, r2, r3), (la, r1, r2, r3, ub4), (la, r1, r2, r3, ub4),
+            (la, r1, r2), (la, r1, r2), (la, r1, r3), (la, r1, r2, r3), (la, r1, r2, r3))
+    out = []
+    for f, a in zip(fns, args):
+        D = ce.Dec()
+        v, s = f(D, *a)
+        out.append((bool(v), s, D.slack))
+    return out
+
+
+def oracle_bits(points):
+    p = np.asarray(points, dtype=np.float64)
+    soa = np.ascontiguousarray(np.stack([p[:, 0], p[:, 0], p[:, 1], p[:, 1], p[:, 2], p[:, 2], p[:, 3], p[:, 3]]))
+    mask, _ = oracle.c2_eval(soa)
+    return (mask[:, None] >> np.arange(10)[None, :]) & 1 == 1
+
+
+def boundary_points(points, v, sl, per_criterion=4):
+    """Cells on both sides of each criterion's exact decision boundary:
+    bisect the segment between a cell the criterion clearly proves and one it
+    clearly rejects (convex combinations keep the radii sorted), then step
+    off the boundary by 1e-4, 1e-6 and 1e-8 of the segment."""
+    rng = np.random.default_rng(7)
+    out = []
+    for k in range(10):
+        yes = np.flatnonzero(v[:, k] & (sl[:, k] >= CLEAR))
+        no = np.flatnonzero(~v[:, k] & (sl[:, k] >= CLEAR))
+        if len(yes) == 0 or len(no) == 0:
+            continue
+        for _ in range(per_criterion):
+            a = [mpmath.mpf(x) for x in points[rng.choice(yes)]]
+            b = [mpmath.mpf(x) for x in points[rng.choice(no)]]
+            lo, hi = mpmath.mpf(0), mpmath.mpf(1)  # t = 0 proven, t = 1 rejected
+            for _ in range(64):
+                t = (lo + hi) / 2
+                q = [x + t * (y - x) for x, y in zip(a, b)]
+                if ce.c2_criteria(*q)[k][0]:
+                    lo = t
+                else:
+                    hi = t
+            for d in (1e-4, 1e-6, 1e-8):
+                for t in (lo - d, hi + d):
+                    out.append([float(x + t * (y - x)) for x, y in zip(a, b)])
+    return np.array(out)
+
+
+@pytest.fixture(scope="module")
+def pinned():
+    pts = point_sets()
+    v, st, sl = exact(pts)
+    bnd = boundary_points(pts, v, sl)
+    bv, bst, bsl = exact(bnd)
+    return (np.concatenate([pts, bnd]), np.concatenate([v, bv]), np.concatenate([st, bst]),
+            np.concatenate([sl, bsl]), len(pts))
+
+
+def test_oracle_verdicts_equal_exact_verdicts_at_clear_slack(pinned):
+    pts, v, st, sl, _ = pinned
+    o = oracle_bits(pts)
+    clear = sl >= CLEAR
+    bad = np.argwhere(clear & (o != v))
+    assert len(bad) == 0, [(int(i), int(k), bool(o[i, k]), bool(v[i, k]), float(sl[i, k]), list(pts[i]))
+                           for i, k in bad[:10]]
+    # both directions are exercised, for every criterion and every strategy
+    for k in range(10):
+        assert (clear[:, k] & v[:, k]).sum() >= 15, f"criterion {k} proves too few cells"
+        assert (clear[:, k] & ~v[:, k]).sum() >= 15, f"criterion {k} rejects too few cells"
+        fired = set(st[clear[:, k] & v[:, k], k].tolist())
+        assert STRATEGIES[k] <= fired, f"criterion {k}: strategies {STRATEGIES[k] - fired} never decide"
+
+
+def test_boundary_cells_are_close_and_agree(pinned):
+    pts, v, st, sl, n_plain = pinned
+    o = oracle_bits(pts[n_plain:])
+    bsl, bv = sl[n_plain:], v[n_plain:]
+    for k in range(10):
+        near = bsl[:, k] < 1e-5
+        assert near.sum() >= 4, f"criterion {k}: no cells near its boundary"
+        agree = near & (bsl[:, k] >= CLEAR)
+        assert (o[agree, k] == bv[agree, k]).all()
+
+
+def _mutants():
+    """Plausible transcription slips, each as a replacement of one exact criterion."""
+    m = {}
+
+    def r1_no_split_cost(D, la, r1, ub_r2):  # strategy 2 without the splitting cost ub_r2
+        rem = ce.tr_critical_ratio(la) * la - r1
+        side = mpmath.sqrt(2 * r1)
+        if D.cmp(4 * r1, ">", 1) and ce.tr_can_recurse(D, la - mpmath.sqrt(4 * r1 - 1), mpmath.mpf(1), ub_r2, rem):
+            return True, 1
+        if D.cmp(side, "<", 1):
+            wright = la - side
+            wgt = wright * ce.CR if ce.size_bound_ok(D, ub_r2, min(wright, mpmath.mpf(1))) else \
+                ce.tr_critical_ratio_wh(wright, mpmath.mpf(1)) * wright
+            top = rem - wgt
+            if D.cmp(top, ">", 0) and ce.tr_can_recurse(D, side, 1 - side, ub_r2, top):
+                return True, 2
+        return False, 0
+    m["dropped splitting cost (criterion 0)"] = (0, r1_no_split_cost)
+
+    def simple_fixed_q10(D, la, r1, r2, ub_r3):  # lambda - width instead of the reference's lambda + width
+        w = ce.two_disks_maximize_height(D, r1, r2, mpmath.mpf(1))
+        if w <= 0:
+            return False, 0
+        rem = ce.tr_critical_ratio(la) * la - r1 - r2
+        return (True, 3) if ce.tr_can_recurse(D, la - w, mpmath.mpf(1), ub_r3, rem) else (False, 0)
+    m["sign of the remaining width (criterion 1, quirk Q10)"] = (1, simple_fixed_q10)
+
+    def strip_sign(D, la, r1, r2, r3, ub_4):  # lambda + w for the width left of the strip
+        w = ce.cover_strip_width(D, la, r1, r2, r3)
+        if w is None:
+            return False, 0
+        rem = ce.tr_critical_ratio(la) * la - r1 - r2 - r3
+        return (True, 1) if ce.tr_can_recurse(D, la + w, mpmath.mpf(1), ub_4, rem) else (False, 0)
+    m["sign of the remaining width (criterion 4)"] = (4, strip_sign)
+
+    def even_mul(D, la, r1, r2):  # r2 * 0.61 instead of r2 / 0.61
+        big = la / 2
+        if D.cmp(big, "<", 1) and not ce.size_bound_ok(D, r1, big):
+            return False, 0
+        small = big - r2 * ce.CR
+        return (True, 1) if (D.cmp(small, ">=", 1) or ce.size_bound_ok(D, r2, small)) else (False, 0)
+    m["operation slip r2 / 0.61 -> r2 * 0.61 (criterion 5)"] = (5, even_mul)
+
+    def uneven_swapped(D, la, r1, r2):  # the size bound checked on r1 instead of r2
+        if r1 <= 0:
+            return False, 0
+        need = mpmath.sqrt(r1) / ce.UB_DISK_RADIUS
+        if D.cmp(need, ">=", la) or D.cmp(la * ce.CR, "<=", r2 + need / ce.CR):
+            return False, 0
+        wrem = (la * ce.CR - (r2 + need / ce.CR)) / ce.CR
+        return (True, 1) if (D.cmp(wrem, ">=", 1) or ce.size_bound_ok(D, r1, wrem)) else (False, 0)
+    m["swapped operand r2 -> r1 (criterion 6)"] = (6, uneven_swapped)
+
+    def vstrip_area(D, la, r1, r2, r3):  # the weight requirement without the 0.61
+        w = ce.two_disks_maximize_height(D, r1, r2, mpmath.mpf(1))
+        if w <= 0:
+            return False, 0
+        rem = la * ce.CR - (r1 + r2)
+        wrem = la - w
+        if D.cmp(wrem, ">=", 1) or ce.size_bound_ok(D, r3, wrem):
+            return (True, 1) if D.cmp(rem, ">=", wrem) else (False, 0)
+        return (True, 2) if ce.rsb_can_recurse(D, rem, r3, wrem, mpmath.mpf(1)) else (False, 0)
+    m["dropped factor 0.61 (criterion 9)"] = (9, vstrip_area)
+    return m
+
+
+def test_planted_slips_are_caught(pinned):
+    """Each plausible slip, planted in one exact criterion, disagrees with
+    the oracle on some cell at clear slack: the comparison pins the
+    criteria's formulas, not just their rough behaviour."""
+    pts, v, st, sl, n_plain = pinned
+    o = oracle_bits(pts)
+    base = [ce.r1_strategies, ce.r1_r2_simple, ce.r1_r2_stacked, ce.r123_horizontal, ce.r123_cover_strip,
+            ce.s1_even_split, ce.s23_uneven_split, ce.s23_uneven_split, ce.s4_opposite_corners,
+            ce.s5_vertical_strip]
+    sample = np.arange(0, n_plain, 3)
+    for name, (k, f) in _mutants().items():
+        fns = list(base)
+        fns[k] = f
+        if k == 6:
+            fns[7] = f
+        mv, _, msl = exact(pts[sample], fns)
+        caught = (msl[:, k] >= CLEAR) & (mv[:, k] != o[sample, k])
+        assert caught.any(), f"slip not caught: {name}"
