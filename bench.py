@@ -303,6 +303,29 @@ def extended(p, soa: np.ndarray, mask: np.ndarray) -> dict:
     return out
 
 
+def c3_only() -> int:
+    """The rectangle_size_bound branch-and-bound alone: plain and batched form
+    on the 256x64x64 grid over lambda in [1, 8], depth 24."""
+    import circlecover_amd as cc
+
+    cells = cc.rsb_grid(1.0, 8.0, 256, 64, 64)
+    with cc.Prover(0) as p:
+        p.rsb_bnb_batched(cells, 24)
+        log("batched form")
+        t = time.perf_counter()
+        ub, sb = p.rsb_bnb_batched(cells, 24)
+        t_b = time.perf_counter() - t
+        log("level-synchronous form")
+        t = time.perf_counter()
+        up, sp = p.rsb_bnb(cells, 24)
+        t_p = time.perf_counter() - t
+    print(json.dumps({"workload": "rsb_bnb 256x64x64 lambda [1,8] depth 24", "plain_s": t_p, "batched_s": t_b,
+                      "unproven": int(ub.shape[0]), "identical": bool(np.array_equal(np.sort(up, axis=0),
+                                                                                     np.sort(ub, axis=0))),
+                      "batched": sb.as_dict(), "plain": sp.as_dict()}), flush=True)
+    return 0
+
+
 def log(msg: str) -> None:
     """A progress line on stderr (the JSON result alone goes to stdout)."""
     print(f"bench.py: {msg}", file=sys.stderr, flush=True)
@@ -342,9 +365,14 @@ def main() -> int:
     ap.add_argument("--no-traffic", action="store_true", help="skip the HBM counter runs (traffic = null)")
     ap.add_argument("--no-full-proof", action="store_true", help="skip the end-to-end proofs (workloads.full_proof)")
     ap.add_argument("--traffic-probe", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--c3-only", action="store_true",
+                    help="run only the rectangle_size_bound branch-and-bound (256x64x64 grid, depth 24) in both "
+                         "forms, once each after a warm-up, and print its timings (for profiling)")
     args = ap.parse_args()
     if args.traffic_probe:
         return traffic_probe()
+    if args.c3_only:
+        return c3_only()
     if args.gpus != 1:
         print(f"bench.py: measures one GPU; --gpus {args.gpus} is not supported "
               "(cells are independent: each GPU would take a slice of the boxes or initial cells)", file=sys.stderr)
