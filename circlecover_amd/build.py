@@ -115,11 +115,20 @@ def build_library(hsaco: str) -> None:
             "\t.globl cc_prover_rd_hsaco_end\ncc_prover_rd_hsaco_end:\n"
             '\t.section .note.GNU-stack,"",@progbits\n'
         )
-    host_srcs = [os.path.join(CSRC, "host", f) for f in sorted(os.listdir(os.path.join(CSRC, "host")))
-                 if f.endswith(".cpp")]
+    host_dir = os.path.join(CSRC, "host")
+    host_srcs = [os.path.join(host_dir, f) for f in sorted(os.listdir(host_dir)) if f.endswith(".cpp")]
+    # Host units with device globals (noalloc.hip) are ordinary HIP code:
+    # hipcc embeds their device part and registers it with the runtime.
+    hip_objs = []
+    for f in sorted(os.listdir(host_dir)):
+        if f.endswith(".hip"):
+            obj = os.path.join(BUILD, "host_" + os.path.splitext(f)[0] + ".o")
+            _run(["hipcc", *DEVICE_FLAGS, "-fPIC", f"-I{ROOT}/include", f"-I{host_dir}", "-c", "-o", obj,
+                  os.path.join(host_dir, f)])
+            hip_objs.append(obj)
     tmp = LIB + ".tmp"
-    _run(["g++", *HOST_FLAGS, f"-I{ROOT}/include", f"-I{CSRC}/host", "-shared", "-o", tmp, *host_srcs, blob,
-          f"-L{ROCM}/lib", "-lamdhip64", f"-Wl,-rpath,{ROCM}/lib", "-lpthread"])
+    _run(["g++", *HOST_FLAGS, f"-I{ROOT}/include", f"-I{CSRC}/host", "-shared", "-o", tmp, *host_srcs, *hip_objs,
+          blob, f"-L{ROCM}/lib", "-lamdhip64", f"-Wl,-rpath,{ROCM}/lib", "-lpthread"])
     os.replace(tmp, LIB)
 
 

@@ -124,6 +124,7 @@ int cc_init(int device, cc_ctx** out) {
   if ((e = hipHostMalloc((void**)&ctx->h_counts, 64 * sizeof(unsigned long long), hipHostMallocDefault)) !=
       hipSuccess)
     return bail(e, "hipHostMalloc(counts)");
+  (void)noalloc_setup(ctx);  // a failure disables cc_eval_batch_noalloc_async only
   if ((e = launch(ctx->f_init_konst, dim3(1), dim3(64), 0, ctx->stream, ctx->d_konst)) != hipSuccess)
     return bail(e, "ccrd_init_konst");
   if ((e = hipStreamSynchronize(ctx->stream)) != hipSuccess) return bail(e, "hipStreamSynchronize");

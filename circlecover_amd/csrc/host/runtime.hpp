@@ -32,6 +32,8 @@ struct cc_ctx {
   hipFunction_t f_zero_u64 = nullptr;
   hipFunction_t f_c2_deferred = nullptr;     // the batch's second passes
   hipFunction_t f_c2_r1 = nullptr;
+  void* noalloc = nullptr;                   // this device's static batch scratch (noalloc.hip)
+  std::string noalloc_err;                   // why noalloc is null, if it is
   std::unordered_map<std::string, hipFunction_t> fns;  // kernels looked up by name on first use
   void* d_konst = nullptr;
   unsigned long long* h_counts = nullptr;  // pinned host mirror of level counters
@@ -159,6 +161,11 @@ int c2_batch_launch(cc_ctx* ctx, int prover, int precision, const double* d_soa,
 // be launched; n == 0 is valid and launches nothing).
 int c2_batch_check(cc_ctx* ctx, int prover, int precision, const double* d_soa, size_t n, size_t ld,
                    const uint32_t* d_mask, const uint8_t* d_first);
+// Looks up this device's static batch scratch (once per device, the device
+// being current) and stores it in ctx->noalloc; on failure ctx->noalloc
+// stays null, the reason goes to ctx->noalloc_err, and only the entry point
+// that needs the array reports it.
+int noalloc_setup(cc_ctx* ctx);
 
 // Level buffers shared by the enumeration and branch-and-bound drivers.
 struct Scratch {

@@ -12,6 +12,7 @@ library_path = os.path.join(_PKG, "libcirclecover_amd.so")
 CC_OK, CC_E_HIP, CC_E_OVERFLOW, CC_E_TRAP, CC_E_INTERRUPTED, CC_E_ARG, CC_E_NODEVICE = range(7)
 CC_BATCH_C2 = 0
 CC_PREC_FP64, CC_PREC_FP32_PRE = 0, 1
+NOALLOC_MAX_BOXES = 262144  # CC_NOALLOC_MAX_BOXES
 
 # Criterion names of the ten-criterion 4-D box set, in evaluation order
 # (bit k of a verdict mask).
@@ -59,6 +60,7 @@ lib.cc_copy_async.argtypes = [_vp, _vp, _vp, _sz]
 lib.cc_timer_stop.argtypes = [_vp, _P(ctypes.c_float)]
 lib.cc_eval_batch.argtypes = [_vp, _i, _i, _vp, _sz, _sz, _vp, _vp]
 lib.cc_eval_batch_async.argtypes = [_vp, _i, _i, _vp, _sz, _sz, _vp, _vp]
+lib.cc_eval_batch_noalloc_async.argtypes = [_vp, _i, _i, _vp, _sz, _sz, _vp, _vp]
 lib.cc_gen_c2_boxes.argtypes = [_u64, _sz, _vp]
 lib.cc_gen_c2_boxes.restype = None
 lib.cc_prim_selftest.argtypes = [_vp, _vp, _vp, _sz, _vp]
@@ -261,6 +263,13 @@ class Prover:
                          precision: int = CC_PREC_FP64) -> None:
         self._check(lib.cc_eval_batch_async(self.ctx, CC_BATCH_C2, precision, d_soa.ptr, n, ld, d_mask.ptr,
                                             d_first.ptr))
+
+    def eval_batch_noalloc_async(self, d_soa: DeviceBuffer, n: int, ld: int, d_mask: DeviceBuffer,
+                                 d_first: DeviceBuffer, precision: int = CC_PREC_FP64) -> None:
+        """cc_eval_batch_noalloc_async: eval_batch_async with the library's
+        static per-GPU scratch instead of allocated buffers (n <= NOALLOC_MAX_BOXES)."""
+        self._check(lib.cc_eval_batch_noalloc_async(self.ctx, CC_BATCH_C2, precision, d_soa.ptr, n, ld, d_mask.ptr,
+                                                    d_first.ptr))
 
     def rsb_eval_cells(self, cells: np.ndarray):
         """cells: (n, 14) float64 (la, r1..r6 as lb,ub).  Returns (verdict int32, trap uint8)."""

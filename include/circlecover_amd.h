@@ -102,6 +102,20 @@ int cc_eval_batch(cc_ctx* ctx, int prover, int precision, const double* soa, siz
                   uint32_t* verdict_mask, uint8_t* first_ok);
 int cc_eval_batch_async(cc_ctx* ctx, int prover, int precision, const double* d_soa, size_t n, size_t ld,
                         uint32_t* d_verdict_mask, uint8_t* d_first_ok);
+/* No counterpart in the reference: cc_eval_batch_async for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  It calls no hipMalloc, hipFree
+ * or allocating library routine: the batch's scratch (deferred-box list and
+ * rows, criterion 0's staging rows, a counter) is one static device array of
+ * CC_NOALLOC_SCRATCH_BYTES per GPU, set up by cc_init and held until the
+ * process ends.  n <= CC_NOALLOC_MAX_BOXES (CC_E_ARG beyond).  Same outputs
+ * as cc_eval_batch_async.  Every call on one GPU shares that GPU's array:
+ * calls that overlap on one GPU (two streams or two contexts) race, and the
+ * caller must order them. */
+#define CC_NOALLOC_MAX_BOXES 262144u
+#define CC_NOALLOC_SCRATCH_BYTES 29376768u
+int cc_eval_batch_noalloc_async(cc_ctx* ctx, int prover, int precision, const double* d_soa, size_t n, size_t ld,
+                                uint32_t* d_verdict_mask, uint8_t* d_first_ok);
 
 /* rectangle_size_bound verdict of n 7-D cells (cc_rsb_cell layout), each
  * evaluated exactly as the reference kernel treats a leaf cell
