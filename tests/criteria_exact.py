@@ -1,5 +1,6 @@
-"""The ten criteria of the 4-D box set, decided in exact real arithmetic
-(test support, independent of oracle/ and of the kernels).
+"""The ten criteria of the 4-D box set and the rectangle_size_bound leaf
+steps C1, C2, C4 and C7, decided in exact real arithmetic (test support,
+independent of oracle/ and of the kernels).
 
 Each function restates the mathematics of one reference criterion on a
 point cell (lambda, squared radii r1 >= r2 >= r3): the same placements,
@@ -21,7 +22,10 @@ r1_r2_r3_strategies.cu:28-94; rectangle_size_bound/values.hpp:28-35,
 disk_satisfies_size_bound.cu:25-37, maximize_covered.cu:106-130,
 can_recurse.cu:26-88, 221-235, even_split_recursion.cu:42-57,
 uneven_split_recursion.cu:78-125, r1_r2_opposite_corners_recursion.cu:28-71,
-r1_r2_vertical_strip.cu:28-41.
+r1_r2_vertical_strip.cu:28-41; for the leaf steps variables.cuh:30-52,
+compute_even_split_distance_bound.cu:25-76, even_split_recursion.cu:25-40,
+uneven_split_recursion.cu:25-76, vertical_wall_building_recursion.cu:25-78,
+r1_r2_vertical_strip.cu:43-95.
 """
 from __future__ import annotations
 
@@ -384,4 +388,156 @@ def c2_criteria(la, r1, r2, r3):
         D = Dec()
         v, s = f(D)
         out.append((bool(v), s, D.slack))
+    return out
+
+
+# ----------------------------------- rectangle_size_bound leaf chain (C-steps)
+# Point cells (lambda, r1..r6), r1 >= ... >= r6, with weight left over:
+# lambda * 0.61 - sum(r) > 0.  Intermediate_values (variables.cuh:30-52):
+# R = lambda * 0.61 - sum(r) with its lower bound clamped at 0, and the
+# cell-independent wall-building constants.
+
+CR_SQ = CR * CR
+_WB_DEN = mpmath.sqrt(1 + mpmath.sqrt(1 - mpf("0.25") / CR_SQ))
+WEIGHT_BOUND_WALL_BUILDING = (1 - 1 / _WB_DEN) ** 2 / 2
+PHASE_FACTOR_WALL_BUILDING = 4 * CR_SQ - CR * mpmath.sqrt(16 * CR_SQ - 4)
+
+
+def rsb_R(la, r):
+    return la * CR - sum(r)
+
+
+def bound_required_height(r_sq):
+    """bound_required_height (disk_satisfies_size_bound.cu:30-32): sqrt(r^2) / (3/8)."""
+    return mpmath.sqrt(r_sq) / UB_DISK_RADIUS
+
+
+def even_split_distance_bound(D, la, r):
+    """compute_even_split_distance_bound (compute_even_split_distance_bound.cu:25-76):
+    greedy two groups by weight (r1 | r2 + r3, then r4..r6 to the lighter
+    group by upper bound), their distance bound, then the rest R added to
+    the lighter group if the smallest disk does not already bound it."""
+    g1, g2 = r[0], r[1] + r[2]
+    for i in range(3, 6):
+        if D.cmp(g1, "<=", g2):
+            g1 += r[i]
+        else:
+            g2 += r[i]
+    dist = abs(g1 - g2)
+    R = max(rsb_R(la, r), mpf(0))
+    if D.cmp(dist - r[5], "<=", R):
+        return r[5]
+    if D.cmp(g1, "<=", g2):
+        g1 += R
+    else:
+        g2 += R
+    dist = abs(g1 - g2)
+    return dist if D.cmp(dist, "<=", r[0]) else r[0]
+
+
+def rsb_c1_even_split(D, la, r):
+    """C1, even_split_recursion (even_split_recursion.cu:25-40)."""
+    dist = even_split_distance_bound(D, la, r)
+    width = (mpf("0.5") - dist / (2 * la * CR)) * la
+    if D.cmp(width, ">=", 1):
+        return True, 1
+    return (True, 2) if size_bound_ok(D, r[0], width) else (False, 0)
+
+
+def rsb_c2_uneven_split(D, la, r):
+    """C2, uneven_split_recursion (uneven_split_recursion.cu:25-76): a strip
+    for r1 filled by the largest disks up to its weight, the rest
+    size-bounded."""
+    if r[0] <= 0:
+        return False, 0
+    need = bound_required_height(r[0])
+    if D.cmp(need, ">=", la):
+        return False, 0
+    wneed = need * CR
+    w, at = mpf(0), -1
+    for i in range(6):
+        w += r[i]
+        if D.cmp(w, ">=", wneed):
+            at = i
+            break
+    largest = r[5] if at < 0 or at == 5 else r[at + 1]
+    d1 = wneed + r[5] if at < 0 else w
+    if D.cmp(d1, ">=", la * CR):
+        return False, 0
+    wrem = la - d1 / CR
+    if D.cmp(wrem, ">=", 1):
+        return True, 1
+    return (True, 2) if size_bound_ok(D, largest, wrem) else (False, 0)
+
+
+def rsb_c4_wall_building(D, la, r):
+    """C4, vertical_wall_building_recursion (vertical_wall_building_recursion.cu:
+    25-78), with the reference's "remaining width" lambda + width (Q5)."""
+    R = max(rsb_R(la, r), mpf(0))
+    rem_w = la + mpmath.sqrt(2 * r[5])
+    if r[5] <= 0 or D.cmp(R, "<=", 0):
+        return False, 0
+    if D.cmp(rem_w, "<", 1) and not size_bound_ok(D, r[0], rem_w):
+        return False, 0
+    if D.cmp(r[5], ">", WEIGHT_BOUND_WALL_BUILDING):
+        return False, 0
+    r6_total = r[5] + R
+    below = r[5] * PHASE_FACTOR_WALL_BUILDING
+    used_in_phases = CR * mpmath.sqrt(2 * r[5])
+    for k in range(32):
+        total = r6_total - used_in_phases
+        if D.cmp(r6_total, "<=", used_in_phases):
+            break
+        req = CR * bound_required_height(below)
+        if D.cmp(req, "<=", total):
+            arw = la + (req + below) / CR
+            if D.cmp(arw, ">=", 1) or size_bound_ok(D, r[0], arw):
+                return True, 1
+        used_in_phases += CR * mpmath.sqrt(2 * below)
+        below *= PHASE_FACTOR_WALL_BUILDING
+    return False, 0
+
+
+def rsb_c7_vertical_strip(D, la, r):
+    """C7, r1_r2_vertical_strip (r1_r2_vertical_strip.cu:43-95): r1 and r2 in
+    a full-height strip, then each of r3..r6 recursion or a strip off the
+    longer side of the rest."""
+    w = two_disks_maximize_height(D, r[0], r[1], mpf(1))
+    if w <= 0:
+        return False, 0
+    rem = la * CR - (r[0] + r[1])
+    wrem, hrem = la - w, mpf(1)
+    for md in range(2, 6):
+        if rsb_can_recurse(D, rem, r[md], wrem, hrem):
+            return True, 1
+        ms = min(wrem, hrem)
+        if D.cmp(4 * r[md], "<=", ms * ms):
+            return False, 0
+        cut = mpmath.sqrt(4 * r[md] - ms * ms)
+        if D.cmp(wrem, "<", hrem):
+            hrem -= cut
+            if D.cmp(hrem, "<=", 0):
+                return True, 2
+        else:
+            wrem -= cut
+            if D.cmp(wrem, "<=", 0):
+                return True, 2
+        rem -= r[md]
+    r7 = min(r[5], rsb_R(la, r))
+    return (True, 3) if rsb_can_recurse(D, rem, r7, wrem, hrem) else (False, 0)
+
+
+RSB_STEPS = {6: rsb_c1_even_split, 7: rsb_c2_uneven_split, 9: rsb_c4_wall_building, 12: rsb_c7_vertical_strip}
+
+
+def rsb_steps(la, r):
+    """The pinned leaf-chain steps on the point cell: {step: (verdict, strategy, slack)}
+    with the step numbering of the oracle's per-step masks (C1 = 6, ...)."""
+    la = mpf(la)
+    r = [mpf(x) for x in r]
+    out = {}
+    for k, f in RSB_STEPS.items():
+        D = Dec()
+        v, s = f(D, la, r)
+        out[k] = (bool(v), s, D.slack)
     return out

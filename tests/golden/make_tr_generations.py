@@ -43,6 +43,7 @@ def main():
         cur.extend(oracle.tr_merge_adjacent_r1(leaves))
     out = {"lambda_launches": [FIRST, COUNT], "generations": [enc(cur)]}
     print(f"generation 0: {len(cur)} records ({time.time() - t0:.0f} s)", flush=True)
+    save(out)
     for g in range(1, gens + 1):
         nxt = []
         for s in cur:
@@ -56,10 +57,17 @@ def main():
         cur = nxt
         out["generations"].append(enc(cur))
         print(f"generation {g}: {len(cur)} records ({time.time() - t0:.0f} s)", flush=True)
+        save(out)
         if not cur:
             break
-    with open(os.path.join(ROOT, "tests", "golden", "tr_generations.json"), "w") as f:
+
+
+def save(out):
+    """Writes the generations so far (the file is valid after each one)."""
+    path = os.path.join(ROOT, "tests", "golden", "tr_generations.json")
+    with open(path + ".tmp", "w") as f:
         json.dump(out, f, indent=0)
+    os.replace(path + ".tmp", path)
 
 
 if __name__ == "__main__":

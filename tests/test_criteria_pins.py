@@ -231,3 +231,115 @@ def test_planted_slips_are_caught(pinned):
         mv, _, msl = exact(pts[sample], fns)
         caught = (msl[:, k] >= CLEAR) & (mv[:, k] != o[sample, k])
         assert caught.any(), f"slip not caught: {name}"
+
+
+# --------------------------------------------- rectangle_size_bound leaf steps
+
+RSB_STRATEGIES = {6: {1, 2}, 7: {1, 2}, 9: {1}, 12: {1, 3}}
+
+# Cells whose C7 (r1_r2_vertical_strip) is decided by its last recursion, with
+# all six disks placed (found by a seeded random search; radii beyond 9/64,
+# which the criterion accepts as it accepts any input).
+C7_LAST = [[float.fromhex(x) for x in c] for c in (
+    ['0x1.847f8e46becebp+0', '0x1.8e6e42ad8d75bp-3', '0x1.74bb84dbb90a1p-3', '0x1.7098ab7bf229dp-3',
+     '0x1.4f9999d5000f7p-3', '0x1.d35efbf9248c4p-4', '0x1.cf8da5ce5719fp-5'],
+    ['0x1.36ccbb221e819p+0', '0x1.291dc48deb66bp-3', '0x1.1c8222dd928f3p-3', '0x1.06239e07b7794p-3',
+     '0x1.e27b30807f9cdp-4', '0x1.28b31f5fea714p-4', '0x1.a07a3747ec74ep-5'],
+    ['0x1.ba526feec5591p+0', '0x1.1829b90094e46p-2', '0x1.e98a431737f4ep-3', '0x1.bdb4a6da5c134p-3',
+     '0x1.ee3f08a9da42fp-4', '0x1.97a77f7637698p-4', '0x1.541bffa5d600fp-4'],
+    ['0x1.5a1c24f8e3243p+0', '0x1.597f2759440cap-3', '0x1.52b3d0f58d97dp-3', '0x1.294f0e596b163p-3',
+     '0x1.15f81c194d31ap-3', '0x1.a850591c23438p-4', '0x1.3ec58a11fc791p-5'])]
+
+
+def rsb_point_cells():
+    rng = np.random.default_rng(0xC1C1EC + 1)
+
+    def pts(n, la_rng, rmax):
+        la = rng.uniform(*la_rng, n)
+        r = np.sort(rng.uniform(0, rmax, (n, 6)), axis=1)[:, ::-1]
+        keep = la * 0.61 - r.sum(axis=1) > 1e-3  # weight left over (a non-vacuous cell)
+        return np.column_stack([la, r])[keep]
+
+    return np.concatenate([pts(500, (1.0, 2.5), 9.0 / 64.0), pts(500, (1.0, 2.5), 0.06), pts(500, (1.0, 8.0), 0.3),
+                           pts(300, (1.0, 2.5), 0.03), np.array(C7_LAST)])
+
+
+def test_rsb_leaf_steps_equal_exact_at_clear_slack():
+    """The leaf-chain steps C1 (even split), C2 (uneven split), C4 (vertical
+    wall building) and C7 (r1, r2 strip and greedy strips), each evaluated
+    alone on point cells by the oracle's per-step masks, against their exact
+    statement; planted slips in C2 and C4 are caught."""
+    P = rsb_point_cells()
+    cells = np.zeros((len(P), 14))
+    cells[:, 0] = cells[:, 1] = P[:, 0]
+    for k in range(6):
+        cells[:, 2 + 2 * k] = cells[:, 3 + 2 * k] = P[:, 1 + k]
+    mask = oracle.rsb_masks(cells)
+    fired = {k: set() for k in ce.RSB_STEPS}
+    counts = {k: [0, 0] for k in ce.RSB_STEPS}
+    for i, p in enumerate(P):
+        for k, (v, s, sl) in ce.rsb_steps(p[0], p[1:]).items():
+            if sl < CLEAR:
+                continue
+            o = bool(mask[i] >> k & 1)
+            assert o == v, (k, list(p), o, v, float(sl))
+            counts[k][int(v)] += 1
+            if v:
+                fired[k].add(s)
+    for k in ce.RSB_STEPS:
+        assert min(counts[k]) >= 15, f"step {k}: proves {counts[k][1]}, rejects {counts[k][0]}"
+        assert RSB_STRATEGIES[k] <= fired[k], f"step {k}: strategies {RSB_STRATEGIES[k] - fired[k]} never decide"
+
+    # planted slips: C2 with the largest remaining disk one index early,
+    # C4 with its remaining width lambda - w instead of the reference's lambda + w (Q5)
+    def c2_index(D, la, r):
+        if r[0] <= 0:
+            return False, 0
+        need = ce.bound_required_height(r[0])
+        if D.cmp(need, ">=", la):
+            return False, 0
+        wneed, w, at = need * ce.CR, mpmath.mpf(0), -1
+        for i in range(6):
+            w += r[i]
+            if D.cmp(w, ">=", wneed):
+                at = i
+                break
+        largest = r[5] if at < 0 or at == 5 else r[at]
+        d1 = wneed + r[5] if at < 0 else w
+        if D.cmp(d1, ">=", la * ce.CR):
+            return False, 0
+        wrem = la - d1 / ce.CR
+        return (True, 1) if (D.cmp(wrem, ">=", 1) or ce.size_bound_ok(D, largest, wrem)) else (False, 0)
+
+    def _c4_sub(D, la, r):
+        R = max(ce.rsb_R(la, r), mpmath.mpf(0))
+        rem_w = la - mpmath.sqrt(2 * r[5])
+        if r[5] <= 0 or D.cmp(R, "<=", 0):
+            return False, 0
+        if D.cmp(rem_w, "<", 1) and not ce.size_bound_ok(D, r[0], rem_w):
+            return False, 0
+        if D.cmp(r[5], ">", ce.WEIGHT_BOUND_WALL_BUILDING):
+            return False, 0
+        below, used = r[5] * ce.PHASE_FACTOR_WALL_BUILDING, ce.CR * mpmath.sqrt(2 * r[5])
+        for _ in range(32):
+            total = r[5] + R - used
+            if D.cmp(r[5] + R, "<=", used):
+                break
+            req = ce.CR * ce.bound_required_height(below)
+            if D.cmp(req, "<=", total):
+                arw = la - (req + below) / ce.CR
+                if D.cmp(arw, ">=", 1) or ce.size_bound_ok(D, r[0], arw):
+                    return True, 1
+            used += ce.CR * mpmath.sqrt(2 * below)
+            below *= ce.PHASE_FACTOR_WALL_BUILDING
+        return False, 0
+
+    for k, f in ((7, c2_index), (9, _c4_sub)):
+        caught = False
+        for i, p in enumerate(P):
+            D = ce.Dec()
+            v, _ = f(D, mpmath.mpf(p[0]), [mpmath.mpf(x) for x in p[1:]])
+            if D.slack >= CLEAR and v != bool(mask[i] >> k & 1):
+                caught = True
+                break
+        assert caught, f"planted slip in step {k} not caught"
