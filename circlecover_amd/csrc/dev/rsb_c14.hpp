@@ -657,9 +657,21 @@ static __device__ __noinline__ void r7_four_disk_r1_square(const Konst& K, const
 }
 
 // ------------------------------------------------------------------ C14
-// seven_disk_strategies.cu:1396-1496.  The reference prints a diagnostic
-// line for each failing cell; here the failing cell is simply left unproven.
-static __device__ __noinline__ bool c14_seven_disk(const Konst& K, const cell7& c, const vals7& v, Trap& tp) {
+// seven_disk_strategies.cu:1396-1496, in two parts: the prefix (the widest
+// strip of r1..r6 and the restrictions of the r7 range) and one r7 piece of
+// the loop over its 16 pieces.  The reference prints a diagnostic line for
+// each failing cell; here the failing cell is simply left unproven.
+
+// What the loop needs from the prefix.
+struct c14_state {
+  iv range;        // the restricted r7 range
+  double w_rem_hi; // upper bound of the width r1..r6 leave uncovered
+  double h34;      // two_disks_maximize_height(r3, r4, la.ub)
+};
+
+// seven_disk_strategies.cu:1398-1432: true if the prefix alone proves the
+// cell, else false with `s` set for the loop.
+CCD bool c14_prefix_body(const Konst& K, const cell7& c, const vals7& v, Trap& tp, c14_state& s) {
   double covered = six_max_width(c, tp);
   if (covered >= c.la.hi) return true;
   iv w_rem = c.la - covered;
@@ -670,21 +682,47 @@ static __device__ __noinline__ bool c14_seven_disk(const Konst& K, const cell7& 
   if (empty(range)) return true;
   r7_four_disk_r1_square(K, c, v.R, range);
   if (empty(range)) return true;
-  double h34 = trc::two_disk_height(c.r[2].lo, c.r[3].lo, c.la.hi);
-  for (int k = 0; k < 16; ++k) {
-    iv r7 = subinterval(range, k, 16);
-    if (r7_in_corner(K, v.R, r7, w_rem.hi)) continue;
-    if (three_top_strip(c, v.R, r7)) continue;
-    if (six_222(K, c, v.R, r7)) continue;
-    if (r435_strip(K, c, r7, v.R, tp)) continue;
-    if (diff_height_r12_22(K, c, v.R, r7)) continue;
-    if (h34 > 0.0) {
-      double h5 = five_height_23(c.la.hi, c.r[0].lo, c.r[1].lo, c.r[4].lo, c.r[5].lo, r7.lo, tp);
-      if (add_rd(h34, h5) >= 1.0) continue;
-    }
-    return false;
+  s.range = range;
+  s.w_rem_hi = w_rem.hi;
+  s.h34 = trc::two_disk_height(c.r[2].lo, c.r[3].lo, c.la.hi);
+  return false;
+}
+
+// seven_disk_strategies.cu:1434-1471: one of the strategies covers the
+// cell with r7 in piece k of the 16 pieces of s.range.
+CCD bool c14_piece_body(const Konst& K, const cell7& c, const vals7& v, const c14_state& s, int k, Trap& tp) {
+  iv r7 = subinterval(s.range, k, 16);
+  if (r7_in_corner(K, v.R, r7, s.w_rem_hi)) return true;
+  if (three_top_strip(c, v.R, r7)) return true;
+  if (six_222(K, c, v.R, r7)) return true;
+  if (r435_strip(K, c, r7, v.R, tp)) return true;
+  if (diff_height_r12_22(K, c, v.R, r7)) return true;
+  if (s.h34 > 0.0) {
+    double h5 = five_height_23(c.la.hi, c.r[0].lo, c.r[1].lo, c.r[4].lo, c.r[5].lo, r7.lo, tp);
+    if (add_rd(s.h34, h5) >= 1.0) return true;
   }
+  return false;
+}
+
+// The whole of C14 in one call: the prefix, then the 16 pieces in order
+// until one fails.
+static __device__ __noinline__ bool c14_seven_disk(const Konst& K, const cell7& c, const vals7& v, Trap& tp) {
+  c14_state s;
+  if (c14_prefix_body(K, c, v, tp, s)) return true;
+  for (int k = 0; k < 16; ++k)
+    if (!c14_piece_body(K, c, v, s, k, tp)) return false;
   return true;
+}
+
+// The two parts as calls of their own, for drivers that run the pieces of
+// one cell on several lanes.
+static __device__ __noinline__ bool c14_prefix(const Konst& K, const cell7& c, const vals7& v, Trap& tp,
+                                                c14_state& s) {
+  return c14_prefix_body(K, c, v, tp, s);
+}
+static __device__ __noinline__ bool c14_piece(const Konst& K, const cell7& c, const vals7& v, const c14_state& s,
+                                               int k, Trap& tp) {
+  return c14_piece_body(K, c, v, s, k, tp);
 }
 
 }  // namespace rsbc

@@ -81,6 +81,29 @@ CCD int rsb_verdict_c14(const Konst& K, const cell7& in, Trap& tp) {
   return rsbc::c14_seven_disk(K, c, v, tp) ? 19 : -1;
 }
 
+// C14 in two parts, for the batched driver, which runs the prefix and piece
+// 0 on one lane per cell and each further piece on a lane of its own:
+// rsb_c14_head returns 19 if the prefix proves the cell, -1 if piece 0 is
+// not covered (the cell is not proven), and -3 if pieces 1..15 decide
+// (`s` set).  The cell is proven exactly when every piece is covered; the
+// trap sites the reference reaches are those of the prefix and of the
+// pieces up to the first uncovered one.
+CCD int rsb_c14_head(const Konst& K, const cell7& in, Trap& tp, rsbc::c14_state& s) {
+  const cell7 c = rsbc::tighten(in);
+  vals7 v;
+  v.R = rsbc::remaining_weight(c);
+  v.cover_width = 0.0;
+  if (rsbc::c14_prefix(K, c, v, tp, s)) return 19;
+  return rsbc::c14_piece(K, c, v, s, 0, tp) ? -3 : -1;
+}
+CCD bool rsb_c14_piece(const Konst& K, const cell7& in, const rsbc::c14_state& s, int k, Trap& tp) {
+  const cell7 c = rsbc::tighten(in);
+  vals7 v;
+  v.R = rsbc::remaining_weight(c);
+  v.cover_width = 0.0;
+  return rsbc::c14_piece(K, c, v, s, k, tp);
+}
+
 // The cell-level shortcuts S1..S5 (rectangle_size_bound.cu:162 and :174-176)
 // on the untightened cell, then the leaf part.
 CCD int rsb_cell_verdict(const Konst& K, const cell7& in, Trap& tp) {

@@ -25,10 +25,15 @@ struct FastCtrl {
   unsigned long long trap_cells;
   unsigned long long overflow;
   unsigned long long hist[21];
+  unsigned long long nb2[kMaxDepth + 1];
+  unsigned long long head_b2[kMaxDepth + 1];
+  unsigned long long head_b3[kMaxDepth + 1];
 };
+constexpr size_t kB2Rec = 48;  // struct b2rec of rsb_fast.hip
+static_assert(sizeof(FastCtrl) == 8 * (1 + 7 * (kMaxDepth + 1) + 3 + 21), "mirror of fast_ctrl (rsb_fast.hip)");
 
 struct FastState {
-  DevBuf lvl[2], listb, unproven, ctrl;
+  DevBuf lvl[2], listb, listb2, unproven, ctrl;
   FastCtrl* h_ctrl = nullptr;
   hipGraphExec_t exec = nullptr;
   hipGraph_t graph = nullptr;
@@ -67,24 +72,28 @@ int cu_count(cc_ctx* ctx, int* n) {
 }
 
 struct Fns {
-  hipFunction_t reset, a, b;
+  hipFunction_t reset, a, b1, b2, b3;
   int cus;
 };
 
 int resolve(cc_ctx* ctx, const cc_fast_params& p, Fns* f) {
   const int w = p.waves_per_simd;
   const char* a = w == 1 ? "ccrd_fast_stage_a_w1" : w == 2 ? "ccrd_fast_stage_a_w2" : "ccrd_fast_stage_a_w4";
-  const char* b = w == 1 ? "ccrd_fast_stage_b_w1" : w == 2 ? "ccrd_fast_stage_b_w2" : "ccrd_fast_stage_b_w4";
+  const char* b1 = w == 1 ? "ccrd_fast_stage_b1_w1" : w == 2 ? "ccrd_fast_stage_b1_w2" : "ccrd_fast_stage_b1_w4";
+  const char* b2 = w == 1 ? "ccrd_fast_stage_b2_w1" : w == 2 ? "ccrd_fast_stage_b2_w2" : "ccrd_fast_stage_b2_w4";
   CC_HIP(ctx, kernel(ctx, "ccrd_fast_reset", &f->reset));
   CC_HIP(ctx, kernel(ctx, a, &f->a));
-  CC_HIP(ctx, kernel(ctx, b, &f->b));
+  CC_HIP(ctx, kernel(ctx, b1, &f->b1));
+  CC_HIP(ctx, kernel(ctx, b2, &f->b2));
+  CC_HIP(ctx, kernel(ctx, "ccrd_fast_stage_b3", &f->b3));
   return cu_count(ctx, &f->cus);
 }
 
-// Enqueues the whole run (reset + two stages per depth) on the stream.
+// Enqueues the whole run (reset, then stages A, B1, B2, B3 per depth) on
+// the stream.
 int enqueue(cc_ctx* ctx, const Fns& f, FastState& F, size_t ncells, size_t cap, int max_depth,
             const cc_fast_params& p) {
-  const hipFunction_t fr = f.reset, fa = f.a, fb = f.b;
+  const hipFunction_t fr = f.reset, fa = f.a;
   const unsigned grid = (unsigned)(f.cus * p.blocks_per_cu);
   const unsigned long long n0 = ncells;
   CC_HIP(ctx, launch(fr, dim3(1), dim3(256), 0, ctx->stream, F.ctrl.p, n0));
@@ -96,8 +105,12 @@ int enqueue(cc_ctx* ctx, const Fns& f, FastState& F, size_t ncells, size_t cap, 
     const unsigned cpw_a = (unsigned)p.cpw_a, cpw_b = (unsigned)p.cpw_b;
     CC_HIP(ctx, launch(fa, dim3(grid), dim3(256), 0, ctx->stream, in, cap64, d, cpw_a, ctx->d_konst, F.listb.p,
                        cap64, F.ctrl.p));
-    CC_HIP(ctx, launch(fb, dim3(grid), dim3(256), 0, ctx->stream, in, F.listb.p, cap64, d, at_limit, cpw_b,
-                       ctx->d_konst, next, cap64, F.unproven.p, cap64, F.ctrl.p));
+    CC_HIP(ctx, launch(f.b1, dim3(grid), dim3(256), 0, ctx->stream, in, F.listb.p, cap64, d, at_limit, cpw_b,
+                       ctx->d_konst, next, cap64, F.unproven.p, cap64, F.listb2.p, cap64, F.ctrl.p));
+    CC_HIP(ctx, launch(f.b2, dim3(grid), dim3(256), 0, ctx->stream, in, F.listb2.p, cap64, d, cpw_b, ctx->d_konst,
+                       F.ctrl.p));
+    CC_HIP(ctx, launch(f.b3, dim3(grid), dim3(256), 0, ctx->stream, in, F.listb2.p, cap64, d, at_limit, cpw_b, next,
+                       cap64, F.unproven.p, cap64, F.ctrl.p));
   }
   return CC_OK;
 }
@@ -127,6 +140,7 @@ int cc_rsb_bnb_batched(cc_ctx* ctx, const cc_rsb_cell* cells, size_t ncells, int
   CC_HIP(ctx, F.lvl[0].ensure(cap * kRec));
   CC_HIP(ctx, F.lvl[1].ensure(cap * kRec));
   CC_HIP(ctx, F.listb.ensure(cap * sizeof(uint32_t)));
+  CC_HIP(ctx, F.listb2.ensure(cap * kB2Rec));
   CC_HIP(ctx, F.unproven.ensure(cap * kRec));
   CC_HIP(ctx, F.ctrl.ensure(sizeof(FastCtrl)));
   if (!F.h_ctrl) CC_HIP(ctx, hipHostMalloc((void**)&F.h_ctrl, sizeof(FastCtrl), hipHostMallocDefault));

@@ -6,9 +6,15 @@
 //     workgroups that pull chunks of `cpw` cells per wave from a device
 //     work counter until the level is drained, so a few expensive cells no
 //     longer hold a whole launch's tail;
-//   * two stages per depth: stage A runs every step before C14; the cells it
+//   * stages per depth: stage A runs every step before C14; the cells it
 //     leaves undecided are compacted into a list that stage B runs C14 on,
-//     so the waves of stage B hold only cells that need the expensive step;
+//     so the waves of stage B hold only cells that need the expensive step.
+//     C14 itself is split: B1 runs its prefix and the first of its 16 r7
+//     pieces on one lane per cell (most cells fail there), B2 runs every
+//     further piece of the remaining cells on a lane of its own, B3 combines
+//     the pieces' outcomes per cell.  A cell that C14 proves costs 16
+//     pieces on one lane otherwise, about 15 times what a failing cell
+//     costs, and its wave waits for it;
 //   * every count lives in device memory (struct fast_ctrl), so the host
 //     loop over depths needs no synchronisation and is captured once as a
 //     hipGraph;
@@ -41,7 +47,24 @@ struct fast_ctrl {
   unsigned long long trap_cells;
   unsigned long long overflow;              // a queue append exceeded its capacity
   unsigned long long hist[21];              // verdict v -> [v+1]
+  unsigned long long nb2[kMaxDepth + 1];    // cells of depth d whose C14 pieces 1..15 decide
+  unsigned long long head_b2[kMaxDepth + 1];
+  unsigned long long head_b3[kMaxDepth + 1];
 };
+
+// A cell whose C14 pieces 1..15 decide: its index in the level buffer, the
+// trap sites reached so far (stage A, the prefix, piece 0), the C14 state of
+// the prefix, and the outcomes of the pieces (bit k), filled in by stage B2.
+struct b2rec {
+  uint32_t j;
+  uint32_t trap;
+  uint32_t fail;   // bit k: piece k is not covered
+  uint32_t traps;  // bit k: piece k reached a trap site
+  double range_lo, range_hi, w_rem_hi, h34;
+};
+
+static_assert(sizeof(b2rec) == 48, "b2rec layout (host: kB2Rec)");
+static_assert(sizeof(fast_ctrl) == 8 * (1 + 7 * (kMaxDepth + 1) + 3 + 21), "fast_ctrl layout (host: FastCtrl)");
 
 CCD cell7 load7(const rec112* p) { return load_cell7(p->v); }
 CCD void store7(rec112* out, const cell7& c) {
@@ -137,18 +160,50 @@ __device__ __forceinline__ void stage_a(const rec112* __restrict__ in, uint64_t 
   if (threadIdx.x == 0 && traps) atomicAdd(&ctrl->trap_cells, (unsigned long long)traps);
 }
 
-// Stage B of depth d: C14 on the listed cells.  A proven cell is counted;
-// an unproven cell is split into `next` (depth d+1) or, at the depth limit,
-// appended to `unproven`.
-__device__ __forceinline__ void stage_b(const rec112* __restrict__ in, const uint32_t* __restrict__ listb,
-                                        uint64_t cap_list, int d, int at_limit, unsigned cpw,
-                                        const Konst* __restrict__ kp, rec112* __restrict__ next, uint64_t cap_next,
-                                        rec112* __restrict__ unproven, uint64_t cap_u,
-                                        fast_ctrl* __restrict__ ctrl) {
-  __shared__ unsigned int bins[2];
-  __shared__ unsigned int traps;
-  if (threadIdx.x < 2) bins[threadIdx.x] = 0;
-  if (threadIdx.x == 0) traps = 0;
+// An unproven cell of depth d (wave-uniform call): split into `next`
+// (depth d+1), or, at the depth limit, appended to `unproven`.
+CCD void put_unproven(const cell7& c, bool split, bool keep, int d, rec112* __restrict__ next, uint64_t cap_next,
+                      rec112* __restrict__ unproven, uint64_t cap_u, fast_ctrl* __restrict__ ctrl) {
+  const uint64_t ps = wave_append(split, &ctrl->pairs[d + 1]);
+  if (split) {
+    if (2 * ps + 1 < cap_next) {
+      store7(&next[2 * ps], half7(c, 0));
+      store7(&next[2 * ps + 1], half7(c, 1));
+    } else {
+      atomicOr(&ctrl->overflow, 1ull);
+    }
+  }
+  const uint64_t us = wave_append(keep, &ctrl->unproven);
+  if (keep) {
+    if (us < cap_u) store7(&unproven[us], c);
+    else atomicOr(&ctrl->overflow, 1ull);
+  }
+}
+
+// Per-workgroup tallies of stage B, added to the control block at the end.
+struct b_tally {
+  unsigned int unproven, proven, traps;
+};
+CCD void tally_flush(b_tally* t, fast_ctrl* __restrict__ ctrl) {
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    if (t->unproven) atomicAdd(&ctrl->hist[0], (unsigned long long)t->unproven);
+    if (t->proven) atomicAdd(&ctrl->hist[20], (unsigned long long)t->proven);
+    if (t->traps) atomicAdd(&ctrl->trap_cells, (unsigned long long)t->traps);
+  }
+}
+
+// Stage B1 of depth d: C14's prefix and piece 0 on the listed cells.  A
+// cell the prefix proves is counted; a cell whose piece 0 is not covered is
+// unproven (split or kept); the others go to `listb2` for their pieces
+// 1..15 (stage B2).
+__device__ __forceinline__ void stage_b1(const rec112* __restrict__ in, const uint32_t* __restrict__ listb,
+                                         uint64_t cap_list, int d, int at_limit, unsigned cpw,
+                                         const Konst* __restrict__ kp, rec112* __restrict__ next, uint64_t cap_next,
+                                         rec112* __restrict__ unproven, uint64_t cap_u, b2rec* __restrict__ listb2,
+                                         uint64_t cap_b2, fast_ctrl* __restrict__ ctrl) {
+  __shared__ b_tally t;
+  if (threadIdx.x == 0) t = b_tally{0, 0, 0};
   __syncthreads();
   uint64_t n = ctrl->nb[d];
   if (n > cap_list) n = cap_list;  // entries past the list capacity were never written (overflow is flagged)
@@ -162,38 +217,114 @@ __device__ __forceinline__ void stage_b(const rec112* __restrict__ in, const uin
       const bool live = j < end;
       int v = 0;
       cell7 c;
+      rsbc::c14_state s;
+      uint32_t idx = 0;
+      Trap tp{0};
       if (live) {
         const uint32_t e = listb[j];
-        Trap tp{(int)(e >> 31)};
-        c = load7(&in[e & 0x7fffffffu]);
-        v = rsb_verdict_c14(K, c, tp);
-        atomicAdd(&bins[v < 0 ? 0 : 1], 1u);
-        if (tp.flag) atomicAdd(&traps, 1u);
+        idx = e & 0x7fffffffu;
+        tp.flag = (int)(e >> 31);
+        c = load7(&in[idx]);
+        v = rsb_c14_head(K, c, tp, s);
+        if (v != -3) {
+          atomicAdd(v < 0 ? &t.unproven : &t.proven, 1u);
+          if (tp.flag) atomicAdd(&t.traps, 1u);
+        }
       }
-      const bool split = live && v < 0 && !at_limit;
-      const bool keep = live && v < 0 && at_limit;
-      const uint64_t ps = wave_append(split, &ctrl->pairs[d + 1]);
-      if (split) {
-        if (2 * ps + 1 < cap_next) {
-          store7(&next[2 * ps], half7(c, 0));
-          store7(&next[2 * ps + 1], half7(c, 1));
+      const bool to_b2 = live && v == -3;
+      const uint64_t s2 = wave_append(to_b2, &ctrl->nb2[d]);
+      if (to_b2) {
+        if (s2 < cap_b2) {
+          b2rec r;
+          r.j = idx;
+          r.trap = (uint32_t)tp.flag;
+          r.fail = 0u;
+          r.traps = 0u;
+          r.range_lo = s.range.lo;
+          r.range_hi = s.range.hi;
+          r.w_rem_hi = s.w_rem_hi;
+          r.h34 = s.h34;
+          listb2[s2] = r;
         } else {
           atomicOr(&ctrl->overflow, 1ull);
         }
       }
-      const uint64_t us = wave_append(keep, &ctrl->unproven);
-      if (keep) {
-        if (us < cap_u) store7(&unproven[us], c);
-        else atomicOr(&ctrl->overflow, 1ull);
-      }
+      put_unproven(c, live && v == -1 && !at_limit, live && v == -1 && at_limit, d, next, cap_next, unproven,
+                   cap_u, ctrl);
     }
   }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    if (bins[0]) atomicAdd(&ctrl->hist[0], (unsigned long long)bins[0]);
-    if (bins[1]) atomicAdd(&ctrl->hist[20], (unsigned long long)bins[1]);
-    if (traps) atomicAdd(&ctrl->trap_cells, (unsigned long long)traps);
+  tally_flush(&t, ctrl);
+}
+
+// Stage B2 of depth d: one lane per (listed cell, C14 piece 1..15).  The
+// outcome goes to the cell's b2rec (bit k of fail / traps); every piece is
+// evaluated, and stage B3 keeps the trap sites of the pieces up to the first
+// uncovered one, as the reference's loop reaches them.
+__device__ __forceinline__ void stage_b2(const rec112* __restrict__ in, b2rec* __restrict__ listb2, uint64_t cap_b2,
+                                         int d, unsigned cpw, const Konst* __restrict__ kp,
+                                         fast_ctrl* __restrict__ ctrl) {
+  uint64_t n2 = ctrl->nb2[d];
+  if (n2 > cap_b2) n2 = cap_b2;
+  const uint64_t items = 15 * n2;
+  const Konst K = *kp;
+  for (;;) {
+    const uint64_t base = next_chunk(&ctrl->head_b2[d], cpw);
+    if (base >= items) break;
+    const uint64_t end = base + cpw < items ? base + cpw : items;
+    for (uint64_t j0 = base; j0 < end; j0 += 64) {
+      const uint64_t i = j0 + __lane_id();
+      if (i >= end) continue;
+      const uint64_t r = i / 15;
+      const int k = 1 + (int)(i - 15 * r);
+      const b2rec rec = listb2[r];
+      rsbc::c14_state s;
+      s.range = iv{rec.range_lo, rec.range_hi};
+      s.w_rem_hi = rec.w_rem_hi;
+      s.h34 = rec.h34;
+      Trap tp{0};
+      const bool ok = rsb_c14_piece(K, load7(&in[rec.j]), s, k, tp);
+      if (!ok) atomicOr(&listb2[r].fail, 1u << k);
+      if (tp.flag) atomicOr(&listb2[r].traps, 1u << k);
+    }
   }
+}
+
+// Stage B3 of depth d: the verdict of each cell of `listb2` from its
+// pieces: proven if every piece is covered, else unproven (split or kept);
+// its trap sites are those up to its first uncovered piece.
+__device__ __forceinline__ void stage_b3(const rec112* __restrict__ in, const b2rec* __restrict__ listb2,
+                                         uint64_t cap_b2, int d, int at_limit, unsigned cpw,
+                                         rec112* __restrict__ next, uint64_t cap_next, rec112* __restrict__ unproven,
+                                         uint64_t cap_u, fast_ctrl* __restrict__ ctrl) {
+  __shared__ b_tally t;
+  if (threadIdx.x == 0) t = b_tally{0, 0, 0};
+  __syncthreads();
+  uint64_t n2 = ctrl->nb2[d];
+  if (n2 > cap_b2) n2 = cap_b2;
+  for (;;) {
+    const uint64_t base = next_chunk(&ctrl->head_b3[d], cpw);
+    if (base >= n2) break;
+    const uint64_t end = base + cpw < n2 ? base + cpw : n2;
+    for (uint64_t j0 = base; j0 < end; j0 += 64) {
+      const uint64_t j = j0 + __lane_id();
+      const bool live = j < end;
+      bool failed = false;
+      cell7 c;
+      if (live) {
+        const b2rec rec = listb2[j];
+        failed = rec.fail != 0u;
+        // trap sites of pieces 1..f, f the first uncovered piece (all pieces if none)
+        const uint32_t upto = failed ? (2u << __builtin_ctz(rec.fail)) - 1u : 0xffffffffu;
+        const bool trap = rec.trap != 0u || (rec.traps & upto) != 0u;
+        atomicAdd(failed ? &t.unproven : &t.proven, 1u);
+        if (trap) atomicAdd(&t.traps, 1u);
+        if (failed) c = load7(&in[rec.j]);
+      }
+      put_unproven(c, live && failed && !at_limit, live && failed && at_limit, d, next, cap_next, unproven, cap_u,
+                   ctrl);
+    }
+  }
+  tally_flush(&t, ctrl);
 }
 
 // Kernel variants per occupancy target: waves per SIMD W caps the register
@@ -206,13 +337,30 @@ __device__ __forceinline__ void stage_b(const rec112* __restrict__ in, const uin
     stage_a(in, cap_in, d, cpw, kp, listb, cap, ctrl);                                                         \
   }                                                                                                           \
   extern "C" __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, W)))                 \
-  ccrd_fast_stage_b_w##W(const rec112* __restrict__ in, const uint32_t* __restrict__ listb, uint64_t cap_list, \
-                         int d, int at_limit, unsigned cpw, const Konst* __restrict__ kp,                      \
-                         rec112* __restrict__ next, uint64_t cap_next, rec112* __restrict__ unproven,         \
-                         uint64_t cap_u, fast_ctrl* __restrict__ ctrl) {                                      \
-    stage_b(in, listb, cap_list, d, at_limit, cpw, kp, next, cap_next, unproven, cap_u, ctrl);                \
+  ccrd_fast_stage_b1_w##W(const rec112* __restrict__ in, const uint32_t* __restrict__ listb, uint64_t cap_list,\
+                          int d, int at_limit, unsigned cpw, const Konst* __restrict__ kp,                     \
+                          rec112* __restrict__ next, uint64_t cap_next, rec112* __restrict__ unproven,        \
+                          uint64_t cap_u, b2rec* __restrict__ listb2, uint64_t cap_b2,                        \
+                          fast_ctrl* __restrict__ ctrl) {                                                     \
+    stage_b1(in, listb, cap_list, d, at_limit, cpw, kp, next, cap_next, unproven, cap_u, listb2, cap_b2, ctrl); \
+  }                                                                                                           \
+  extern "C" __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, W)))                 \
+  ccrd_fast_stage_b2_w##W(const rec112* __restrict__ in, b2rec* __restrict__ listb2, uint64_t cap_b2, int d,   \
+                          unsigned cpw, const Konst* __restrict__ kp, fast_ctrl* __restrict__ ctrl) {          \
+    stage_b2(in, listb2, cap_b2, d, cpw, kp, ctrl);                                                            \
   }
 
 CC_STAGE_KERNELS(1)
 CC_STAGE_KERNELS(2)
 CC_STAGE_KERNELS(4)
+
+// Stage B3 is a few loads per cell: one variant.
+extern "C" __global__ void __launch_bounds__(256) ccrd_fast_stage_b3(const rec112* __restrict__ in,
+                                                                      const b2rec* __restrict__ listb2,
+                                                                      uint64_t cap_b2, int d, int at_limit,
+                                                                      unsigned cpw, rec112* __restrict__ next,
+                                                                      uint64_t cap_next,
+                                                                      rec112* __restrict__ unproven, uint64_t cap_u,
+                                                                      fast_ctrl* __restrict__ ctrl) {
+  stage_b3(in, listb2, cap_b2, d, at_limit, cpw, next, cap_next, unproven, cap_u, ctrl);
+}
