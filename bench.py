@@ -214,21 +214,8 @@ def extended(p, soa: np.ndarray, mask: np.ndarray) -> dict:
     import circlecover_amd as cc
 
     out = {}
-    # rectangle_size_bound branch-and-bound: plain vs batched
-    cells = cc.rsb_grid(1.0, 8.0, 256, 64, 64)
-    p.rsb_bnb_batched(cells, 24)  # warm-up, graph capture
-    t = time.perf_counter()
-    ub, sb = p.rsb_bnb_batched(cells, 24)
-    t_b = time.perf_counter() - t
-    t = time.perf_counter()
-    up, sp = p.rsb_bnb(cells, 24)
-    t_p = time.perf_counter() - t
-    same = (sorted(map(tuple, up)) == sorted(map(tuple, ub)) and list(sp.verdict_hist) == list(sb.verdict_hist))
-    out["rsb_bnb"] = {"cells": int(cells.shape[0]), "max_depth": 24, "plain_s": t_p, "batched_s": t_b,
-                      "speedup": t_p / t_b, "unproven": int(ub.shape[0]), "identical": bool(same),
-                      "cells_evaluated": int(sum(sb.cells_per_depth[: sb.depth_reached + 1])),
-                      "cell_evals": int(sb.cell_evals), "cell_evals_per_s_batched": sb.cell_evals / t_b,
-                      "cell_evals_per_s_plain": sp.cell_evals / t_p, "trap_cells": int(sb.trap_cells)}
+    # rectangle_size_bound branch-and-bound: plain, batched, learned+batched
+    out["rsb_bnb"] = c3_forms(p)
     # the CPU leg's sample of the same grid, on the GPU (batched form)
     sample = c3_sample_cells()
     p.rsb_bnb_batched(sample, 24)
@@ -303,26 +290,62 @@ def extended(p, soa: np.ndarray, mask: np.ndarray) -> dict:
     return out
 
 
-def c3_only() -> int:
-    """The rectangle_size_bound branch-and-bound alone: plain and batched form
-    on the 256x64x64 grid over lambda in [1, 8], depth 24."""
+def c3_forms(p) -> dict:
+    """The rectangle_size_bound branch-and-bound on the 256x64x64 grid over
+    lambda in [1, 8], depth 24, in its three forms, each timed once after a
+    warm-up run (wall time of the call, cells uploaded from the host):
+    level-synchronous (plain order), batched, and learned+batched.  The
+    ranker of the learned form is trained on the GPU on the records the
+    prover logs on another grid of the same domain (128x48x48)."""
     import circlecover_amd as cc
 
     cells = cc.rsb_grid(1.0, 8.0, 256, 64, 64)
+    log("C3: ranker training data (level-synchronous run on a 128x48x48 grid, logged)")
+    t = time.perf_counter()
+    _, _, rec = p.rsb_bnb_log(cc.rsb_grid(1.0, 8.0, 128, 48, 48), 24)
+    t_log = time.perf_counter() - t
+    t = time.perf_counter()
+    model, losses = p.rsb_ranker_train(rec, epochs=20, batch=4096, seed=1)
+    t_train = time.perf_counter() - t
+    out = {"cells": int(cells.shape[0]), "max_depth": 24, "train_records": int(len(rec)),
+           "train_depths": int(rec["depth"].max()) + 1, "log_s": t_log, "train_s": t_train,
+           "final_loss": float(losses[-1]), "class_step": list(model.class_step)}
+    runs = {}
+    for name, call in (("batched", lambda: p.rsb_bnb_batched(cells, 24)),
+                       ("learned_batched", lambda: p.rsb_bnb_learned(cells, 24, model)),
+                       ("plain", lambda: p.rsb_bnb(cells, 24))):
+        log(f"C3: {name} form")
+        if name != "plain":
+            call()  # warm-up: graph capture
+        t = time.perf_counter()
+        u, st = call()
+        runs[name] = (time.perf_counter() - t, u, st)
+    t_p, up, sp = runs["plain"]
+    t_b, ub, sb = runs["batched"]
+    t_l, ul, sl = runs["learned_batched"]
+    d = sp.depth_reached
+    same_tree = lambda st: list(st.cells_per_depth[: d + 1]) == list(sp.cells_per_depth[: d + 1])
+    canon = lambda u: u[np.lexsort(u.T[::-1])] if len(u) else u
+    out.update({"plain_s": t_p, "batched_s": t_b, "learned_batched_s": t_l,
+                "speedup_batched": t_p / t_b, "speedup_learned_batched": t_p / t_l,
+                "device_ms": {"plain": sp.device_ms, "batched": sb.device_ms, "learned_batched": sl.device_ms},
+                "unproven": int(ub.shape[0]),
+                "identical": bool(np.array_equal(canon(up), canon(ub)) and np.array_equal(canon(up), canon(ul))
+                                  and same_tree(sb) and same_tree(sl)
+                                  and list(sp.verdict_hist) == list(sb.verdict_hist)),
+                "cells_evaluated": int(sum(sb.cells_per_depth[: sb.depth_reached + 1])),
+                "cell_evals": int(sb.cell_evals), "cell_evals_learned": int(sl.cell_evals),
+                "cell_evals_per_s_batched": sb.cell_evals / t_b, "cell_evals_per_s_plain": sp.cell_evals / t_p,
+                "trap_cells": int(sb.trap_cells), "verdict_hist_learned": list(sl.verdict_hist)})
+    return out
+
+
+def c3_only() -> int:
+    """The rectangle_size_bound branch-and-bound alone in its three forms."""
+    import circlecover_amd as cc
+
     with cc.Prover(0) as p:
-        p.rsb_bnb_batched(cells, 24)
-        log("batched form")
-        t = time.perf_counter()
-        ub, sb = p.rsb_bnb_batched(cells, 24)
-        t_b = time.perf_counter() - t
-        log("level-synchronous form")
-        t = time.perf_counter()
-        up, sp = p.rsb_bnb(cells, 24)
-        t_p = time.perf_counter() - t
-    print(json.dumps({"workload": "rsb_bnb 256x64x64 lambda [1,8] depth 24", "plain_s": t_p, "batched_s": t_b,
-                      "unproven": int(ub.shape[0]), "identical": bool(np.array_equal(np.sort(up, axis=0),
-                                                                                     np.sort(ub, axis=0))),
-                      "batched": sb.as_dict(), "plain": sp.as_dict()}), flush=True)
+        print(json.dumps(c3_forms(p)), flush=True)
     return 0
 
 

@@ -81,6 +81,64 @@ CCD int rsb_verdict_c14(const Konst& K, const cell7& in, Trap& tp) {
   return rsbc::c14_seven_disk(K, c, v, tp) ? 19 : -1;
 }
 
+// Step s (0..18) of rsb_verdict_before_c14 alone: S1..S5 on the untightened
+// cell `in`, the emptiness test and C1..C13 on its tightened form `c` with
+// remaining weight v.R.  C8's 2x2 cover is computed when first needed.
+CCD bool rsb_step_before_c14(const Konst& K, const cell7& in, const cell7& c, vals7& v, bool& have_cover, int s,
+                             Trap& tp) {
+  switch (s) {
+    case 0: return rsbc::s1_even_split(in.la, in.r[0], in.r[1]);
+    case 1: return rsbc::s23_uneven_split(in.la, in.r[0], in.r[1]);
+    case 2: return rsbc::s23_uneven_split(in.la, in.r[0], in.r[2]);
+    case 3: return rsbc::s4_opposite_corners(in.la, in.r[0], in.r[1], in.r[2]);
+    case 4: return rsbc::s5_vertical_strip(K, in.la, in.r[0], in.r[1], in.r[2]);
+    case 5: return rsbc::is_empty(c) || empty(v.R);
+    case 6: return rsbc::c1_even_split(c, v);
+    case 7: return rsbc::c2_uneven_split(c);
+    case 8: return rsbc::c3_multi_disk_strip(c, v, tp);
+    case 9: return rsbc::c4_wall_building(K, c, v);
+    case 10: return rsbc::c5_corner_explicit(c, v, tp);
+    case 11: return rsbc::c6_opposite_corners(c, v, tp);
+    case 12: return rsbc::c7_vertical_strip(K, c, v);
+    case 13: {
+      if (!have_cover) {
+        const rsbc::cover22 cov = rsbc::two_by_two(c, tp);
+        v.cover_width = cov.width;
+        v.cover_c4 = cov.c4;
+        have_cover = true;
+      }
+      return v.cover_width > 0 && (v.cover_width > c.la.hi || rsbc::c8_two_by_two_strip(K, c, v, tp));
+    }
+    case 14: return rsbc::c9_l_shaped(K, c);
+    case 15: return rsbc::c10_corner_wall(K, c, tp);
+    case 16: return rsbc::c11_advanced_strip(c);
+    case 17: return rsbc::c12_five_disk(K, c, v, tp);
+    case 18: return rsbc::c13_six_disk(K, c, v, tp);
+    default: return false;
+  }
+}
+
+// rsb_verdict_before_c14 with step `first` (0..18; -1: none) tried before
+// the others, which follow in the reference's order.  A cell is proven here
+// exactly when rsb_verdict_before_c14 proves it (a step proves only what
+// the cell's verdict proves: an empty cell, vacuous in the reference's
+// order, is proven either way); the step reported is the first that
+// succeeds in this order, and trap sites of steps tried out of order can be
+// reached.
+CCD int rsb_verdict_before_c14_ranked(const Konst& K, const cell7& in, int first, Trap& tp) {
+  const cell7 c = rsbc::tighten(in);
+  vals7 v;
+  v.R = rsbc::remaining_weight(c);
+  v.cover_width = 0.0;
+  bool have_cover = false;
+  for (int k = -1; k < 19; ++k) {
+    const int s = k < 0 ? first : k;
+    if (s < 0 || (k >= 0 && s == first)) continue;
+    if (rsb_step_before_c14(K, in, c, v, have_cover, s, tp)) return s;
+  }
+  return -2;
+}
+
 // C14 in two parts, for the batched driver, which runs the prefix and piece
 // 0 on one lane per cell and each further piece on a lane of its own:
 // rsb_c14_head returns 19 if the prefix proves the cell, -1 if piece 0 is

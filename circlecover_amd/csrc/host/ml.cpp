@@ -1,6 +1,7 @@
 // Learned criterion ranker (ml.hip) and first-success evaluation
 // (c2_first.hip): training loop, gradient access, batched ranking and the
 // ranked-versus-plain evaluation of the ten-criterion box set.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -122,12 +123,26 @@ int rank_dev(cc_ctx* ctx, MlState& M, const void* d_soa, size_t ld, const uint32
   unsigned blocks = (unsigned)((tiles + 3) / 4);
   if (blocks > 512) blocks = 512;
   if (blocks == 0) blocks = 1;
-  CC_HIP(ctx, launch(f, dim3(blocks), dim3(256), 0, ctx->stream, d_soa, n64, ld64, list, n_dev, M.norm.p, M.wb.p,
+  const uint64_t bs = 1;
+  CC_HIP(ctx, launch(f, dim3(blocks), dim3(256), 0, ctx->stream, d_soa, n64, ld64, bs, list, n_dev, M.norm.p, M.wb.p,
                      M.bias.p, M.top1.p, d_logits));
   return CC_OK;
 }
 
 }  // namespace
+
+namespace cchost {
+
+int ml_upload_inference(cc_ctx* ctx, const cc_ranker* model, const void** norm, const void** wb, const void** bias) {
+  MlState& M = ml(ctx);
+  if (int rc = upload_inference(ctx, M, model)) return rc;
+  *norm = M.norm.p;
+  *wb = M.wb.p;
+  *bias = M.bias.p;
+  return CC_OK;
+}
+
+}  // namespace cchost
 
 extern "C" {
 
@@ -478,6 +493,40 @@ int cc_c2_eval_first_f32(cc_ctx* ctx, const double* soa, size_t n, size_t ld, ui
     }
   }
   return CC_OK;
+}
+
+}  // extern "C"
+
+extern "C" {
+
+int cc_rsb_ranker_train(cc_ctx* ctx, const cc_bnb_record* records, size_t n, int epochs, int batch, float lr,
+                        uint64_t seed, cc_rsb_ranker* model, float* epoch_loss) {
+  if (!ctx || !model || (n && !records)) return CC_E_ARG;
+  // the ten steps before C14 (0..18) that prove the most records first
+  uint64_t count[19] = {0};
+  for (size_t i = 0; i < n; ++i)
+    if (records[i].first_step >= 0 && records[i].first_step < 19) ++count[records[i].first_step];
+  int order[19];
+  for (int k = 0; k < 19; ++k) order[k] = k;
+  std::sort(order, order + 19, [&](int a, int b) { return count[a] > count[b] || (count[a] == count[b] && a < b); });
+  int cls_of_step[20];
+  for (int k = 0; k < 20; ++k) cls_of_step[k] = kCrit;  // class 10: the reference's order
+  for (int c = 0; c < kCrit; ++c) {
+    model->class_step[c] = count[order[c]] ? order[c] : -1;
+    if (count[order[c]]) cls_of_step[order[c]] = c;
+  }
+  model->class_step[kCrit] = -1;
+  // features: lambda, r1, r2, r3 of each record (8 SoA rows), labels by class
+  std::vector<double> soa(8 * n);
+  std::vector<uint8_t> label(n);
+  for (size_t i = 0; i < n; ++i) {
+    const cc_rsb_cell& c = records[i].cell;
+    const double v[8] = {c.la.lb, c.la.ub, c.r[0].lb, c.r[0].ub, c.r[1].lb, c.r[1].ub, c.r[2].lb, c.r[2].ub};
+    for (int r = 0; r < 8; ++r) soa[r * n + i] = v[r];
+    const int s = records[i].first_step;
+    label[i] = (uint8_t)(s >= 0 && s < 19 ? cls_of_step[s] : kCrit);
+  }
+  return cc_ranker_train(ctx, soa.data(), n, n, label.data(), epochs, batch, lr, seed, &model->net, epoch_loss);
 }
 
 }  // extern "C"

@@ -209,10 +209,17 @@ int cc_rsb_find_critical_in(cc_ctx* ctx, const cc_rsb_cell* cell, int is_init, c
   return cc_rsb_enumerate(ctx, cell, 1, is_init ? kInitCounts : kSubdivCounts, out, cap, n_out);
 }
 
-int cc_rsb_bnb(cc_ctx* ctx, const cc_rsb_cell* cells, size_t ncells, int max_depth, cc_rsb_cell* out, size_t cap,
-               size_t* n_out, cc_bnb_stats* stats) {
+}  // extern "C"
+
+namespace {
+
+// cc_rsb_bnb, and with log != NULL the records of every evaluated cell
+// (cc_rsb_bnb_log).
+int rsb_bnb_run(cc_ctx* ctx, const cc_rsb_cell* cells, size_t ncells, int max_depth, cc_rsb_cell* out, size_t cap,
+                size_t* n_out, cc_bnb_stats* stats, cc_bnb_record* log, size_t log_cap, size_t* n_log) {
   if (!ctx || !n_out || (ncells && !cells) || (cap && !out) || max_depth < 0 || max_depth >= CC_BNB_MAX_DEPTH)
     return CC_E_ARG;
+  if (n_log && !log && log_cap) return CC_E_ARG;
   CC_HIP(ctx, hipSetDevice(ctx->device));
   clear_interrupt(ctx);
   const auto t0 = std::chrono::steady_clock::now();
@@ -226,6 +233,12 @@ int cc_rsb_bnb(cc_ctx* ctx, const cc_rsb_cell* cells, size_t ncells, int max_dep
   unsigned long long* cnt = static_cast<unsigned long long*>(ctx->d_counts);
   unsigned long long* hist = cnt + 8;  // 21 bins in slots 8..28
   CC_HIP(ctx, hipMemsetAsync(cnt, 0, 32 * sizeof(unsigned long long), ctx->stream));
+  void* d_log = nullptr;
+  const uint64_t log_cap64 = n_log ? log_cap : 0;
+  if (n_log) {
+    CC_HIP(ctx, S.log.ensure(log_cap64 * sizeof(cc_bnb_record) + 8));
+    d_log = S.log.p;
+  }
   CC_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
   uint64_t n = ncells, unproven = 0;
   int d = 0;
@@ -238,7 +251,7 @@ int cc_rsb_bnb(cc_ctx* ctx, const cc_rsb_cell* cells, size_t ncells, int max_dep
     const uint64_t cap_u = S.leaves.cap / kRec;
     CC_HIP(ctx, hipMemsetAsync(cnt, 0, sizeof(unsigned long long), ctx->stream));
     CC_HIP(ctx, launch(f, dim3(blocks_for(n, 128)), dim3(128), 0, ctx->stream, S.cells.p, n, at_limit, ctx->d_konst,
-                       S.next.p, S.leaves.p, cap_u, cnt, hist));
+                       S.next.p, S.leaves.p, cap_u, cnt, hist, d_log, log_cap64, d));
     if ((rc = read_counts(ctx, 3))) return rc;
     n = 2 * ctx->h_counts[0];
     unproven = ctx->h_counts[1];
@@ -257,10 +270,34 @@ int cc_rsb_bnb(cc_ctx* ctx, const cc_rsb_cell* cells, size_t ncells, int max_dep
   }
   st.depth_reached = d - 1;
   st.device_ms = ms;
+  if (n_log) {
+    const uint64_t logged = ctx->h_counts[3];
+    *n_log = logged;
+    const size_t m = logged < log_cap64 ? logged : log_cap64;
+    if (m) {
+      CC_HIP(ctx, hipMemcpyAsync(log, d_log, m * sizeof(cc_bnb_record), hipMemcpyDeviceToHost, ctx->stream));
+      CC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    }
+  }
   if ((rc = copy_out(ctx, S.leaves.p, unproven, out, cap, n_out))) return rc;
   st.wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   if (stats) *stats = st;
   return CC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int cc_rsb_bnb(cc_ctx* ctx, const cc_rsb_cell* cells, size_t ncells, int max_depth, cc_rsb_cell* out, size_t cap,
+               size_t* n_out, cc_bnb_stats* stats) {
+  return rsb_bnb_run(ctx, cells, ncells, max_depth, out, cap, n_out, stats, nullptr, 0, nullptr);
+}
+
+int cc_rsb_bnb_log(cc_ctx* ctx, const cc_rsb_cell* cells, size_t ncells, int max_depth, cc_bnb_record* log,
+                   size_t log_cap, size_t* n_log, cc_rsb_cell* out, size_t cap, size_t* n_out, cc_bnb_stats* stats) {
+  if (!n_log) return CC_E_ARG;
+  return rsb_bnb_run(ctx, cells, ncells, max_depth, out, cap, n_out, stats, log, log_cap, n_log);
 }
 
 }  // extern "C"

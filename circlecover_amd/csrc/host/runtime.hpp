@@ -161,6 +161,10 @@ int c2_batch_launch(cc_ctx* ctx, int prover, int precision, const double* d_soa,
 // be launched; n == 0 is valid and launches nothing).
 int c2_batch_check(cc_ctx* ctx, int prover, int precision, const double* d_soa, size_t n, size_t ld,
                    const uint32_t* d_mask, const uint8_t* d_first);
+// Uploads the inference weights of a ranker (ml.cpp) and returns the device
+// blocks ccml_rank takes: feature normalisation, BF16 weights, FP32 biases.
+int ml_upload_inference(cc_ctx* ctx, const cc_ranker* model, const void** norm, const void** wb, const void** bias);
+
 // Looks up this device's static batch scratch (once per device, the device
 // being current) and stores it in ctx->noalloc; on failure ctx->noalloc
 // stays null, the reason goes to ctx->noalloc_err, and only the entry point
@@ -173,6 +177,7 @@ struct Scratch {
   DevBuf tr_piece, tr_buckets;          // TR leaves' r1 piece (4 B each); per-(cell, piece) hulls
   DevBuf c2_defer;                      // batch kernel scratch (C2Scratch layout)
   DevBuf enum_list, enum_b;             // RSB enumeration: non-empty leaves, leaves for C14 (8 B each)
+  DevBuf log;                           // records of cc_rsb_bnb_log
   void* fast = nullptr;                 // batched branch-and-bound state (rsb_fast.cpp)
   void (*fast_free)(void*) = nullptr;
   void* ml = nullptr;                   // ranker buffers (ml.cpp)

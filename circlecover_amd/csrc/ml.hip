@@ -251,8 +251,10 @@ __device__ __forceinline__ int unit_of_k(int k) {
 // bias: FP32 [b1 | b2 | b3].  top1[i] = argmax logit (lowest index on
 // ties); logits (may be null) receives the 10 logits of each box.  list
 // (may be null): position i ranks box list[i] (outputs at position i); the
-// positions are [0, n), n = *n_dev when n_dev is not null.
-extern "C" __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) ccml_rank(const double* __restrict__ soa, uint64_t n, uint64_t ld,
+// positions are [0, n), n = *n_dev when n_dev is not null.  Value r of box
+// b is soa[r * ld + b * bs]: a SoA of stride ld (bs = 1) or the lambda, r1,
+// r2, r3 bounds at the start of 14-double cell records (ld = 1, bs = 14).
+extern "C" __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) ccml_rank(const double* __restrict__ soa, uint64_t n, uint64_t ld, uint64_t bs,
                                                              const uint32_t* __restrict__ list,
                                                              const uint32_t* __restrict__ n_dev,
                                                              const float* __restrict__ norm,
@@ -314,7 +316,7 @@ extern "C" __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_pe
     if (t < tiles && i < n) {
       const uint64_t box = list ? list[i] : i;
 #pragma unroll
-      for (int e = 0; e < 4; ++e) v[e] = soa[(4 * g + e) * ld + box];
+      for (int e = 0; e < 4; ++e) v[e] = soa[(4 * g + e) * ld + box * bs];
     }
   };
   auto step = [&](double (&v)[4], uint64_t tile) {

@@ -35,14 +35,22 @@ CCD cell7 bisect7(cell7 c, int h) {
 // halves to `children` (pair slot from counts[0]); at the limit it is
 // appended to `unproven` (counts[1], records past cap_unproven dropped).
 // counts[2] counts cells that reached a trap site; hist[v+1] += cells with
-// verdict v.
+// verdict v.  log (may be null): every cell's record (the cell, its verdict,
+// `depth`) is appended (slot from counts[3], records past log_cap dropped).
+struct bnb_log_rec {
+  rsb_rec cell;
+  int32_t verdict;
+  int32_t depth;
+};
 extern "C" __global__ void __launch_bounds__(128) ccrd_rsb_bnb_level(const rsb_rec* __restrict__ in, uint64_t n,
                                                                       int at_limit, const Konst* __restrict__ kp,
                                                                       rsb_rec* __restrict__ children,
                                                                       rsb_rec* __restrict__ unproven,
                                                                       uint64_t cap_unproven,
                                                                       unsigned long long* __restrict__ counts,
-                                                                      unsigned long long* __restrict__ hist) {
+                                                                      unsigned long long* __restrict__ hist,
+                                                                      bnb_log_rec* __restrict__ log, uint64_t log_cap,
+                                                                      int depth) {
   __shared__ unsigned int bins[21];
   if (threadIdx.x < 21) bins[threadIdx.x] = 0;
   __syncthreads();
@@ -67,6 +75,14 @@ extern "C" __global__ void __launch_bounds__(128) ccrd_rsb_bnb_level(const rsb_r
   const uint64_t us = wave_append(keep, &counts[1]);
   if (keep && us < cap_unproven) store_cell7(&unproven[us], c);
   if (live && tp.flag) atomicAdd(&counts[2], 1ull);
+  if (log) {
+    const uint64_t ls = wave_append(live, &counts[3]);
+    if (live && ls < log_cap) {
+      store_cell7(&log[ls].cell, c);
+      log[ls].verdict = verdict;
+      log[ls].depth = depth;
+    }
+  }
   __syncthreads();
   if (threadIdx.x < 21 && bins[threadIdx.x]) atomicAdd(&hist[threadIdx.x], (unsigned long long)bins[threadIdx.x]);
 }

@@ -160,6 +160,67 @@ __device__ __forceinline__ void stage_a(const rec112* __restrict__ in, uint64_t 
   if (threadIdx.x == 0 && traps) atomicAdd(&ctrl->trap_cells, (unsigned long long)traps);
 }
 
+// Stage A with the learned order: positions [0, n) of `order` (the level's
+// cell indices grouped by predicted class, so a wave mostly holds one
+// class); cell j first tries step class_step[cls[j]] (-1: none), then the
+// others in the reference's order.  Same outputs as stage_a otherwise.
+__device__ __forceinline__ void stage_a_ranked(const rec112* __restrict__ in, uint64_t cap_in, int d, unsigned cpw,
+                                               const Konst* __restrict__ kp, uint32_t* __restrict__ listb,
+                                               uint64_t cap, fast_ctrl* __restrict__ ctrl,
+                                               const uint32_t* __restrict__ order, const uint8_t* __restrict__ cls,
+                                               const int* __restrict__ class_step) {
+  __shared__ unsigned int bins[21];
+  __shared__ unsigned int traps;
+  __shared__ int steps[16];
+  if (threadIdx.x < 21) bins[threadIdx.x] = 0;
+  if (threadIdx.x < 16) steps[threadIdx.x] = threadIdx.x < 11 ? class_step[threadIdx.x] : -1;
+  if (threadIdx.x == 0) traps = 0;
+  __syncthreads();
+  uint64_t n = d == 0 ? ctrl->n0 : 2 * ctrl->pairs[d];
+  if (n > cap_in) n = cap_in;
+  const Konst K = *kp;
+  for (;;) {
+    const uint64_t base = next_chunk(&ctrl->head_a[d], cpw);
+    if (base >= n) break;
+    const uint64_t end = base + cpw < n ? base + cpw : n;
+    for (uint64_t p0 = base; p0 < end; p0 += 64) {
+      const uint64_t p = p0 + __lane_id();
+      const bool live = p < end;
+      int v = 0;
+      uint32_t j = 0;
+      Trap tp{0};
+      if (live) {
+        j = order[p];
+        const uint32_t c = cls[j];
+        v = rsb_verdict_before_c14_ranked(K, load7(&in[j]), c < 11 ? steps[c] : -1, tp);
+        if (v >= 0) {
+          atomicAdd(&bins[v + 1], 1u);
+          if (tp.flag) atomicAdd(&traps, 1u);
+        }
+      }
+      const bool to_b = live && v == -2;
+      const uint64_t s = wave_append(to_b, &ctrl->nb[d]);
+      if (to_b) {
+        if (s < cap) listb[s] = j | ((uint32_t)tp.flag << 31);
+        else atomicOr(&ctrl->overflow, 1ull);
+      }
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < 21 && bins[threadIdx.x]) atomicAdd(&ctrl->hist[threadIdx.x], (unsigned long long)bins[threadIdx.x]);
+  if (threadIdx.x == 0 && traps) atomicAdd(&ctrl->trap_cells, (unsigned long long)traps);
+}
+
+// The number of cells at depth d (capped at the level capacity) as the
+// 32-bit count the ranking and grouping kernels read.
+extern "C" __global__ void cc_fast_ncount(const fast_ctrl* __restrict__ ctrl, int d, uint64_t cap,
+                                          uint32_t* __restrict__ n32) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    const uint64_t n = d == 0 ? ctrl->n0 : 2 * ctrl->pairs[d];
+    n32[d] = (uint32_t)(n < cap ? n : cap);
+  }
+}
+
 // An unproven cell of depth d (wave-uniform call): split into `next`
 // (depth d+1), or, at the depth limit, appended to `unproven`.
 CCD void put_unproven(const cell7& c, bool split, bool keep, int d, rec112* __restrict__ next, uint64_t cap_next,
@@ -335,6 +396,13 @@ __device__ __forceinline__ void stage_b3(const rec112* __restrict__ in, const b2
                          const Konst* __restrict__ kp, uint32_t* __restrict__ listb, uint64_t cap,            \
                          fast_ctrl* __restrict__ ctrl) {                                                      \
     stage_a(in, cap_in, d, cpw, kp, listb, cap, ctrl);                                                         \
+  }                                                                                                           \
+  extern "C" __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, W)))                 \
+  ccrd_fast_stage_ar_w##W(const rec112* __restrict__ in, uint64_t cap_in, int d, unsigned cpw,                 \
+                          const Konst* __restrict__ kp, uint32_t* __restrict__ listb, uint64_t cap,           \
+                          fast_ctrl* __restrict__ ctrl, const uint32_t* __restrict__ order,                   \
+                          const uint8_t* __restrict__ cls, const int* __restrict__ class_step) {              \
+    stage_a_ranked(in, cap_in, d, cpw, kp, listb, cap, ctrl, order, cls, class_step);                          \
   }                                                                                                           \
   extern "C" __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, W)))                 \
   ccrd_fast_stage_b1_w##W(const rec112* __restrict__ in, const uint32_t* __restrict__ listb, uint64_t cap_list,\

@@ -97,6 +97,11 @@ class Ranker(ctypes.Structure):
         return np.ctypeslib.as_array(self.params).copy()
 
 
+class RsbRanker(ctypes.Structure):
+    """cc_rsb_ranker: the network and, per class, the step it stands for (-1: the reference's order)."""
+    _fields_ = [("net", Ranker), ("class_step", ctypes.c_int32 * 11)]
+
+
 class FirstStats(ctypes.Structure):
     _fields_ = [("proven", ctypes.c_uint64), ("criterion_evals", ctypes.c_uint64), ("evals_proven", ctypes.c_uint64),
                 ("rank_ms", ctypes.c_double), ("eval_ms", ctypes.c_double), ("second_stage", ctypes.c_uint64)]
@@ -106,6 +111,9 @@ lib.cc_ranker_train.argtypes = [_vp, _vp, _sz, _sz, _vp, _i, _i, ctypes.c_float,
 lib.cc_ranker_gradient.argtypes = [_vp, _vp, _sz, _sz, _vp, _vp, _i, _vp, _vp, _vp]
 lib.cc_ranker_rank.argtypes = [_vp, _vp, _sz, _sz, _vp, _vp, _vp]
 lib.cc_c2_eval_first.argtypes = [_vp, _vp, _sz, _sz, _vp, _vp, _vp, _vp, _vp]
+lib.cc_rsb_bnb_log.argtypes = [_vp, _vp, _sz, _i, _vp, _sz, _P(_sz), _vp, _sz, _P(_sz), _vp]
+lib.cc_rsb_ranker_train.argtypes = [_vp, _vp, _sz, _i, _i, ctypes.c_float, _u64, _vp, _vp]
+lib.cc_rsb_bnb_learned.argtypes = [_vp, _vp, _sz, _i, _vp, _vp, _vp, _sz, _P(_sz), _vp]
 lib.cc_format_interval.argtypes = [_vp, ctypes.c_char_p, _sz, _P(_sz)]
 lib.cc_parse_interval.argtypes = [ctypes.c_char_p, _sz, _vp, _P(_sz)]
 lib.cc_c2_eval_first_ex.argtypes = [_vp, _vp, _sz, _sz, _vp, _i, _vp, _vp, _vp, _vp]
@@ -162,13 +170,22 @@ class BnbStats(ctypes.Structure):
                 "device_ms": self.device_ms, "wall_ms": self.wall_ms}
 
 
-CC_BNB_PLAIN, CC_BNB_BATCHED = 0, 1
+CC_BNB_PLAIN, CC_BNB_BATCHED, CC_BNB_LEARNED = 0, 1, 2
 
 
 class BnbConfig(ctypes.Structure):
     _fields_ = [("cells", ctypes.c_void_p), ("ncells", ctypes.c_size_t), ("la_lb", ctypes.c_double),
                 ("la_ub", ctypes.c_double), ("nla", ctypes.c_int), ("nr1", ctypes.c_int), ("nr2", ctypes.c_int),
-                ("max_depth", ctypes.c_int), ("form", ctypes.c_int), ("params", ctypes.c_void_p)]
+                ("max_depth", ctypes.c_int), ("form", ctypes.c_int), ("params", ctypes.c_void_p),
+                ("ranker", ctypes.c_void_p)]
+
+
+class BnbRecord(ctypes.Structure):
+    """cc_bnb_record: a logged cell (14 doubles), its first proving step (-1: unproven) and depth."""
+    _fields_ = [("cell", ctypes.c_double * 14), ("first_step", ctypes.c_int32), ("depth", ctypes.c_int32)]
+
+
+BNB_RECORD_DTYPE = np.dtype([("cell", np.float64, (14,)), ("first_step", np.int32), ("depth", np.int32)])
 
 
 def gen_c2_boxes(seed: int, n: int) -> np.ndarray:
@@ -341,7 +358,7 @@ class Prover:
         return out, st
 
     def run_bnb(self, max_depth: int, form: int = CC_BNB_BATCHED, cells: np.ndarray | None = None,
-                grid=None, params=None):
+                grid=None, params=None, ranker=None):
         """cc_run_bnb over `cells` ((m, 14)) or grid = (la_lb, la_ub, nla, nr1, nr2).
         Returns (unproven cells in lexicographic order, BnbStats)."""
         cfg = BnbConfig()
@@ -353,6 +370,9 @@ class Prover:
         else:
             cfg.la_lb, cfg.la_ub, cfg.nla, cfg.nr1, cfg.nr2 = grid
         cfg.max_depth, cfg.form = max_depth, form
+        if ranker is not None:
+            keep.append(ranker)
+            cfg.ranker = ctypes.cast(ctypes.byref(ranker), ctypes.c_void_p)
         fp = FastParams(*params) if params is not None else None
         cfg.params = ctypes.cast(ctypes.byref(fp), ctypes.c_void_p) if fp is not None else None
         out, n, st = ctypes.c_void_p(), ctypes.c_size_t(), BnbStats()
@@ -390,6 +410,43 @@ class Prover:
         return (best.cpw_a, best.cpw_b, best.blocks_per_cu, best.use_graph, best.waves_per_simd), ms.value
 
     # ---- learned criterion ranker
+    def rsb_bnb_log(self, cells: np.ndarray, max_depth: int, log_cap: int | None = None):
+        """Level-synchronous branch-and-bound that logs every evaluated cell.
+        Returns (unproven, BnbStats, records) with records a BNB_RECORD_DTYPE array."""
+        cells = np.ascontiguousarray(cells, dtype=np.float64).reshape(-1, 14)
+        st = BnbStats()
+        cap = log_cap if log_cap is not None else 8 * cells.shape[0] + 1024
+        while True:
+            log = np.zeros(cap, dtype=BNB_RECORD_DTYPE)
+            n_log = ctypes.c_size_t()
+            out = self._grow(lambda o, c, n: lib.cc_rsb_bnb_log(self.ctx, _ptr(cells), cells.shape[0], max_depth,
+                                                                _ptr(log), cap, ctypes.byref(n_log), _ptr(o), c,
+                                                                ctypes.byref(n), ctypes.byref(st)), 1 << 16, 14)
+            if n_log.value <= cap or log_cap is not None:
+                return out, st, log[: min(n_log.value, cap)]
+            cap = n_log.value
+
+    def rsb_ranker_train(self, records: np.ndarray, epochs: int = 20, batch: int = 4096, lr: float = 1e-3,
+                         seed: int = 1):
+        """cc_rsb_ranker_train on BNB_RECORD_DTYPE records.  Returns (RsbRanker, epoch losses)."""
+        records = np.ascontiguousarray(records, dtype=BNB_RECORD_DTYPE)
+        model = RsbRanker()
+        losses = np.zeros(max(epochs, 1), dtype=np.float32)
+        self._check(lib.cc_rsb_ranker_train(self.ctx, _ptr(records), records.shape[0], epochs, batch, lr, seed,
+                                            ctypes.byref(model), _ptr(losses)))
+        return model, losses[:epochs]
+
+    def rsb_bnb_learned(self, cells: np.ndarray, max_depth: int, ranker: "RsbRanker", params=None,
+                        cap: int = 1 << 16):
+        """Batched branch-and-bound with the learned step order.  Returns (unproven, BnbStats)."""
+        cells = np.ascontiguousarray(cells, dtype=np.float64).reshape(-1, 14)
+        st = BnbStats()
+        pp = ctypes.byref(FastParams(*params)) if params is not None else None
+        out = self._grow(lambda o, c, n: lib.cc_rsb_bnb_learned(self.ctx, _ptr(cells), cells.shape[0], max_depth,
+                                                                ctypes.byref(ranker), pp, _ptr(o), c, ctypes.byref(n),
+                                                                ctypes.byref(st)), cap, 14)
+        return out, st
+
     def ranker_train(self, soa: np.ndarray, label: np.ndarray, epochs: int = 20, batch: int = 4096,
                      lr: float = 1e-3, seed: int = 1):
         soa = np.ascontiguousarray(soa, dtype=np.float64)

@@ -230,29 +230,6 @@ int cc_rsb_bnb_batched(cc_ctx* ctx, const cc_rsb_cell* cells, size_t ncells, int
 int cc_rsb_autotune(cc_ctx* ctx, const cc_rsb_cell* cells, size_t ncells, int max_depth, cc_fast_params* best,
                     double* best_ms);
 
-/* ---- the branch-and-bound in one call (no reference counterpart; the
- *      reference subdivides with fixed piece counts instead) ---- */
-#define CC_BNB_PLAIN 0   /* level-synchronous: cc_rsb_bnb */
-#define CC_BNB_BATCHED 1 /* persistent stage kernels and one hipGraph: cc_rsb_bnb_batched */
-typedef struct cc_bnb_config {
-  const cc_rsb_cell* cells; /* initial cells (host memory); NULL: the grid below */
-  size_t ncells;
-  double la_lb, la_ub;      /* grid: lambda in [la_lb, la_ub] split into nla pieces, squared radii in */
-  int nla, nr1, nr2;        /* [0, 9/64] with r1 in nr1 and r2 (<= r1) in nr2 pieces, built as cc_rsb_grid */
-  int max_depth;            /* 0 .. CC_BNB_MAX_DEPTH-1 */
-  int form;                 /* CC_BNB_PLAIN or CC_BNB_BATCHED */
-  const cc_fast_params* params; /* CC_BNB_BATCHED launch shape; NULL: autotuned or default */
-} cc_bnb_config;
-/* Runs the widest-dimension branch-and-bound of cc_rsb_bnb in the chosen
- * form and returns the unproven cells in lexicographic (lambda, r1..r6)
- * order, the order of the reference's Variables::operator<
- * (rectangle_size_bound/variables.hpp:82-94), in an array the library
- * allocates: *unproven (NULL when *n == 0), released with cc_free.  Both
- * forms give the same cells and statistics.  Interruptible (cc_interrupt)
- * in the plain form. */
-int cc_run_bnb(cc_ctx* ctx, const cc_bnb_config* cfg, cc_rsb_cell** unproven, size_t* n, cc_bnb_stats* stats);
-/* Releases an array returned by cc_run_bnb. */
-void cc_free(void* p);
 
 /* ---- learned criterion ranker (no reference counterpart) ----
  * An 8 -> 64 -> 64 -> 11 MLP (ReLU; BF16 weights, FP32 accumulate) that
@@ -280,6 +257,78 @@ int cc_ranker_gradient(cc_ctx* ctx, const double* soa, size_t n, size_t ld, cons
  * (n*11, may be NULL) receives the raw outputs. */
 int cc_ranker_rank(cc_ctx* ctx, const double* soa, size_t n, size_t ld, const cc_ranker* model, uint8_t* top1,
                    float* logits);
+
+/* ---- learned step order in the batched branch-and-bound (no reference
+ *      counterpart; the reference tries its steps in one fixed order,
+ *      RSB/rectangle_size_bound.cu:162-229) ---- */
+
+/* One cell of a branch-and-bound run as the prover logs it: the cell, the
+ * first step that proved it (cc_rsb_eval_cells numbering, -1: unproven at
+ * its depth) and its depth. */
+typedef struct cc_bnb_record {
+  cc_rsb_cell cell;
+  int32_t first_step;
+  int32_t depth;
+} cc_bnb_record;
+/* cc_rsb_bnb that also logs every evaluated cell: log[0 .. min(*n_log,
+ * log_cap)) receives the records (unspecified order), *n_log the number of
+ * cells evaluated. */
+int cc_rsb_bnb_log(cc_ctx* ctx, const cc_rsb_cell* cells, size_t ncells, int max_depth, cc_bnb_record* log,
+                   size_t log_cap, size_t* n_log, cc_rsb_cell* out, size_t cap, size_t* n_out, cc_bnb_stats* stats);
+
+/* A ranker of rectangle_size_bound steps: the 8 -> 64 -> 64 -> 11 network of
+ * cc_ranker on a cell's lambda, r1, r2, r3 (centre and log2 width), whose
+ * class c stands for step class_step[c] (0..18, the steps before C14), or
+ * for the reference's order when class_step[c] < 0. */
+typedef struct cc_rsb_ranker {
+  cc_ranker net;
+  int32_t class_step[11];
+} cc_rsb_ranker;
+/* Trains a ranker on n logged records: classes 0..9 are the ten steps
+ * before C14 that prove most of the records first, class 10 everything else
+ * (C14, unproven cells, rarer steps); then cc_ranker_train with these
+ * labels (epochs, batch, lr, seed as there). */
+int cc_rsb_ranker_train(cc_ctx* ctx, const cc_bnb_record* records, size_t n, int epochs, int batch, float lr,
+                        uint64_t seed, cc_rsb_ranker* model, float* epoch_loss);
+/* cc_rsb_bnb_batched with the learned order: at every depth the ranker
+ * classifies the level's cells on the GPU (MFMA inference), the cells are
+ * grouped by predicted step (counting sort), and stage A tries each cell's
+ * predicted step first, then the others in the reference's order.  The
+ * interval criteria stay the only judge: the same cells are proven and
+ * split at every depth, so cells_per_depth and the unproven cells equal
+ * cc_rsb_bnb's; verdict_hist counts the first step that succeeded in the
+ * order tried, and trap_cells may count trap sites of steps tried out of
+ * order. */
+int cc_rsb_bnb_learned(cc_ctx* ctx, const cc_rsb_cell* cells, size_t ncells, int max_depth,
+                       const cc_rsb_ranker* ranker, const cc_fast_params* params, cc_rsb_cell* out, size_t cap,
+                       size_t* n_out, cc_bnb_stats* stats);
+
+/* ---- the branch-and-bound in one call (no reference counterpart; the
+ *      reference subdivides with fixed piece counts instead) ---- */
+#define CC_BNB_PLAIN 0   /* level-synchronous: cc_rsb_bnb */
+#define CC_BNB_BATCHED 1 /* persistent stage kernels and one hipGraph: cc_rsb_bnb_batched */
+#define CC_BNB_LEARNED 2 /* batched with the learned step order: cc_rsb_bnb_learned */
+typedef struct cc_bnb_config {
+  const cc_rsb_cell* cells; /* initial cells (host memory); NULL: the grid below */
+  size_t ncells;
+  double la_lb, la_ub;      /* grid: lambda in [la_lb, la_ub] split into nla pieces, squared radii in */
+  int nla, nr1, nr2;        /* [0, 9/64] with r1 in nr1 and r2 (<= r1) in nr2 pieces, built as cc_rsb_grid */
+  int max_depth;            /* 0 .. CC_BNB_MAX_DEPTH-1 */
+  int form;                 /* CC_BNB_PLAIN, CC_BNB_BATCHED or CC_BNB_LEARNED */
+  const cc_fast_params* params; /* batched forms' launch shape; NULL: autotuned or default */
+  const cc_rsb_ranker* ranker;  /* CC_BNB_LEARNED */
+} cc_bnb_config;
+/* Runs the widest-dimension branch-and-bound of cc_rsb_bnb in the chosen
+ * form and returns the unproven cells in lexicographic (lambda, r1..r6)
+ * order, the order of the reference's Variables::operator<
+ * (rectangle_size_bound/variables.hpp:82-94), in an array the library
+ * allocates: *unproven (NULL when *n == 0), released with cc_free.  All
+ * forms give the same unproven cells and cells per depth (the learned
+ * form's verdict histogram and trap count follow its order, see
+ * cc_rsb_bnb_learned).  Interruptible (cc_interrupt) in the plain form. */
+int cc_run_bnb(cc_ctx* ctx, const cc_bnb_config* cfg, cc_rsb_cell** unproven, size_t* n, cc_bnb_stats* stats);
+/* Releases an array returned by cc_run_bnb. */
+void cc_free(void* p);
 
 typedef struct cc_first_stats {
   uint64_t proven;          /* boxes some criterion proves */
