@@ -23,7 +23,11 @@ configurations measured in the same run (not part of `value`):
   tr_small     tight_rectangle: 64 lambda cells of [1, 1.5], r1..r4 in
                64/32/32/32 pieces, three bisection levels;
   fp32_prefilter  the FP32 pre-filter on the 2^22 boxes (first-success order,
-               per-box and cascade forms, each against its FP64 form);
+               per-box and cascade forms, each against its FP64 form), each
+               criterion's FP32 shadow against its FP64 form (proofs decided
+               in FP32, FP32-accept with FP64-reject count, weight-budget
+               width inflation median/p99) and the ten-criterion batch with
+               the shadows first;
   ranker       the learned criterion ranker trained on 2^20 boxes of another
                seed; first-success evaluation of the 2^22 boxes in plain order,
                plain cascade, ranked and ranked cascade (same proven set);
@@ -253,6 +257,41 @@ def extended(p, soa: np.ndarray, mask: np.ndarray) -> dict:
                                                                   float(np.percentile(r_cr, 99))],
                                  "r1_strategies.remaining_weight": [float(np.median(r_rem)),
                                                                     float(np.percentile(r_rem, 99))]}}
+    # every criterion's FP32 shadow against its FP64 form on the 2^22 boxes,
+    # and the ten-criterion batch with the shadows first (CC_PREC_FP32_PRE)
+    m32, infl = p.f32_report(soa)
+    per = {}
+    for k in range(N_CRIT):
+        p64 = ((mask >> k) & 1).astype(bool)
+        if k == 4:
+            per[str(k)] = {"shadow": None, "fp64_proves": int(p64.sum())}
+            continue
+        p32 = ((m32 >> k) & 1).astype(bool)
+        r = infl[:, k][infl[:, k] >= 0]
+        per[str(k)] = {"fp64_proves": int(p64.sum()), "fp32_proves": int(p32.sum()),
+                       "fraction_of_fp64_proofs_decided_fp32": float(p32.sum() / max(1, p64.sum())),
+                       "fp32_accept_fp64_reject": int((p32 & ~p64).sum()),
+                       "budget_width_inflation_median": float(np.median(r)) if r.size else None,
+                       "budget_width_inflation_p99": float(np.percentile(r, 99)) if r.size else None}
+    del infl
+    n = soa.shape[1]
+    d_soa, d_mask, d_first = p.alloc(soa.nbytes), p.alloc(4 * n), p.alloc(n)
+    d_soa.upload(soa)
+    batch_ms = {}
+    for name, prec in (("fp64", 0), ("fp32_pre", 1)):
+        p.eval_batch_async(d_soa, n, n, d_mask, d_first, precision=prec)
+        p.synchronize()
+        p.timer_start()
+        for _ in range(5):
+            p.eval_batch_async(d_soa, n, n, d_mask, d_first, precision=prec)
+        batch_ms[name] = p.timer_stop() / 5
+    pre_mask = d_mask.download(np.empty(n, dtype=np.uint32))
+    for b in (d_soa, d_mask, d_first):
+        b.free()
+    out["fp32_prefilter"]["per_criterion"] = per
+    out["fp32_prefilter"]["batch"] = {"ms_fp64": batch_ms["fp64"], "ms_fp32_pre": batch_ms["fp32_pre"],
+                                      "identical_masks": bool((pre_mask == mask).all()),
+                                      "fp32_accept_fp64_reject": int((m32 & ~(mask & 0x3FF)).astype(bool).sum())}
     # learned ranker: first-success evaluation of the 2^22 boxes in plain
     # order, plain cascade (criterion 0 on every box, the rest on the boxes
     # it leaves open), ranked, and ranked cascade (the ranker trained on, and
@@ -290,7 +329,7 @@ def extended(p, soa: np.ndarray, mask: np.ndarray) -> dict:
     return out
 
 
-def c3_forms(p) -> dict:
+def c3_forms(p, forms=("plain", "batched", "learned_batched")) -> dict:
     """The rectangle_size_bound branch-and-bound on the 256x64x64 grid over
     lambda in [1, 8], depth 24, in its three forms, each timed once after a
     warm-up run (wall time of the call, cells uploaded from the host):
@@ -300,26 +339,34 @@ def c3_forms(p) -> dict:
     import circlecover_amd as cc
 
     cells = cc.rsb_grid(1.0, 8.0, 256, 64, 64)
-    log("C3: ranker training data (level-synchronous run on a 128x48x48 grid, logged)")
-    t = time.perf_counter()
-    _, _, rec = p.rsb_bnb_log(cc.rsb_grid(1.0, 8.0, 128, 48, 48), 24)
-    t_log = time.perf_counter() - t
-    t = time.perf_counter()
-    model, losses = p.rsb_ranker_train(rec, epochs=20, batch=4096, seed=1)
-    t_train = time.perf_counter() - t
-    out = {"cells": int(cells.shape[0]), "max_depth": 24, "train_records": int(len(rec)),
-           "train_depths": int(rec["depth"].max()) + 1, "log_s": t_log, "train_s": t_train,
-           "final_loss": float(losses[-1]), "class_step": list(model.class_step)}
+    out = {"cells": int(cells.shape[0]), "max_depth": 24}
+    model = None
+    if "learned_batched" in forms:
+        log("C3: ranker training data (level-synchronous run on a 128x48x48 grid, logged)")
+        t = time.perf_counter()
+        _, _, rec = p.rsb_bnb_log(cc.rsb_grid(1.0, 8.0, 128, 48, 48), 24)
+        t_log = time.perf_counter() - t
+        t = time.perf_counter()
+        model, losses = p.rsb_ranker_train(rec, epochs=20, batch=4096, seed=1)
+        out.update({"train_records": int(len(rec)), "train_depths": int(rec["depth"].max()) + 1, "log_s": t_log,
+                    "train_s": time.perf_counter() - t, "final_loss": float(losses[-1]),
+                    "class_step": list(model.class_step)})
     runs = {}
     for name, call in (("batched", lambda: p.rsb_bnb_batched(cells, 24)),
                        ("learned_batched", lambda: p.rsb_bnb_learned(cells, 24, model)),
                        ("plain", lambda: p.rsb_bnb(cells, 24))):
+        if name not in forms:
+            continue
         log(f"C3: {name} form")
         if name != "plain":
             call()  # warm-up: graph capture
         t = time.perf_counter()
         u, st = call()
         runs[name] = (time.perf_counter() - t, u, st)
+    if set(runs) != {"plain", "batched", "learned_batched"}:
+        out.update({f"{k}_s": v[0] for k, v in runs.items()})
+        out.update({f"{k}_device_ms": v[2].device_ms for k, v in runs.items()})
+        return out
     t_p, up, sp = runs["plain"]
     t_b, ub, sb = runs["batched"]
     t_l, ul, sl = runs["learned_batched"]
@@ -340,12 +387,12 @@ def c3_forms(p) -> dict:
     return out
 
 
-def c3_only() -> int:
-    """The rectangle_size_bound branch-and-bound alone in its three forms."""
+def c3_only(forms) -> int:
+    """The rectangle_size_bound branch-and-bound alone, in the given forms."""
     import circlecover_amd as cc
 
     with cc.Prover(0) as p:
-        print(json.dumps(c3_forms(p)), flush=True)
+        print(json.dumps(c3_forms(p, forms)), flush=True)
     return 0
 
 
@@ -391,11 +438,13 @@ def main() -> int:
     ap.add_argument("--c3-only", action="store_true",
                     help="run only the rectangle_size_bound branch-and-bound (256x64x64 grid, depth 24) in both "
                          "forms, once each after a warm-up, and print its timings (for profiling)")
+    ap.add_argument("--c3-forms", default="plain,batched,learned_batched",
+                    help="with --c3-only: the forms to run (comma-separated)")
     args = ap.parse_args()
     if args.traffic_probe:
         return traffic_probe()
     if args.c3_only:
-        return c3_only()
+        return c3_only(tuple(args.c3_forms.split(",")))
     if args.gpus != 1:
         print(f"bench.py: measures one GPU; --gpus {args.gpus} is not supported "
               "(cells are independent: each GPU would take a slice of the boxes or initial cells)", file=sys.stderr)

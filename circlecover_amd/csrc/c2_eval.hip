@@ -36,10 +36,10 @@ static constexpr int kStripCap = 4;
 // their number in r1_cnt[wave]: the second pass reads them densely, with no
 // atomics on either side.
 //
-// kF32Pre: criterion 0 is first tried as the FP32 shadow of its FP64
-// computation (dev/f32.hpp), which proves a box only when the FP64 criterion
-// proves it too; the FP64 criterion runs on the boxes the shadow leaves
-// undecided, so the outputs are those of the FP64 form.
+// kF32Pre: every criterion but 4 is first tried as the FP32 shadow of its
+// FP64 computation (dev/f32.hpp), which proves a box only when the FP64
+// criterion proves it too; the FP64 criterion runs on the boxes the shadow
+// leaves undecided, so the outputs are those of the FP64 form.
 template <bool kF32Pre>
 __device__ __forceinline__ void c2_eval_body(const double* __restrict__ soa, uint64_t n, uint64_t ld,
                                              const Konst* __restrict__ kp, uint32_t* __restrict__ mask,
@@ -62,28 +62,25 @@ __device__ __forceinline__ void c2_eval_body(const double* __restrict__ soa, uin
   const double rem4 = (W - r1 - r2 - r3).hi;
   const double ub_r4 = r3.hi < rem4 ? r3.hi : rem4;
 
+  // pre bit k: the FP32 shadow proved criterion k, whose FP64 form is then
+  // skipped (criterion 1's FP64 form reaches its sqrt only with r1.lb >= 0.49,
+  // where the argument r1 - 1/4 is positive: skipping it skips no trap).
+  uint32_t pre = 0;
+  if constexpr (kF32Pre) pre = f32::c2_shadow_mask(K, la, r1, r2, r3);
   uint32_t m = 0;
   Trap tp{0};
-  bool c0 = false;
-  if constexpr (kF32Pre) {
-    const f32::KonstF KF = f32::konst(K);
-    const f32::siv las = f32::sv(la), r1s = f32::sv(r1);
-    f32::Sh st{true};
-    const f32::siv Ws = f32::crit_ratio(KF, las, st) * las;  // required_weight_for(la)
-    c0 = st.ok && f32::r1_strategies(KF, las, Ws, r1s, f32::shd(r2.hi), st) && st.ok;
-  }
-  if (c0 || trc::r1_strip(K, la, W, r1, r2.hi)) m |= 1u << 0;
-  if (trc::r1_r2_simple(K, la, W, r1, r2, r3.hi, tp)) m |= 1u << 1;
+  if ((pre & 1u) || trc::r1_strip(K, la, W, r1, r2.hi)) m |= 1u << 0;
+  if ((pre & 2u) || trc::r1_r2_simple(K, la, W, r1, r2, r3.hi, tp)) m |= 1u << 1;
   if (tp.flag) m |= 1u << 17;
-  if (trc::r1_r2_stacked(K, la, W, r1, r2, r3.hi)) m |= 1u << 2;
-  if (trc::r123_horizontal(K, la, W, r1, r2, r3, ub_r4)) m |= 1u << 3;
+  if ((pre & 4u) || trc::r1_r2_stacked(K, la, W, r1, r2, r3.hi)) m |= 1u << 2;
+  if ((pre & 8u) || trc::r123_horizontal(K, la, W, r1, r2, r3, ub_r4)) m |= 1u << 3;
   bool defer = false;
   if (trc::r123_cover_strip_capped(K, la, W, r1, r2, r3, ub_r4, kStripCap, defer)) m |= 1u << 4;
-  if (rsbc::s1_even_split(la, r1, r2)) m |= 1u << 5;
-  if (rsbc::s23_uneven_split(la, r1, r2)) m |= 1u << 6;
-  if (rsbc::s23_uneven_split(la, r1, r3)) m |= 1u << 7;
-  if (rsbc::s4_opposite_corners(la, r1, r2, r3)) m |= 1u << 8;
-  if (rsbc::s5_vertical_strip(K, la, r1, r2, r3)) m |= 1u << 9;
+  if ((pre & 32u) || rsbc::s1_even_split(la, r1, r2)) m |= 1u << 5;
+  if ((pre & 64u) || rsbc::s23_uneven_split(la, r1, r2)) m |= 1u << 6;
+  if ((pre & 128u) || rsbc::s23_uneven_split(la, r1, r3)) m |= 1u << 7;
+  if ((pre & 256u) || rsbc::s4_opposite_corners(la, r1, r2, r3)) m |= 1u << 8;
+  if ((pre & 512u) || rsbc::s5_vertical_strip(K, la, r1, r2, r3)) m |= 1u << 9;
   const uint64_t slot = wave_append(live && defer, n_deferred);
   const bool r1_open = live && !(m & 1u);
   const uint64_t ob = __ballot(r1_open);

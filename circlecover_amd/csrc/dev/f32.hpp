@@ -249,5 +249,277 @@ CCD bool r1_strategies(const KonstF& K, siv la, siv W, siv r1, sf ub_r2, Sh& st)
   return false;
 }
 
+// two_disks_maximize_height(r1, r2, 1) (maximize_covered.cu:106-130)
+// shadowed; the FP64 value is 0 on its early exits (returned exactly).
+CCD sf two_disk_height_unit(sf r1, sf r2, Sh& st) {
+  const sf d = ssub(r1, r2);  // encloses both RD(r1 - r2) and RU(r1 - r2)
+  const sf dsq = smul(d, d);
+  sf w1 = sadd(shd(0.5), smul(shd(2.0), d));
+  sf w2 = ssub(shd(0.5), smul(shd(2.0), d));
+  w1 = smul(w1, w1);
+  w2 = smul(w2, w2);
+  if (sle(r1, smul(shd(0.25), w1), st) || !st.ok) return shd(0.0);
+  if (sle(r2, smul(shd(0.25), w2), st) || !st.ok) return shd(0.0);
+  const sf h = ssub(smul(shd(2.0), sadd(r1, r2)), sadd(shd(0.25), smul(shd(4.0), dsq)));
+  if (sle(h, shd(0.0), st) || !st.ok) return shd(0.0);
+  return ssqrt(h, st);
+}
+
+// required_weight_for(IV(w, w), IV(h, h)).ub (critical_ratio.cu:80-82)
+CCD sf weight_for_pt_hi(const KonstF& K, sf w, sf h, Sh& st) {
+  const siv sw{w, w}, sh{h, h};
+  return (crit_ratio(K, sw, sh, st) * sw * sh).hi;
+}
+
+// manually_proved_two_pocket (r1_r2_strategies.cu:29-44).  Its sqrt argument
+// r1 - 1/4 is positive once r1.lb >= 0.49, so the FP64 form never traps here.
+CCD bool two_pocket(siv la, siv r1, siv r2, Sh& st) {
+  const bool low = slt(r2.lo, shd(0.2400000000000000188), st) || slt(r1.lo, shd(0.4900000000000000466), st);
+  if (low || !st.ok) return false;
+  if (!sgt(la.lo, shd(0.0), st)) {  // square(la) = [la.lb^2, la.ub^2] needs la > 0
+    st.ok = false;
+    return false;
+  }
+  const sf r1req = ssub(smul(shd(0.25), sadd(smul(la.hi, la.hi), shd(1.0))), shd(0.00999999999999999847));
+  if (slt(r1.lo, r1req, st) || !st.ok) return false;
+  const sf s1_lo = smul(shd(2.0), ssqrt(ssub(r1.lo, shd(0.25)), st));
+  const sf pw_hi = ssub(la.hi, s1_lo);  // tighten_lb(0) leaves the upper bound
+  if (!sgt(pw_hi, shd(0.0), st)) {      // square(pw).ub = pw.ub^2 needs pw.ub > 0
+    st.ok = false;
+    return false;
+  }
+  const sf r2req = ssub(sadd(smul(pw_hi, pw_hi), shd(0.25)), shd(0.00999999999999999847));
+  return sge(r2.lo, r2req, st) && st.ok;
+}
+
+// r1_r2_strategies_simple (r1_r2_strategies.cu:46-72)
+CCD bool r1_r2_simple(const KonstF& K, siv la, siv W, siv r1, siv r2, sf ub_r3, Sh& st) {
+  const bool pocket = two_pocket(la, r1, r2, st);
+  if (!st.ok) return false;
+  if (pocket) return true;
+  sf width = two_disk_height_unit(r1.lo, r2.lo, st);
+  if (!st.ok || sle(width, shd(0.0), st) || !st.ok) return false;
+  sf w2 = ssub(smul(shd(4.0), r2.lo), shd(1.0));
+  if (sgt(w2, shd(0.0), st)) {
+    w2 = ssqrt(w2, st);
+    const sf w = sadd(ssqrt(ssub(smul(shd(4.0), r1.lo), shd(1.0)), st), w2);
+    if (sgt(w, width, st)) {
+      width = w;
+      if (sgt(w, la.hi, st) && st.ok) return true;
+    }
+  }
+  if (!st.ok) return false;
+  // the reference adds the covered width here (r1_r2_strategies.cu:69)
+  return can_recurse(K, sadd(la.hi, width), shd(1.0), ub_r3, (W - r1 - r2).lo, st) && st.ok;
+}
+
+// r1_r2_strategies_stacked (r1_r2_strategies.cu:74-130)
+CCD bool r1_r2_stacked(const KonstF& K, siv la, siv W, siv r1, siv r2, sf ub_r3, Sh& st) {
+  const sf side = ssqrt(smul(shd(2.0), r1.lo), st);
+  if (!st.ok || sgt(side, shd(1.0), st) || !st.ok) return false;
+  const sf top_h = ssub(shd(1.0), side);
+  sf w2 = ssub(smul(shd(4.0), r2.lo), smul(top_h, top_h));
+  if (slt(w2, shd(0.0), st) || !st.ok) return false;
+  w2 = ssqrt(w2, st);
+  const sf w_top = ssub(la.hi, w2), w_bot = ssub(la.hi, side);
+  const sf top_min = smin(w_top, top_h), bot_min = smin(w_bot, side);
+  const sf R3 = (W - r1 - r2).lo;
+  sf need, cw = w_bot, ch = side;
+  if (disk_fits(ub_r3, top_min, st)) {
+    need = sadd(smul(shd(kRsbRatio), smul(w_top, top_h)), ub_r3);
+  } else if (st.ok && disk_fits(ub_r3, bot_min, st)) {
+    need = sadd(smul(shd(kRsbRatio), smul(w_bot, side)), ub_r3);
+    cw = w_top;
+    ch = top_h;
+  } else {
+    need = sadd(weight_for_pt_hi(K, w_top, top_h, st), ub_r3);
+  }
+  if (!st.ok) return false;
+  const sf left = ssub(R3, need);
+  return sgt(left, shd(0.0), st) && can_recurse(K, cw, ch, ub_r3, left, st) && st.ok;
+}
+
+// r1_r2_r3_strategy_horizontal (r1_r2_r3_strategies.cu:28-46)
+CCD bool r123_horizontal(const KonstF& K, siv la, siv W, siv r1, siv r2, siv r3, sf ub_r4, Sh& st) {
+  sf w3 = ssub(smul(shd(4.0), r3.lo), shd(1.0));
+  if (sle(w3, shd(0.0), st) || !st.ok) return false;
+  w3 = ssqrt(w3, st);
+  const sf w2 = ssqrt(ssub(smul(shd(4.0), r2.lo), shd(1.0)), st);
+  const sf w1 = ssqrt(ssub(smul(shd(4.0), r1.lo), shd(1.0)), st);
+  const sf cov = sadd(w1, sadd(w2, w3));
+  const sf wrem = ssub(la.hi, cov);
+  if (!st.ok) return false;
+  if (sle(wrem, shd(0.0), st) && st.ok) return true;
+  if (!st.ok) return false;
+  return can_recurse(K, wrem, shd(1.0), ub_r4, (W - r1 - r2 - r3).lo, st) && st.ok;
+}
+
+// ------------------------------------------- rectangle_size_bound shortcuts
+
+CCD sf required_height(sf r_sq_ub, Sh& st) {  // disk_satisfies_size_bound.cu:30-32
+  return sdiv(ssqrt(r_sq_ub, st), shd(kDiskRadiusMax), st);
+}
+
+// can_recurse (rectangle_size_bound/can_recurse.cu:26-88)
+static __device__ __noinline__ bool rsb_can_recurse(const KonstF& K, sf rem, sf largest, sf w, sf h, Sh& st) {
+  if (sle(rem, shd(0.0), st)) {
+    if (slt(rem, shd(0.0), st)) return false;
+    const bool a = sle(w, shd(0.0), st), b = sle(h, shd(0.0), st);
+    return (a || b) && st.ok;
+  }
+  if (!st.ok) return false;
+  const bool w0 = sle(w, shd(0.0), st), h0 = sle(h, shd(0.0), st);
+  if (!st.ok) return false;
+  if (w0 || h0) return true;
+  const sf smn = smin(w, h), smx = smax(w, h);  // enclose the FP64 min and max of the sides
+  const sf area = sdiv(rem, shd(kRsbRatio), st);
+  sf side = sdiv(area, smx, st);
+  if (slt(side, smn, st) || !st.ok) return false;
+  if (sgt(side, smx, st)) side = ssqrt(area, st);
+  if (!st.ok) return false;
+  if (sge(smul(shd(kDiskWeightMax), smul(side, side)), largest, st) && st.ok) return true;
+  if (!st.ok) return false;
+  const sf skew = sdiv(smx, smn, st);
+  if (sle(skew, shd(kRsbToleratedLambda), st)) {
+    const sf cr = crit_ratio(K, siv{skew, skew}, st).hi;
+    return sle(smul(cr, smul(w, h)), rem, st) && st.ok;
+  }
+  if (!st.ok) return false;
+  sf sig = sdiv(largest, smul(smn, smn), st);
+  sig = smul(sig, sig);
+  const sf cr = smax(smul(shd(0.5), ssqrt(sadd(ssqrt(sadd(sig, shd(1.0)), st), shd(1.0)), st)), shd(kWorstRatio));
+  if (sle(smul(cr, smul(w, h)), rem, st) && st.ok) return true;
+  if (!st.ok) return false;
+  const sf ww = sadd(smul(shd(0.25), smul(smx, smx)), smul(shd(0.5), smul(smn, smn)));
+  return sle(ww, rem, st) && st.ok;
+}
+
+// S1 shortcut_even_split_recursion (even_split_recursion.cu:42-57)
+CCD bool s1_even_split(siv la, siv r1, siv r2, Sh& st) {
+  const sf big = smul(shd(0.5), la.lo);
+  if (slt(big, shd(1.0), st)) {
+    if (!disk_fits(r1.hi, big, st) || !st.ok) return false;
+  }
+  if (!st.ok) return false;
+  const sf small = ssub(big, sdiv(r2.hi, shd(kRsbRatio), st));
+  if (sge(small, shd(1.0), st) && st.ok) return true;
+  if (!st.ok) return false;
+  return disk_fits(r2.hi, small, st) && st.ok;
+}
+
+// S2/S3 shortcut_uneven_split_recursion (uneven_split_recursion.cu:78-125)
+CCD bool s23_uneven_split(siv la, siv r1, siv rk, Sh& st) {
+  if (sle(r1.lo, shd(0.0), st) || !st.ok) return false;
+  const sf need = required_height(r1.hi, st);
+  if (sge(need, la.lo, st) || !st.ok) return false;
+  const sf g1 = sadd(rk.hi, sdiv(need, shd(kRsbRatio), st));
+  const sf rem = ssub(smul(la.lo, shd(kRsbRatio)), g1);
+  if (sle(rem, shd(0.0), st) || !st.ok) return false;
+  const sf rem_w = sdiv(rem, shd(kRsbRatio), st);
+  if (sge(rem_w, shd(1.0), st) && st.ok) return true;
+  if (!st.ok) return false;
+  return disk_fits(rk.hi, rem_w, st) && st.ok;
+}
+
+// bound_worst_case_ratio (can_recurse.cu:221-235) for positive w, h
+CCD sf worst_case_ratio(siv w, siv h, Sh& st) {
+  const sf a = sdiv(w.hi, h.lo, st), b = sdiv(h.hi, w.lo, st);
+  const sf skew = smax(a, b);
+  const sf r = sadd(smul(shd(0.25), skew), smul(shd(0.5), sdiv(shd(1.0), skew, st)));
+  return smax(r, shd(kWorstRatio));
+}
+
+// S4 shortcut_r1_r2_opposite_corners_recursion (r1_r2_opposite_corners_recursion.cu:28-70)
+CCD bool s4_opposite_corners(siv la, siv r1, siv r2, siv r3, Sh& st) {
+  const siv w1{ssqrt(smul(shd(2.0), r1.lo), st), ssqrt(smul(shd(2.0), r1.hi), st)};
+  const siv w2{ssqrt(smul(shd(2.0), r2.lo), st), ssqrt(smul(shd(2.0), r2.hi), st)};
+  const siv both = w1 + w2;
+  if (!st.ok || !slt(both.hi, shd(1.0), st) || !st.ok) return false;
+  const siv wA = la - w2, wB = la - w1, hA = spt(1.0) - w1;
+  const sf minAB = smin(smin(hA.lo, wA.lo), smin(w1.lo, wB.lo));
+  if (!disk_fits(r3.hi, minAB, st) || !st.ok) return false;
+  const siv hC = spt(1.0) - both;
+  const sf minC = smin(hC.lo, w2.lo);
+  const sf extra12 = smul(ssub(smul(shd(2.0), shd(kRsbRatio)), shd(1.0)), sadd(r1.lo, r2.lo));
+  const sf two_r3 = smul(shd(2.0), r3.hi);
+  if (disk_fits(r3.hi, minC, st)) return !sgt(two_r3, extra12, st) && st.ok;
+  if (!st.ok) return false;
+  if (sle(w2.lo, shd(0.0), st) || !st.ok || sle(hC.lo, shd(0.0), st) || !st.ok) return false;  // ratio infinite
+  const sf before = ssub(extra12, two_r3);
+  const sf extra = smul(ssub(worst_case_ratio(w2, hC, st), shd(kRsbRatio)), smul(w2.hi, hC.hi));
+  return !slt(before, extra, st) && st.ok;
+}
+
+// S5 shortcut_r1_r2_vertical_strip (r1_r2_vertical_strip.cu:28-41)
+CCD bool s5_vertical_strip(const KonstF& K, siv la, siv r1, siv r2, siv r3, Sh& st) {
+  const sf w = two_disk_height_unit(r1.lo, r2.lo, st);
+  if (!st.ok || sle(w, shd(0.0), st) || !st.ok) return false;
+  const sf R3 = ssub(smul(la.lo, shd(kRsbRatio)), sadd(r1.hi, r2.hi));
+  const siv wrem{ssub(la.lo, w), ssub(la.hi, w)};
+  bool strip = sge(wrem.lo, shd(1.0), st);
+  if (!st.ok) return false;
+  if (!strip) {
+    strip = disk_fits(r3.hi, wrem.lo, st);
+    if (!st.ok) return false;
+  }
+  if (strip) return sge(R3, smul(shd(kRsbRatio), wrem.hi), st) && st.ok;
+  return rsb_can_recurse(K, R3, r3.hi, wrem.hi, shd(1.0), st);
+}
+
+// ------------------------------------------------- the ten-criterion set
+
+// Inputs of the ten criteria in shadow form: the box, required_weight_for(la)
+// and ub_r4 = min(r3.ub, (W - r1 - r2 - r3).ub), all computed in FP32.
+struct C2In {
+  KonstF K;
+  siv la, r1, r2, r3, W;
+  sf ub_r4;
+  bool ok;  // false when required_weight_for(la) could not be shadowed
+};
+CCD C2In c2_inputs(const Konst& K, iv la, iv r1, iv r2, iv r3) {
+  C2In c;
+  c.K = konst(K);
+  c.la = sv(la);
+  c.r1 = sv(r1);
+  c.r2 = sv(r2);
+  c.r3 = sv(r3);
+  Sh st{true};
+  c.W = crit_ratio(c.K, c.la, st) * c.la;
+  c.ub_r4 = smin(shd(r3.hi), (c.W - c.r1 - c.r2 - c.r3).hi);
+  c.ok = st.ok;
+  return c;
+}
+
+// Criterion k (0..9 except 4, whose width bisection stays in FP64) in shadow
+// form: true only when the FP64 criterion certainly proves the box.
+CCD bool c2_shadow(int k, const C2In& c) {
+  if (!c.ok) return false;
+  Sh st{true};
+  bool r = false;
+  switch (k) {
+    case 0: r = r1_strategies(c.K, c.la, c.W, c.r1, c.r2.hi, st); break;
+    case 1: r = r1_r2_simple(c.K, c.la, c.W, c.r1, c.r2, c.r3.hi, st); break;
+    case 2: r = r1_r2_stacked(c.K, c.la, c.W, c.r1, c.r2, c.r3.hi, st); break;
+    case 3: r = r123_horizontal(c.K, c.la, c.W, c.r1, c.r2, c.r3, c.ub_r4, st); break;
+    case 5: r = s1_even_split(c.la, c.r1, c.r2, st); break;
+    case 6: r = s23_uneven_split(c.la, c.r1, c.r2, st); break;
+    case 7: r = s23_uneven_split(c.la, c.r1, c.r3, st); break;
+    case 8: r = s4_opposite_corners(c.la, c.r1, c.r2, c.r3, st); break;
+    case 9: r = s5_vertical_strip(c.K, c.la, c.r1, c.r2, c.r3, st); break;
+    default: break;
+  }
+  return r && st.ok;
+}
+
+// Bit k set: the shadow of criterion k proves the box (bit 4 never set).
+CCD uint32_t c2_shadow_mask(const Konst& K, iv la, iv r1, iv r2, iv r3) {
+  const C2In c = c2_inputs(K, la, r1, r2, r3);
+  uint32_t m = 0;
+#pragma unroll 1
+  for (int k = 0; k < 10; ++k)
+    if (k != 4 && c2_shadow(k, c)) m |= 1u << k;
+  return m;
+}
+
 }  // namespace f32
 }  // namespace ccd

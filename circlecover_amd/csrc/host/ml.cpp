@@ -495,6 +495,29 @@ int cc_c2_eval_first_f32(cc_ctx* ctx, const double* soa, size_t n, size_t ld, ui
   return CC_OK;
 }
 
+int cc_c2_f32_report(cc_ctx* ctx, const double* soa, size_t n, size_t ld, uint32_t* f32_mask, float* inflation) {
+  if (!ctx || !soa || !f32_mask || ld < n) return CC_E_ARG;
+  CC_HIP(ctx, hipSetDevice(ctx->device));
+  MlState& M = ml(ctx);
+  int rc = upload_soa(ctx, M.soa, soa, n, ld);
+  if (rc) return rc;
+  CC_HIP(ctx, M.grad.ensure((n + 1) * 10 * sizeof(float) + (n + 1) * sizeof(uint32_t)));
+  uint32_t* d_mask = static_cast<uint32_t*>(M.grad.p);
+  float* d_infl = inflation ? reinterpret_cast<float*>(d_mask + n + 1) : nullptr;
+  hipFunction_t f;
+  CC_HIP(ctx, kernel(ctx, "ccrd_c2_f32_report", &f));
+  if (n) {
+    const uint64_t n64 = n, ld_dev = n;
+    CC_HIP(ctx, launch(f, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, M.soa.p, ld_dev, n64,
+                       ctx->d_konst, d_mask, d_infl));
+  }
+  CC_HIP(ctx, hipMemcpyAsync(f32_mask, d_mask, n * sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+  if (inflation)
+    CC_HIP(ctx, hipMemcpyAsync(inflation, d_infl, 10 * n * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+  CC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return CC_OK;
+}
+
 }  // extern "C"
 
 extern "C" {

@@ -119,6 +119,7 @@ lib.cc_parse_interval.argtypes = [ctypes.c_char_p, _sz, _vp, _P(_sz)]
 lib.cc_c2_eval_first_ex.argtypes = [_vp, _vp, _sz, _sz, _vp, _i, _vp, _vp, _vp, _vp]
 lib.cc_c2_eval_first_f32.argtypes = [_vp, _vp, _sz, _sz, _vp, _vp, _vp, _vp, _vp, _vp]
 lib.cc_c2_eval_first_f32_ex.argtypes = [_vp, _vp, _sz, _sz, _i, _vp, _vp, _vp, _vp, _vp, _vp]
+lib.cc_c2_f32_report.argtypes = [_vp, _vp, _sz, _sz, _vp, _vp]
 lib.cc_tr_find_critical_in.argtypes = [_vp, _vp, _i, _vp, _sz, _P(_sz)]
 lib.cc_tr_enumerate.argtypes = [_vp, _vp, _sz, _vp, _vp, _sz, _P(_sz)]
 lib.cc_tr_bnb.argtypes = [_vp, _vp, _sz, _vp, _i, _vp, _sz, _P(_sz), _vp]
@@ -504,6 +505,18 @@ class Prover:
                                                 _ptr(evals), _ptr(trap), _ptr(dec), _ptr(w) if with_widths else None,
                                                 ctypes.byref(st)))
         return first, evals, trap, dec, w, st
+
+    def f32_report(self, soa: np.ndarray, with_inflation: bool = True):
+        """Per-criterion FP32 shadow report: (mask (n,) uint32 with bit k =
+        the shadow of criterion k proves the box, inflation (n, 10) float32
+        FP32/FP64 weight-budget width ratios, -1 where undefined, or None)."""
+        soa = np.ascontiguousarray(soa, dtype=np.float64)
+        n = soa.shape[1]
+        mask = np.empty(n, dtype=np.uint32)
+        infl = np.empty((n, 10), dtype=np.float32) if with_inflation else None
+        self._check(lib.cc_c2_f32_report(self.ctx, _ptr(soa), n, n, _ptr(mask),
+                                         _ptr(infl) if with_inflation else None))
+        return mask, infl
 
     def rsb_eval_masks(self, cells: np.ndarray) -> np.ndarray:
         cells = np.ascontiguousarray(cells, dtype=np.float64).reshape(-1, 14)

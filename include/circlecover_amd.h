@@ -77,10 +77,10 @@ int cc_timer_stop(cc_ctx* ctx, float* ms);
  *      TR/tight_rectangle.cu:169-262, RSB/rectangle_size_bound.cu:161-236) ---- */
 #define CC_BATCH_C2 0      /* the ten-criterion 4-D box set (see cc_eval_batch) */
 #define CC_PREC_FP64 0     /* outward-rounded FP64 intervals */
-#define CC_PREC_FP32_PRE 1 /* FP32 pre-filter: criterion 0 is first tried as the FP32 shadow of its
-                              FP64 computation, which proves a box only when the FP64 criterion
-                              does; boxes it leaves undecided, and criteria 1..9, run in FP64.
-                              Outputs equal CC_PREC_FP64's. */
+#define CC_PREC_FP32_PRE 1 /* FP32 pre-filter: every criterion but 4 is first tried as the FP32
+                              shadow of its FP64 computation, which proves a box only when the
+                              FP64 criterion does; the FP64 criterion runs where the shadow
+                              leaves the box undecided.  Outputs equal CC_PREC_FP64's. */
 
 /* Evaluates criterion set `prover` on n boxes stored as 8 SoA FP64 arrays of
  * stride ld >= n: (la.lb, la.ub, r1.lb, r1.ub, r2.lb, r2.ub, r3.lb, r3.ub).
@@ -457,6 +457,18 @@ int cc_c2_eval_first_f32(cc_ctx* ctx, const double* soa, size_t n, size_t ld, ui
 int cc_c2_eval_first_f32_ex(cc_ctx* ctx, const double* soa, size_t n, size_t ld, int cascade, uint8_t* first,
                             uint8_t* evals, uint8_t* trap, uint8_t* f32_decided, float* widths,
                             cc_first_stats* stats);
+
+/* Per-criterion report of the FP32 shadows used by CC_PREC_FP32_PRE (no
+ * reference counterpart: the reference computes in FP64 only).  f32_mask[i]
+ * bit k = the FP32 shadow of criterion k (the cc_eval_batch numbering)
+ * proves box i; a set bit implies bit k of cc_eval_batch's verdict mask.
+ * Criterion 4 has no shadow (bit 4 never set).  inflation (10*n floats, may
+ * be NULL): per box and criterion, the width of the criterion's weight
+ * budget in FP32 over its width in FP64 (-1 where undefined); the budget is
+ * the weight a criterion starts from minus the disks it places:
+ *   0: W-r1  1,2: W-r1-r2  3: W-r1-r2-r3  (W = required_weight_for(la))
+ *   5,6,9: R-r1-r2  7: R-r1-r3  8: R-r1-r2-r3  (R = 0.61*la). */
+int cc_c2_f32_report(cc_ctx* ctx, const double* soa, size_t n, size_t ld, uint32_t* f32_mask, float* inflation);
 
 /* get_subinterval (Prover/subintervals.hpp:32-43) on the host: the n pieces
  * of [lb, ub] as 2n doubles (lb + diff*i with one rounding, the last piece

@@ -49,3 +49,45 @@ def test_gpu_f32_cascade_matches_per_box_form(prover, n):
     for x, y in zip(a[:5], b[:5]):
         assert (x == y).all()
     assert a[5].proven == b[5].proven and a[5].evals_proven == b[5].evals_proven
+
+
+def _f32_sets():
+    from test_c2_batch import cover_strip_boxes, edge_boxes, small_radius_boxes
+    return {"seeded": gen_c2_boxes(SEED + 3, 1 << 16), "small_radii": small_radius_boxes(1 << 15),
+            "cover_strip": cover_strip_boxes(1 << 14), "edge": edge_boxes()}
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["seeded", "small_radii", "cover_strip", "edge"])
+def test_gpu_f32_shadows_imply_fp64(prover, name):
+    """Each criterion's FP32 shadow proves only boxes its FP64 form proves
+    (per criterion, FP32-accept and FP64-reject never happen), and the
+    pre-filtered batch (every shadow first, FP64 where undecided) returns the
+    FP64 batch's masks and first criteria exactly."""
+    soa = _f32_sets()[name]
+    m64, f64 = prover.eval_batch(soa)
+    m32, infl = prover.f32_report(soa)
+    bad = m32 & ~(m64 & 0x3FF)
+    assert not bad.any(), f"{int((bad != 0).sum())} boxes: FP32 proves a criterion FP64 rejects"
+    assert not (m32 & (1 << 4)).any()  # criterion 4 has no shadow
+    mp, fp = prover.eval_batch(soa, precision=1)
+    assert (mp == m64).all() and (fp == f64).all()
+    assert infl.shape == (soa.shape[1], 10) and (infl[:, 4] == -1).all()
+
+
+@pytest.mark.gpu
+def test_gpu_f32_shadows_decide_most_fp64_proofs(prover):
+    """The shadows are not vacuous: on the seeded and size-bounded boxes each
+    criterion's shadow proves most of the boxes its FP64 form proves, and
+    the FP32 budgets are wider than the FP64 ones (outward FP32 rounding)."""
+    sets = _f32_sets()
+    soa = np.concatenate([sets["seeded"], sets["small_radii"]], axis=1)
+    m64, _ = prover.eval_batch(soa)
+    m32, infl = prover.f32_report(soa)
+    for k in (0, 1, 2, 3, 5, 6, 7, 8, 9):
+        p64 = ((m64 >> k) & 1).astype(bool)
+        p32 = ((m32 >> k) & 1).astype(bool)
+        assert p64.sum() > 100, (k, int(p64.sum()))
+        assert p32.sum() >= 0.5 * p64.sum(), (k, int(p32.sum()), int(p64.sum()))
+        r = infl[:, k][infl[:, k] >= 0]
+        assert r.size > 0.9 * soa.shape[1] and np.median(r) > 1.0, (k, r.size)
