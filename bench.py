@@ -18,8 +18,10 @@ second pass over the boxes whose cover-strip search it deferred).
 Unless --core-only is given, the line also carries `workloads`: the other
 configurations measured in the same run (not part of `value`):
   rsb_bnb      branch-and-bound over the 256x64x64 rectangle_size_bound grid,
-               lambda in [1, 8], depth 24: level-synchronous vs batched wall
-               time, identical unproven set required;
+               lambda in [1, 8], depth 24: wall time of the level-synchronous
+               form (reference operation sequence, and with the batched
+               form's exact shortcuts), the batched and the learned+batched
+               forms; identical unproven set required;
   tr_small     tight_rectangle: 64 lambda cells of [1, 1.5], r1..r4 in
                64/32/32/32 pieces, three bisection levels;
   fp32_prefilter  the FP32 pre-filter on the 2^22 boxes (first-success order,
@@ -289,6 +291,22 @@ def extended(p, soa: np.ndarray, mask: np.ndarray) -> dict:
     for b in (d_soa, d_mask, d_first):
         b.free()
     out["fp32_prefilter"]["per_criterion"] = per
+    # the shortcuts S1..S5 (criteria 5..9, on a cell's lambda, r1, r2, r3) on
+    # the cells the rectangle_size_bound branch-and-bound evaluates
+    # (64x32x32 grid over lambda in [1, 8], depth 24, every depth logged)
+    _, _, rec = p.rsb_bnb_log(cc.rsb_grid(1.0, 8.0, 64, 32, 32), 24)
+    cs = rec["cell"]
+    csoa = np.ascontiguousarray(cs[:, :8].T)
+    c64, _ = p.eval_batch(csoa)
+    c32, _ = p.f32_report(csoa, with_inflation=False)
+    s15 = {}
+    for k in range(5, 10):
+        a64 = ((c64 >> k) & 1).astype(bool)
+        a32 = ((c32 >> k) & 1).astype(bool)
+        s15[f"S{k - 4}"] = {"fp64_proves": int(a64.sum()), "fp32_proves": int(a32.sum()),
+                            "fraction_of_fp64_proofs_decided_fp32": float(a32.sum() / max(1, a64.sum())),
+                            "fp32_accept_fp64_reject": int((a32 & ~a64).sum())}
+    out["fp32_prefilter"]["rsb_bnb_cells"] = {"cells": int(cs.shape[0]), "shortcuts": s15}
     out["fp32_prefilter"]["batch"] = {"ms_fp64": batch_ms["fp64"], "ms_fp32_pre": batch_ms["fp32_pre"],
                                       "identical_masks": bool((pre_mask == mask).all()),
                                       "fp32_accept_fp64_reject": int((m32 & ~(mask & 0x3FF)).astype(bool).sum())}
@@ -329,13 +347,18 @@ def extended(p, soa: np.ndarray, mask: np.ndarray) -> dict:
     return out
 
 
-def c3_forms(p, forms=("plain", "batched", "learned_batched")) -> dict:
+C3_FORMS = ("plain", "plain_exact", "batched", "learned_batched")
+
+
+def c3_forms(p, forms=C3_FORMS) -> dict:
     """The rectangle_size_bound branch-and-bound on the 256x64x64 grid over
-    lambda in [1, 8], depth 24, in its three forms, each timed once after a
+    lambda in [1, 8], depth 24, in its forms, each timed once after a
     warm-up run (wall time of the call, cells uploaded from the host):
-    level-synchronous (plain order), batched, and learned+batched.  The
-    ranker of the learned form is trained on the GPU on the records the
-    prover logs on another grid of the same domain (128x48x48)."""
+    level-synchronous with the reference's operation sequence (plain), the
+    same with the batched form's exact shortcuts (plain_exact), batched, and
+    learned+batched.  The ranker of the learned form is trained on the GPU
+    on the records the prover logs on another grid of the same domain
+    (128x48x48)."""
     import circlecover_amd as cc
 
     cells = cc.rsb_grid(1.0, 8.0, 256, 64, 64)
@@ -351,39 +374,45 @@ def c3_forms(p, forms=("plain", "batched", "learned_batched")) -> dict:
         out.update({"train_records": int(len(rec)), "train_depths": int(rec["depth"].max()) + 1, "log_s": t_log,
                     "train_s": time.perf_counter() - t, "final_loss": float(losses[-1]),
                     "class_step": list(model.class_step)})
+    calls = {"batched": lambda: p.rsb_bnb_batched(cells, 24),
+             "learned_batched": lambda: p.rsb_bnb_learned(cells, 24, model),
+             "plain": lambda: p.rsb_bnb(cells, 24),
+             "plain_exact": lambda: p.rsb_bnb(cells, 24, exact=True)}
     runs = {}
-    for name, call in (("batched", lambda: p.rsb_bnb_batched(cells, 24)),
-                       ("learned_batched", lambda: p.rsb_bnb_learned(cells, 24, model)),
-                       ("plain", lambda: p.rsb_bnb(cells, 24))):
+    for name in ("batched", "learned_batched", "plain", "plain_exact"):
         if name not in forms:
             continue
         log(f"C3: {name} form")
-        if name != "plain":
-            call()  # warm-up: graph capture
+        if name in ("batched", "learned_batched"):
+            calls[name]()  # warm-up: graph capture
         t = time.perf_counter()
-        u, st = call()
+        u, st = calls[name]()
         runs[name] = (time.perf_counter() - t, u, st)
-    if set(runs) != {"plain", "batched", "learned_batched"}:
-        out.update({f"{k}_s": v[0] for k, v in runs.items()})
-        out.update({f"{k}_device_ms": v[2].device_ms for k, v in runs.items()})
-        return out
-    t_p, up, sp = runs["plain"]
-    t_b, ub, sb = runs["batched"]
-    t_l, ul, sl = runs["learned_batched"]
-    d = sp.depth_reached
-    same_tree = lambda st: list(st.cells_per_depth[: d + 1]) == list(sp.cells_per_depth[: d + 1])
+    out.update({f"{k}_s": v[0] for k, v in runs.items()})
+    out["device_ms"] = {k: v[2].device_ms for k, v in runs.items()}
+    for base in ("plain", "plain_exact"):
+        for fast in ("batched", "learned_batched"):
+            if base in runs and fast in runs:
+                out[f"speedup_{fast}" + ("" if base == "plain" else "_vs_plain_exact")] = runs[base][0] / runs[fast][0]
+    ref = runs.get("plain") or next(iter(runs.values()))
+    _, u0, s0 = ref
+    d = s0.depth_reached
     canon = lambda u: u[np.lexsort(u.T[::-1])] if len(u) else u
-    out.update({"plain_s": t_p, "batched_s": t_b, "learned_batched_s": t_l,
-                "speedup_batched": t_p / t_b, "speedup_learned_batched": t_p / t_l,
-                "device_ms": {"plain": sp.device_ms, "batched": sb.device_ms, "learned_batched": sl.device_ms},
-                "unproven": int(ub.shape[0]),
-                "identical": bool(np.array_equal(canon(up), canon(ub)) and np.array_equal(canon(up), canon(ul))
-                                  and same_tree(sb) and same_tree(sl)
-                                  and list(sp.verdict_hist) == list(sb.verdict_hist)),
-                "cells_evaluated": int(sum(sb.cells_per_depth[: sb.depth_reached + 1])),
-                "cell_evals": int(sb.cell_evals), "cell_evals_learned": int(sl.cell_evals),
-                "cell_evals_per_s_batched": sb.cell_evals / t_b, "cell_evals_per_s_plain": sp.cell_evals / t_p,
-                "trap_cells": int(sb.trap_cells), "verdict_hist_learned": list(sl.verdict_hist)})
+    out["unproven"] = int(u0.shape[0])
+    # every form: the same unproven cells and cells per depth; the forms in
+    # the reference's step order also the same verdict histogram
+    out["identical"] = bool(all(
+        np.array_equal(canon(u0), canon(u)) and list(st.cells_per_depth[: d + 1]) == list(s0.cells_per_depth[: d + 1])
+        and (name == "learned_batched" or list(st.verdict_hist) == list(s0.verdict_hist))
+        for name, (_, u, st) in runs.items()))
+    out["cells_evaluated"] = int(sum(s0.cells_per_depth[: d + 1]))
+    out["cell_evals"] = int(s0.cell_evals)
+    for name, (t, _, st) in runs.items():
+        out[f"cell_evals_per_s_{name}"] = st.cell_evals / t
+    if "learned_batched" in runs:
+        out["cell_evals_learned"] = int(runs["learned_batched"][2].cell_evals)
+        out["verdict_hist_learned"] = list(runs["learned_batched"][2].verdict_hist)
+    out["trap_cells"] = int(s0.trap_cells)
     return out
 
 
@@ -438,7 +467,7 @@ def main() -> int:
     ap.add_argument("--c3-only", action="store_true",
                     help="run only the rectangle_size_bound branch-and-bound (256x64x64 grid, depth 24) in both "
                          "forms, once each after a warm-up, and print its timings (for profiling)")
-    ap.add_argument("--c3-forms", default="plain,batched,learned_batched",
+    ap.add_argument("--c3-forms", default=",".join(C3_FORMS),
                     help="with --c3-only: the forms to run (comma-separated)")
     args = ap.parse_args()
     if args.traffic_probe:
@@ -513,6 +542,18 @@ def main() -> int:
                      "kernel": "ccrd_c2_eval"},
     }
     if workloads is not None:
+        # the metric's comparison: learned+batched against the plain GPU form
+        # on the branch-and-bound workload (not part of value)
+        c3 = workloads["rsb_bnb"]
+        if "learned_batched_s" in c3 and "plain_s" in c3:
+            out["comparison"] = {
+                "workload": "rsb_bnb 256x64x64 lambda [1, 8] depth 24",
+                "plain_s": c3["plain_s"], "plain_exact_s": c3.get("plain_exact_s"),
+                "batched_s": c3.get("batched_s"), "learned_batched_s": c3["learned_batched_s"],
+                "learned_batched_vs_plain": c3["plain_s"] / c3["learned_batched_s"],
+                "learned_batched_vs_plain_exact": (c3["plain_exact_s"] / c3["learned_batched_s"]
+                                                   if "plain_exact_s" in c3 else None),
+                "identical_unproven": c3["identical"]}
         if not args.no_full_proof:
             workloads["full_proof"] = full_proof()
         out["workloads"] = workloads

@@ -38,7 +38,8 @@ int cc_run_bnb(cc_ctx* ctx, const cc_bnb_config* cfg, cc_rsb_cell** unproven, si
   *unproven = nullptr;
   *n = 0;
   if (cfg->max_depth < 0 || cfg->max_depth >= CC_BNB_MAX_DEPTH) return fail(ctx, CC_E_ARG, "max_depth out of range");
-  if (cfg->form != CC_BNB_PLAIN && cfg->form != CC_BNB_BATCHED && cfg->form != CC_BNB_LEARNED)
+  if (cfg->form != CC_BNB_PLAIN && cfg->form != CC_BNB_BATCHED && cfg->form != CC_BNB_LEARNED &&
+      cfg->form != CC_BNB_PLAIN_EXACT)
     return fail(ctx, CC_E_ARG, "unknown form");
   if (cfg->form == CC_BNB_LEARNED && !cfg->ranker) return fail(ctx, CC_E_ARG, "the learned form needs a ranker");
   clear_interrupt(ctx);
@@ -74,11 +75,18 @@ int cc_run_bnb(cc_ctx* ctx, const cc_bnb_config* cfg, cc_rsb_cell** unproven, si
     std::free(out);
     out = static_cast<cc_rsb_cell*>(std::malloc((cap ? cap : 1) * sizeof(cc_rsb_cell)));
     if (!out) return fail(ctx, CC_E_ARG, "cannot allocate the output array");
-    rc = cfg->form == CC_BNB_PLAIN     ? cc_rsb_bnb(ctx, cells, ncells, cfg->max_depth, out, cap, &got, stats)
-         : cfg->form == CC_BNB_BATCHED ? cc_rsb_bnb_batched(ctx, cells, ncells, cfg->max_depth, cfg->params, out,
-                                                            cap, &got, stats)
-                                       : cc_rsb_bnb_learned(ctx, cells, ncells, cfg->max_depth, cfg->ranker,
-                                                            cfg->params, out, cap, &got, stats);
+    switch (cfg->form) {
+      case CC_BNB_PLAIN: rc = cc_rsb_bnb(ctx, cells, ncells, cfg->max_depth, out, cap, &got, stats); break;
+      case CC_BNB_PLAIN_EXACT:
+        rc = cc_rsb_bnb_exact(ctx, cells, ncells, cfg->max_depth, out, cap, &got, stats);
+        break;
+      case CC_BNB_BATCHED:
+        rc = cc_rsb_bnb_batched(ctx, cells, ncells, cfg->max_depth, cfg->params, out, cap, &got, stats);
+        break;
+      default:
+        rc = cc_rsb_bnb_learned(ctx, cells, ncells, cfg->max_depth, cfg->ranker, cfg->params, out, cap, &got,
+                                stats);
+    }
     if (rc == CC_E_OVERFLOW) cap = got;
   }
   if (rc != CC_OK) {

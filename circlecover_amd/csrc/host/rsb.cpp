@@ -213,10 +213,10 @@ int cc_rsb_find_critical_in(cc_ctx* ctx, const cc_rsb_cell* cell, int is_init, c
 
 namespace {
 
-// cc_rsb_bnb, and with log != NULL the records of every evaluated cell
-// (cc_rsb_bnb_log).
-int rsb_bnb_run(cc_ctx* ctx, const cc_rsb_cell* cells, size_t ncells, int max_depth, cc_rsb_cell* out, size_t cap,
-                size_t* n_out, cc_bnb_stats* stats, cc_bnb_record* log, size_t log_cap, size_t* n_log) {
+// cc_rsb_bnb (exact: cc_rsb_bnb_exact), and with log != NULL the records
+// of every evaluated cell (cc_rsb_bnb_log).
+int rsb_bnb_run(cc_ctx* ctx, const cc_rsb_cell* cells, size_t ncells, int max_depth, bool exact, cc_rsb_cell* out,
+                size_t cap, size_t* n_out, cc_bnb_stats* stats, cc_bnb_record* log, size_t log_cap, size_t* n_log) {
   if (!ctx || !n_out || (ncells && !cells) || (cap && !out) || max_depth < 0 || max_depth >= CC_BNB_MAX_DEPTH)
     return CC_E_ARG;
   if (n_log && !log && log_cap) return CC_E_ARG;
@@ -225,7 +225,7 @@ int rsb_bnb_run(cc_ctx* ctx, const cc_rsb_cell* cells, size_t ncells, int max_de
   const auto t0 = std::chrono::steady_clock::now();
   Scratch& S = scratch(ctx);
   hipFunction_t f;
-  CC_HIP(ctx, kernel(ctx, "ccrd_rsb_bnb_level", &f));
+  CC_HIP(ctx, kernel(ctx, exact ? "ccrd_rsb_bnb_level_x" : "ccrd_rsb_bnb_level", &f));
   cc_bnb_stats st;
   std::memset(&st, 0, sizeof st);
   int rc = upload(ctx, S.cells, cells, ncells * kRec);
@@ -291,13 +291,18 @@ extern "C" {
 
 int cc_rsb_bnb(cc_ctx* ctx, const cc_rsb_cell* cells, size_t ncells, int max_depth, cc_rsb_cell* out, size_t cap,
                size_t* n_out, cc_bnb_stats* stats) {
-  return rsb_bnb_run(ctx, cells, ncells, max_depth, out, cap, n_out, stats, nullptr, 0, nullptr);
+  return rsb_bnb_run(ctx, cells, ncells, max_depth, false, out, cap, n_out, stats, nullptr, 0, nullptr);
+}
+
+int cc_rsb_bnb_exact(cc_ctx* ctx, const cc_rsb_cell* cells, size_t ncells, int max_depth, cc_rsb_cell* out,
+                     size_t cap, size_t* n_out, cc_bnb_stats* stats) {
+  return rsb_bnb_run(ctx, cells, ncells, max_depth, true, out, cap, n_out, stats, nullptr, 0, nullptr);
 }
 
 int cc_rsb_bnb_log(cc_ctx* ctx, const cc_rsb_cell* cells, size_t ncells, int max_depth, cc_bnb_record* log,
                    size_t log_cap, size_t* n_log, cc_rsb_cell* out, size_t cap, size_t* n_out, cc_bnb_stats* stats) {
   if (!n_log) return CC_E_ARG;
-  return rsb_bnb_run(ctx, cells, ncells, max_depth, out, cap, n_out, stats, log, log_cap, n_log);
+  return rsb_bnb_run(ctx, cells, ncells, max_depth, false, out, cap, n_out, stats, log, log_cap, n_log);
 }
 
 }  // extern "C"

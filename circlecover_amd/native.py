@@ -75,6 +75,7 @@ lib.cc_rsb_grid.restype = _sz
 lib.cc_rsb_find_critical_in.argtypes = [_vp, _vp, _i, _vp, _sz, _P(_sz)]
 lib.cc_rsb_enumerate.argtypes = [_vp, _vp, _sz, _vp, _vp, _sz, _P(_sz)]
 lib.cc_rsb_bnb.argtypes = [_vp, _vp, _sz, _i, _vp, _sz, _P(_sz), _vp]
+lib.cc_rsb_bnb_exact.argtypes = [_vp, _vp, _sz, _i, _vp, _sz, _P(_sz), _vp]
 class FastParams(ctypes.Structure):
     _fields_ = [("cpw_a", ctypes.c_int), ("cpw_b", ctypes.c_int), ("blocks_per_cu", ctypes.c_int),
                 ("use_graph", ctypes.c_int), ("waves_per_simd", ctypes.c_int)]
@@ -171,7 +172,7 @@ class BnbStats(ctypes.Structure):
                 "device_ms": self.device_ms, "wall_ms": self.wall_ms}
 
 
-CC_BNB_PLAIN, CC_BNB_BATCHED, CC_BNB_LEARNED = 0, 1, 2
+CC_BNB_PLAIN, CC_BNB_BATCHED, CC_BNB_LEARNED, CC_BNB_PLAIN_EXACT = 0, 1, 2, 3
 
 
 class BnbConfig(ctypes.Structure):
@@ -350,12 +351,15 @@ class Prover:
         return self._grow(lambda o, c, n: lib.cc_rsb_enumerate(self.ctx, _ptr(cells), cells.shape[0], _ptr(cnt),
                                                                _ptr(o), c, ctypes.byref(n)), cap, 14, with_trap)
 
-    def rsb_bnb(self, cells: np.ndarray, max_depth: int, cap: int = 1 << 16):
-        """Returns (unproven cells at max_depth, BnbStats)."""
+    def rsb_bnb(self, cells: np.ndarray, max_depth: int, cap: int = 1 << 16, exact: bool = False):
+        """Level-synchronous branch-and-bound (exact: with the batched form's
+        exact shortcuts, cc_rsb_bnb_exact).  Returns (unproven cells at
+        max_depth, BnbStats)."""
         cells = np.ascontiguousarray(cells, dtype=np.float64).reshape(-1, 14)
         st = BnbStats()
-        out = self._grow(lambda o, c, n: lib.cc_rsb_bnb(self.ctx, _ptr(cells), cells.shape[0], max_depth, _ptr(o), c,
-                                                        ctypes.byref(n), ctypes.byref(st)), cap, 14)
+        fn = lib.cc_rsb_bnb_exact if exact else lib.cc_rsb_bnb
+        out = self._grow(lambda o, c, n: fn(self.ctx, _ptr(cells), cells.shape[0], max_depth, _ptr(o), c,
+                                            ctypes.byref(n), ctypes.byref(st)), cap, 14)
         return out, st
 
     def run_bnb(self, max_depth: int, form: int = CC_BNB_BATCHED, cells: np.ndarray | None = None,

@@ -203,6 +203,12 @@ typedef struct cc_bnb_stats {
  * with wave-aggregated compaction of the children. */
 int cc_rsb_bnb(cc_ctx* ctx, const cc_rsb_cell* cells, size_t ncells, int max_depth, cc_rsb_cell* out, size_t cap,
                size_t* n_out, cc_bnb_stats* stats);
+/* cc_rsb_bnb with the exact shortcuts of cc_rsb_bnb_batched (bisections whose
+ * test fails at their smallest argument exit at once, returning what the
+ * full loop returns) in the level kernel: the same per-cell arithmetic as
+ * the batched form, level-synchronous.  Same outputs as cc_rsb_bnb. */
+int cc_rsb_bnb_exact(cc_ctx* ctx, const cc_rsb_cell* cells, size_t ncells, int max_depth, cc_rsb_cell* out,
+                     size_t cap, size_t* n_out, cc_bnb_stats* stats);
 
 /* Launch shape of the batched branch-and-bound. */
 typedef struct cc_fast_params {
@@ -308,13 +314,14 @@ int cc_rsb_bnb_learned(cc_ctx* ctx, const cc_rsb_cell* cells, size_t ncells, int
 #define CC_BNB_PLAIN 0   /* level-synchronous: cc_rsb_bnb */
 #define CC_BNB_BATCHED 1 /* persistent stage kernels and one hipGraph: cc_rsb_bnb_batched */
 #define CC_BNB_LEARNED 2 /* batched with the learned step order: cc_rsb_bnb_learned */
+#define CC_BNB_PLAIN_EXACT 3 /* level-synchronous with the exact shortcuts: cc_rsb_bnb_exact */
 typedef struct cc_bnb_config {
   const cc_rsb_cell* cells; /* initial cells (host memory); NULL: the grid below */
   size_t ncells;
   double la_lb, la_ub;      /* grid: lambda in [la_lb, la_ub] split into nla pieces, squared radii in */
   int nla, nr1, nr2;        /* [0, 9/64] with r1 in nr1 and r2 (<= r1) in nr2 pieces, built as cc_rsb_grid */
   int max_depth;            /* 0 .. CC_BNB_MAX_DEPTH-1 */
-  int form;                 /* CC_BNB_PLAIN, CC_BNB_BATCHED or CC_BNB_LEARNED */
+  int form;                 /* CC_BNB_PLAIN, CC_BNB_BATCHED, CC_BNB_LEARNED or CC_BNB_PLAIN_EXACT */
   const cc_fast_params* params; /* batched forms' launch shape; NULL: autotuned or default */
   const cc_rsb_ranker* ranker;  /* CC_BNB_LEARNED */
 } cc_bnb_config;
@@ -325,7 +332,7 @@ typedef struct cc_bnb_config {
  * allocates: *unproven (NULL when *n == 0), released with cc_free.  All
  * forms give the same unproven cells and cells per depth (the learned
  * form's verdict histogram and trap count follow its order, see
- * cc_rsb_bnb_learned).  Interruptible (cc_interrupt) in the plain form. */
+ * cc_rsb_bnb_learned).  Interruptible (cc_interrupt) in the plain forms. */
 int cc_run_bnb(cc_ctx* ctx, const cc_bnb_config* cfg, cc_rsb_cell** unproven, size_t* n, cc_bnb_stats* stats);
 /* Releases an array returned by cc_run_bnb. */
 void cc_free(void* p);

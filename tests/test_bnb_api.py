@@ -1,4 +1,4 @@
-"""cc_run_bnb, the branch-and-bound in one call: both forms return the CPU
+"""cc_run_bnb, the branch-and-bound in one call: every form returns the CPU
 oracle's unproven cells in lexicographic order with the oracle's statistics,
 from caller cells or from a grid, and cc_interrupt stops a running call
 without affecting the next one (the reference clears its SIGINT flag when an
@@ -10,7 +10,7 @@ import pytest
 
 import oracle
 from circlecover_amd import rsb_grid
-from circlecover_amd.native import CC_BNB_BATCHED, CC_BNB_PLAIN, CC_E_INTERRUPTED, CCError
+from circlecover_amd.native import CC_BNB_BATCHED, CC_BNB_PLAIN, CC_BNB_PLAIN_EXACT, CC_E_INTERRUPTED, CCError
 from test_rsb_bnb import C3_DEPTH, c3_cells
 
 
@@ -20,7 +20,7 @@ def _lexsorted(cells):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("form", [CC_BNB_PLAIN, CC_BNB_BATCHED])
+@pytest.mark.parametrize("form", [CC_BNB_PLAIN, CC_BNB_BATCHED, CC_BNB_PLAIN_EXACT])
 def test_gpu_run_bnb_matches_oracle(prover, form):
     print(f"run_bnb form {form}: 4096 sampled cells, depth {C3_DEPTH}", flush=True)
     cells = c3_cells(4096, seed=11)
