@@ -289,4 +289,20 @@ int or_rsb_verdicts(const double* cells, size_t n, int32_t* out, uint8_t* trap, 
   return 0;
 }
 
+// Counting build: the calling thread's directed-operation counter (the
+// drivers run on the calling thread when given one thread), and its reset.
+uint64_t or_op_count() {
+#ifdef CC_ORACLE_COUNT
+  return g_op_count;
+#else
+  return 0;
+#endif
+}
+void or_op_count_reset() {
+#ifdef CC_ORACLE_COUNT
+  g_op_count = 0;
+#endif
+}
+
 }  // extern "C"
+
