@@ -176,6 +176,94 @@ CCD bool three_can_cover(double r1, double r2, double r3, double w, double h) {
   double w1 = chord(mul_rd(4.0, r1), hsq);
   return add_rd(w1, add_rd(w2, w3)) >= w;
 }
+#if CC_EXACT_SHORTCUTS
+// Real-arithmetic estimate of the height at which three chords cover ub_w.
+// In the third chord's half-width s = sqrt(q3 - h^2) the covered width is
+// G(s) = s + sum over i = 1, 2 of sqrt(max(q_i - q3 + s^2, 0)), increasing
+// with G' >= 1 and convex when q3 <= q1, q2 (the callers' order), so
+// Newton's method from s = sqrt(q3) (h = 0) descends to the root without
+// overshoot; a bracket keeps the other case in range.  Only the starting
+// point of three_height's search: its accuracy changes the number of tests
+// there, never the result.
+CCD double three_height_estimate(double q1, double q2, double q3, double ub_w) {
+  const double c1 = q1 - q3, c2 = q2 - q3;
+  double s_lo = 0.0, s_hi = __builtin_sqrt(q3), s = s_hi;
+  for (int it = 0; it < 10; ++it) {
+    const double ss = s * s;
+    double g = s - ub_w, dg = 1.0;
+    if (c1 + ss > 0.0) {
+      const double t = __builtin_sqrt(c1 + ss);
+      g += t;
+      dg += s / t;
+    }
+    if (c2 + ss > 0.0) {
+      const double t = __builtin_sqrt(c2 + ss);
+      g += t;
+      dg += s / t;
+    }
+    if (g >= 0.0) s_hi = s;  // still covers: the root lies at a smaller s (a larger h)
+    else s_lo = s;
+    double next = s - g / dg;
+    const bool done = __builtin_fabs(next - s) <= 0x1p-52 * s;
+    if (!done && !(next > s_lo && next < s_hi)) next = 0.5 * (s_lo + s_hi);
+    s = next;
+    if (done) break;
+  }
+  const double u = q3 - s * s;
+  return u > 0.0 ? __builtin_sqrt(u) : 0.0;
+}
+// The largest double h in (0, hi) at which the chords cover ub_w (their
+// covered width, non-increasing in h, is >= ub_w), or 0 if there is none:
+// the value three_height's bisection loop returns (see monotone_last_pass).
+// Searched over bit patterns from the estimate: a test there, a gallop of
+// doubling steps to a test of the other outcome, then a bisection of the
+// bit patterns between the two, about 2 log2(d) + 2 tests for an estimate d
+// doubles off (d is a few doubles for most operands), where the loop needs
+// about 50.  Requires 0 < hi < inf.
+CCD double three_height_search(double q1, double q2, double q3, double ub_w, double hi) {
+  const uint64_t H = bits(hi);
+  auto covers = [&](uint64_t B) {
+    const double h = from_bits(B);
+    const double hsq = mul_ru(h, h);
+    return !(add_rd(chord(q1, hsq), add_rd(chord(q2, hsq), chord(q3, hsq))) < ub_w);
+  };
+  if (H <= 1u) return 0.0;            // no double lies in (0, hi)
+  if (covers(H - 1u)) return from_bits(H - 1u);  // every double of (0, hi) passes
+  uint64_t E = bits(three_height_estimate(q1, q2, q3, ub_w));
+  if (E == 0u || E >= H) E = H / 2u;  // estimate outside (0, hi), or NaN: start mid-range
+  uint64_t lo, up;  // covers(lo) or lo == 0; !covers(up) or up == H
+  if (covers(E)) {
+    lo = E;
+    up = H;
+    for (uint64_t step = 1u;; step <<= 1) {
+      if (H - lo <= step) break;
+      if (!covers(lo + step)) {
+        up = lo + step;
+        break;
+      }
+      lo += step;
+    }
+  } else {
+    up = E;
+    lo = 0u;
+    for (uint64_t step = 1u;; step <<= 1) {
+      if (up <= step) break;
+      if (covers(up - step)) {
+        lo = up - step;
+        break;
+      }
+      up -= step;
+    }
+  }
+  while (up - lo > 1u) {
+    const uint64_t M = lo + (up - lo) / 2u;
+    if (covers(M)) lo = M;
+    else up = M;
+  }
+  return from_bits(lo);
+}
+#endif
+
 // three_disks_maximize_height: maximize_covered.cu:198-237
 static __device__ __noinline__ double three_height(double r1, double r2, double r3, double ub_w) {
   const double q1 = mul_rd(4.0, r1), q2 = mul_rd(4.0, r2), q3 = mul_rd(4.0, r3);
@@ -187,10 +275,16 @@ static __device__ __noinline__ double three_height(double r1, double r2, double 
   // test already fails at the smallest positive double, the loop would halve
   // hi down to zero (up to ~1075 iterations) and return 0: return it now.
   // hi > 0 excludes NaN bounds, for which the loop behaves differently.
+  // Otherwise the loop ends at the largest double of (0, hi) that passes
+  // (monotone_last_pass), which three_height_search finds with fewer tests
+  // (finite operands: the test is then monotone and defined throughout).
   if (hi > 0.0) {
     const double tiny = 0x1p-1074;
     const double hsq = mul_ru(tiny, tiny);
     if (add_rd(chord(q1, hsq), add_rd(chord(q2, hsq), chord(q3, hsq))) < ub_w) return 0.0;
+    if (hi < __builtin_inf() && __builtin_isfinite(q1) && __builtin_isfinite(q2) && __builtin_isfinite(q3) &&
+        !__builtin_isnan(ub_w))
+      return three_height_search(q1, q2, q3, ub_w, hi);
   }
 #endif
   for (int it = 0; it < kBisectCap; ++it) {
