@@ -433,7 +433,7 @@ int cc_rsb_prove_shape(cc_ctx* ctx, const int init_counts[4], const int gen_grid
  * TR/tight_rectangle.cu:481-489 and RSB/rectangle_size_bound.cu:342-349).
  * The one function that may be called from another thread while a call runs
  * on ctx.  Polled by cc_tr_prove[_ex], cc_rsb_prove[_ex], cc_tr_bnb,
- * cc_rsb_bnb, cc_run_bnb (level-synchronous form) and cc_ranker_train, each
+ * cc_rsb_bnb, cc_run_bnb (level-synchronous forms) and cc_ranker_train, each
  * of which clears the flag when it starts, as the reference's
  * start_interruptible_computation does (interrupt.cpp:38-41): an interrupt
  * never stops a later call.  Results:
