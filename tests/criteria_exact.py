@@ -1,5 +1,5 @@
 """The ten criteria of the 4-D box set and the rectangle_size_bound leaf
-steps C1, C2, C4 and C7, decided in exact real arithmetic (test support,
+steps C1, C2, C3, C4, C5 and C7, decided in exact real arithmetic (test support,
 independent of oracle/ and of the kernels).
 
 Each function restates the mathematics of one reference criterion on a
@@ -25,7 +25,9 @@ uneven_split_recursion.cu:78-125, r1_r2_opposite_corners_recursion.cu:28-71,
 r1_r2_vertical_strip.cu:28-41; for the leaf steps variables.cuh:30-52,
 compute_even_split_distance_bound.cu:25-76, even_split_recursion.cu:25-40,
 uneven_split_recursion.cu:25-76, vertical_wall_building_recursion.cu:25-78,
-r1_r2_vertical_strip.cu:43-95.
+r1_r2_vertical_strip.cu:43-95, multi_disk_strip.cu:31-262,
+r1_in_corner_explicit_recursion.cu:35-262, maximize_covered.cu:28-104,
+171-237; operations.cu:30-83.
 """
 from __future__ import annotations
 
@@ -527,7 +529,307 @@ def rsb_c7_vertical_strip(D, la, r):
     return (True, 3) if rsb_can_recurse(D, rem, r7, wrem, hrem) else (False, 0)
 
 
-RSB_STEPS = {6: rsb_c1_even_split, 7: rsb_c2_uneven_split, 9: rsb_c4_wall_building, 12: rsb_c7_vertical_strip}
+def circle_intersection(D, c1, c2):
+    """intersection (operations.cu:30-83) of two circles (x, y, squared
+    radius): (definitely intersecting, first point, second point), the first
+    point being the '+' one when c2 lies below c1, or level with it and not
+    to its left (the reference's order rule, Q17)."""
+    (x1, y1, q1), (x2, y2, q2) = c1, c2
+    d2 = (x2 - x1) ** 2 + (y2 - y1) ** 2
+    if d2 == 0:
+        return False, None, None
+    rd = q1 - q2
+    if D.cmp(2 * (q1 + q2) / d2, "<", 1 + rd * rd / (d2 * d2)):
+        return False, None, None
+    f1 = mpmath.sqrt(2 * (q1 + q2) / d2 - 1 - rd * rd / (d2 * d2)) / 2
+    f2 = rd / (2 * d2)
+    xd, yd = x2 - x1, y2 - y1
+    cx, cy = (x1 + x2) / 2 + f2 * xd, (y1 + y2) / 2 + f2 * yd
+    plus, minus = (cx + f1 * yd, cy - f1 * xd), (cx - f1 * yd, cy + f1 * xd)
+    if D.cmp(y2, "<", y1) or (y2 == y1 and D.cmp(x2, ">=", x1)):
+        return True, plus, minus
+    return True, minus, plus
+
+
+def dist2(p, q):
+    return (p[0] - q[0]) ** 2 + (p[1] - q[1]) ** 2
+
+
+def _rem_width_ok(D, la, w, largest):
+    """The strip of width w leaves la - w: wide enough, or the largest
+    remaining disk meets the size bound for it."""
+    rem = la - w
+    return D.cmp(rem, ">=", 1) or size_bound_ok(D, largest, rem)
+
+
+def _c3_nd(D, la, r, k):
+    """multi_disk_strip_vertical_nd (multi_disk_strip.cu:31-62): r1..rk in a
+    vertical strip of the width their weight needs at ratio 0.61, stacked
+    while each spans it; the rest of the rectangle for the largest remaining
+    disk (r6 when all six are in the strip, as the reference has it)."""
+    w = sum(r[:k]) / CR
+    ht = mpf(0)
+    for j in range(k):
+        if D.cmp(4 * r[j], "<=", w * w):
+            break
+        ht += mpmath.sqrt(4 * r[j] - w * w)
+    if D.cmp(ht, "<", 1):
+        return False
+    return _rem_width_ok(D, la, w, r[5] if k == 6 else r[k])
+
+
+def _c3_stacked_r34(D, la, r):
+    """r1_r2_stacked_r3_r4_horizontal (multi_disk_strip.cu:64-98)."""
+    w = sum(r[:4]) / CR
+    if D.cmp(4 * r[0], "<=", w * w) or D.cmp(4 * r[1], "<=", w * w):
+        return False
+    hrem = 1 - (mpmath.sqrt(4 * r[0] - w * w) + mpmath.sqrt(4 * r[1] - w * w))
+    if D.cmp(4 * r[3], "<=", hrem * hrem) or D.cmp(4 * r[2], "<=", hrem * hrem):
+        return False
+    w34 = mpmath.sqrt(4 * r[2] - hrem * hrem) + mpmath.sqrt(4 * r[3] - hrem * hrem)
+    if D.cmp(w34, "<", w):
+        return False
+    return _rem_width_ok(D, la, w, r[4])
+
+
+def _c3_pair_height(D, w, ra, rb):
+    """Two disks side by side in the strip of width w (multi_disk_strip.cu:
+    116-126): the lower of their heights, or None if one cannot span its part."""
+    diff = max(ra - rb, mpf(0))
+    wa = 2 * diff / w + w / 2
+    if D.cmp(4 * ra, "<=", wa * wa) or D.cmp(4 * rb, "<=", (w - wa) ** 2):
+        return None
+    return mpmath.sqrt(4 * ra - wa * wa)
+
+
+def _c3_stacked_r34_r56(D, la, r):
+    """r1_r2_stacked_r3_r4_and_r5_r6_horizontal (multi_disk_strip.cu:100-143)."""
+    w = sum(r) / CR
+    if D.cmp(4 * r[0], "<=", w * w) or D.cmp(4 * r[1], "<=", w * w):
+        return False
+    h34 = _c3_pair_height(D, w, r[2], r[3])
+    if h34 is None:
+        return False
+    h56 = _c3_pair_height(D, w, r[4], r[5])
+    if h56 is None:
+        return False
+    h12 = mpmath.sqrt(4 * r[0] - w * w) + mpmath.sqrt(4 * r[1] - w * w)
+    if D.cmp(h12 + h34 + h56, "<", 1):
+        return False
+    return _rem_width_ok(D, la, w, r[5])
+
+
+# The reference's height of a two-disk row in six_disk_strip_three_rows: the
+# square root taken twice (Q6).
+ROW_HEIGHT = lambda a: mpmath.sqrt(mpmath.sqrt(a))  # noqa: E731
+
+
+def _c3_three_rows(D, la, r):
+    """six_disk_strip_three_rows (multi_disk_strip.cu:145-238): r1 and r4 on
+    top, r2 and r3 below, r5 and r6 covering the middle row's left and right
+    ends, in the strip of the width the six disks need.  The heights of the
+    two-disk rows take the square root twice, as the reference does (Q6)."""
+    w = sum(r) / CR
+    r7 = min(r[5], rsb_R(la, r))
+    if D.cmp(la - w, "<", 1) and not size_bound_ok(D, r7, la - w):
+        return False
+    rows = []
+    for a, b in ((0, 3), (1, 2)):
+        arg = 2 * (r[a] + r[b]) - w * w / 4 - 4 * (r[a] - r[b]) ** 2 / (w * w)
+        if D.cmp(arg, "<", 0):
+            return False  # the reference's NaN path ends in false
+        rows.append(ROW_HEIGHT(arg))
+    h14, h23 = rows
+    hrem = 1 - h23 - h14
+    if D.cmp(hrem, "<=", 0):
+        return True
+    if D.cmp(r[4], "<", hrem * hrem / 4) or D.cmp(r[5], "<", hrem * hrem / 4):
+        return False
+    x5 = mpmath.sqrt(r[4] - hrem * hrem / 4)
+    x6 = w - mpmath.sqrt(r[5] - hrem * hrem / 4)
+    y5 = hrem / 2 + h23
+    y1, y2 = 1 - h14 / 2, h23 / 2
+    w1 = (2 / w) * (r[0] - r[3]) + w / 2
+    w2 = (2 / w) * (r[1] - r[2]) + w / 2
+    c1, c2 = (w - w1 / 2, y1, r[0]), (w - w2 / 2, y2, r[1])
+    c3, c4 = ((w - w2) / 2, y2), ((w - w1) / 2, y1)
+    c5, c6 = (x5, y5, r[4]), (x6, y5)
+    ok15, p15, q15 = circle_intersection(D, c5, c1)
+    ok25, p25, q25 = circle_intersection(D, c5, c2)
+    if not ok15 or not ok25:
+        return False
+    if D.cmp(dist2(c4, p15), ">", r[3]) or D.cmp(dist2(c3, p25), ">", r[2]):
+        return False
+    return D.cmp(dist2(c6, q15), "<=", r[5]) and D.cmp(dist2(c6, q25), "<=", r[5])
+
+
+def rsb_c3_multi_disk_strip(D, la, r):
+    """C3, multi_disk_strip_vertical (multi_disk_strip.cu:240-262)."""
+    for k in range(2, 7):
+        if _c3_nd(D, la, r, k):
+            return True, 1
+    if _c3_stacked_r34(D, la, r):
+        return True, 2
+    if _c3_stacked_r34_r56(D, la, r):
+        return True, 3
+    if _c3_three_rows(D, la, r):
+        return True, 4
+    return False, 0
+
+
+def _chords(d, x):
+    """Sum of the chords the disks of squared radii d cut from a strip of
+    width (or height) x: sqrt(4 d_i - x^2) where positive."""
+    return sum((mpmath.sqrt(4 * q - x * x) for q in d if 4 * q > x * x), mpf(0))
+
+
+def _chords_dec(D, d, x):
+    """_chords with each disk's "spans the strip" decision recorded."""
+    t = mpf(0)
+    for q in d:
+        if not D.cmp(4 * q, "<=", x * x):
+            t += mpmath.sqrt(4 * q - x * x)
+    return t
+
+
+def _sup_decreasing(f, a, b):
+    """The point where the continuous non-increasing f (f(a) >= 0 > f(b))
+    crosses zero, by bisection to far below any compared slack."""
+    for _ in range(220):
+        m = (a + b) / 2
+        if f(m) >= 0:
+            a = m
+        else:
+            b = m
+    return (a + b) / 2
+
+
+def nd_can_cover(D, w, h, d):
+    """nd_can_cover (maximize_covered.cu:28-71): the disks d (squared radii,
+    at most 6) cover a w x h rectangle in one row or one column."""
+    n = len(d)
+    if n == 1:
+        return D.cmp(4 * d[0], ">=", w * w + h * h)
+    if n == 2:
+        return (D.cmp(two_disks_maximize_height(D, d[0], d[1], w), ">=", h)
+                or D.cmp(two_disks_maximize_height(D, d[0], d[1], h), ">=", w))
+    if n == 3:
+        return D.cmp(_chords_dec(D, d, h), ">=", w)
+    return D.cmp(_chords_dec(D, d, w), ">=", h) or D.cmp(_chords_dec(D, d, h), ">=", w)
+
+
+def nd_maximize_covered(D, w, d):
+    """nd_maximize_covered (maximize_covered.cu:73-104, three_disks_maximize_height
+    :198-237): the greatest height of a strip of width w the disks d cover
+    (side by side; for four or more also stacked), below 2 sqrt(d_last), the
+    bound the reference's bisection starts from; 0 if none."""
+    n = len(d)
+    if n == 1:
+        return mpf(0) if D.cmp(4 * d[0], "<=", w * w) else mpmath.sqrt(4 * d[0] - w * w)
+    if n == 2:
+        return two_disks_maximize_height(D, d[0], d[1], w)
+    hi = 2 * mpmath.sqrt(d[-1])
+    side = mpf(0)  # side by side: the chords at height h reach w up to this height
+    if not D.cmp(_chords(d, mpf(0)), "<", w):
+        if D.cmp(_chords(d, hi), ">=", w):
+            side = hi
+        else:
+            side = _sup_decreasing(lambda h: _chords(d, h) - w, mpf(0), hi)
+    if n == 3:
+        return side
+    stacked = _chords_dec(D, d, w)  # stacked: the chords at width w
+    best = side if D.cmp(side, ">=", stacked) else stacked
+    return hi if D.cmp(best, ">=", hi) else best
+
+
+def _c5_backtrack(D, d, reached, target, cross):
+    """The backtracking of r1_in_corner_explicit_recursion (:60-100): the
+    disks d, in order, split into consecutive groups, each covering a strip
+    of extent `cross` as far as it can; true once the strips reach `target`,
+    or the last group covers what is left."""
+    for choice in range(1, len(d) + 1):
+        group = d[:choice]
+        if choice == len(d):
+            return nd_can_cover(D, cross, target - reached, group)
+        got = nd_maximize_covered(D, cross, group)
+        if got == 0:
+            continue
+        if D.cmp(reached + got, ">=", target):
+            return True
+        if _c5_backtrack(D, d[choice:], reached + got, target, cross):
+            return True
+    return False
+
+
+def _c5_max_remaining(la, r, nd):
+    return min(r[5], rsb_R(la, r)) if nd == 5 else r[nd + 1]
+
+
+def _c5_above(D, la, r, nd):
+    """r1_in_corner_explicit_recursion_above_nd (:35-104): r1..r(nd+1) in a
+    bottom-left strip of the width their weight needs, r1 in the corner,
+    groups of the others stacked above it."""
+    w1 = sum(r[: nd + 1]) / CR
+    allowed = la - bound_required_height(_c5_max_remaining(la, r, nd))
+    if D.cmp(w1, ">", allowed) or D.cmp(4 * r[0], "<=", w1 * w1):
+        return False
+    return _c5_backtrack(D, r[1: nd + 1], mpmath.sqrt(4 * r[0] - w1 * w1), mpf(1), w1)
+
+
+def _c5_right(D, la, r, nd):
+    """r1_in_corner_explicit_recursion_right_nd (:106-175): the same in a
+    bottom strip of the height their weight needs, groups to the right of r1."""
+    h1 = sum(r[: nd + 1]) / CR / la
+    allowed = 1 - bound_required_height(_c5_max_remaining(la, r, nd))
+    if D.cmp(h1, ">", allowed) or D.cmp(4 * r[0], "<=", h1 * h1):
+        return False
+    return _c5_backtrack(D, r[1: nd + 1], mpmath.sqrt(4 * r[0] - h1 * h1), la, h1)
+
+
+def _c5_right_34_and_2(D, la, r):
+    """r1_in_corner_right_34_and_2_recursion (:177-247): r1 in a corner
+    square, r3 and r4 stacked to its right, r2 beside them, the extra weight
+    of the efficient cover against the cost of the rest B."""
+    w1 = mpmath.sqrt(2 * r[0])
+    wrem = la - w1
+    if not size_bound_ok(D, r[4], 1 - w1):
+        return False
+    h34 = two_disks_maximize_height(D, r[2], r[3], wrem)
+    if D.cmp(h34, "<=", 0) or D.cmp(h34, ">=", w1):
+        return False
+    hrem = w1 - h34
+    if D.cmp(4 * r[1], "<=", hrem * hrem):
+        return False
+    w2 = mpmath.sqrt(4 * r[1] - hrem * hrem)
+    no_b = D.cmp(w2, ">=", wrem)
+    if no_b or w2 > wrem:
+        w2 = wrem
+    a234 = w2 * w1 + h34 * (wrem - w2)
+    extra = (2 * CR - 1) * r[0] + (CR * a234 - (r[1] + r[2] + r[3]))
+    if D.cmp(extra, "<=", 0):
+        return False
+    if no_b:
+        return True
+    hb, wb = w1 - h34, wrem - w2
+    cost = r[5]
+    if not size_bound_ok(D, r[5], min(hb, wb)):
+        cost += (bound_worst_case_ratio(D, wb, hb) - CR) * hb * wb
+    return D.cmp(cost, "<=", extra)
+
+
+def rsb_c5_r1_in_corner(D, la, r):
+    """C5, r1_in_corner_explicit_recursion (r1_in_corner_explicit_recursion.cu:249-262)."""
+    for nd in range(2, 6):
+        if _c5_above(D, la, r, nd):
+            return True, 1
+        if _c5_right(D, la, r, nd):
+            return True, 2
+    if _c5_right_34_and_2(D, la, r):
+        return True, 3
+    return False, 0
+
+
+RSB_STEPS = {6: rsb_c1_even_split, 7: rsb_c2_uneven_split, 8: rsb_c3_multi_disk_strip, 9: rsb_c4_wall_building, 10: rsb_c5_r1_in_corner, 12: rsb_c7_vertical_strip}
 
 
 def rsb_steps(la, r):

@@ -235,7 +235,7 @@ def test_planted_slips_are_caught(pinned):
 
 # --------------------------------------------- rectangle_size_bound leaf steps
 
-RSB_STRATEGIES = {6: {1, 2}, 7: {1, 2}, 9: {1}, 12: {1, 3}}
+RSB_STRATEGIES = {6: {1, 2}, 7: {1, 2}, 8: {1, 2, 3, 4}, 9: {1}, 10: {1, 2}, 12: {1, 3}}
 
 # Cells whose C7 (r1_r2_vertical_strip) is decided by its last recursion, with
 # all six disks placed (found by a seeded random search; radii beyond 9/64,
@@ -265,10 +265,11 @@ def rsb_point_cells():
 
 
 def test_rsb_leaf_steps_equal_exact_at_clear_slack():
-    """The leaf-chain steps C1 (even split), C2 (uneven split), C4 (vertical
-    wall building) and C7 (r1, r2 strip and greedy strips), each evaluated
-    alone on point cells by the oracle's per-step masks, against their exact
-    statement; planted slips in C2 and C4 are caught."""
+    """The leaf-chain steps C1 (even split), C2 (uneven split), C3 (multi-disk
+    strips), C4 (vertical wall building), C5 (r1 in a corner with explicit
+    recursion) and C7 (r1, r2 strip and greedy strips), each evaluated alone
+    on point cells by the oracle's per-step masks, against their exact
+    statement; planted slips in C2, C3, C4 and C5 are caught."""
     P = rsb_point_cells()
     cells = np.zeros((len(P), 14))
     cells[:, 0] = cells[:, 1] = P[:, 0]
@@ -334,7 +335,23 @@ def test_rsb_leaf_steps_equal_exact_at_clear_slack():
             below *= ce.PHASE_FACTOR_WALL_BUILDING
         return False, 0
 
-    for k, f in ((7, c2_index), (9, _c4_sub)):
+    def c3_single_root(D, la, r):  # C3's two-disk rows with one square root (Q6 "fixed")
+        saved = ce.ROW_HEIGHT
+        ce.ROW_HEIGHT = mpmath.sqrt
+        try:
+            return ce.rsb_c3_multi_disk_strip(D, la, r)
+        finally:
+            ce.ROW_HEIGHT = saved
+
+    def c5_index(D, la, r):  # C5's largest remaining disk one index early
+        saved = ce._c5_max_remaining
+        ce._c5_max_remaining = lambda la_, r_, nd: min(r_[5], ce.rsb_R(la_, r_)) if nd == 5 else r_[nd]
+        try:
+            return ce.rsb_c5_r1_in_corner(D, la, r)
+        finally:
+            ce._c5_max_remaining = saved
+
+    for k, f in ((7, c2_index), (9, _c4_sub), (8, c3_single_root), (10, c5_index)):
         caught = False
         for i, p in enumerate(P):
             D = ce.Dec()
