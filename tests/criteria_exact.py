@@ -795,7 +795,7 @@ def _c5_right_34_and_2(D, la, r):
     if not size_bound_ok(D, r[4], 1 - w1):
         return False
     h34 = two_disks_maximize_height(D, r[2], r[3], wrem)
-    if D.cmp(h34, "<=", 0) or D.cmp(h34, ">=", w1):
+    if h34 <= 0 or D.cmp(h34, ">=", w1):
         return False
     hrem = w1 - h34
     if D.cmp(4 * r[1], "<=", hrem * hrem):
@@ -829,7 +829,209 @@ def rsb_c5_r1_in_corner(D, la, r):
     return False, 0
 
 
-RSB_STEPS = {6: rsb_c1_even_split, 7: rsb_c2_uneven_split, 8: rsb_c3_multi_disk_strip, 9: rsb_c4_wall_building, 10: rsb_c5_r1_in_corner, 12: rsb_c7_vertical_strip}
+WIDTH_FRACTION_INCOMPLETE = 1 / _WB_DEN
+LB_W_INFTY_DENOM = 1 - mpmath.sqrt(PHASE_FACTOR_WALL_BUILDING)
+
+
+def rsb_c10_r1_corner_wall_building(D, la, r):
+    """C10, r1_in_corner_wall_building_recursion (r1_in_corner_wall_building_recursion.cu:
+    27-149): r1 covers a square in the lower-left corner; the weight it saves
+    pays for the region A above it (size-bounded recursion, or a row of wall
+    building with r6 and smaller), r2 (with r3, or r3 and r4, when it does
+    not fit) goes to the region B right of it, and the weight dropped to B
+    by incomplete rows is bounded."""
+    w1 = mpmath.sqrt(2 * r[0])
+    extra_r1 = (2 * CR - 1) * r[0]
+    wb, ha = la - w1, 1 - w1
+    min_a = ha if D.cmp(ha, "<", w1) else w1
+    max_a = w1 if min_a is ha else ha
+    wb_a = UB_DISK_WEIGHT * min_a * min_a
+    extra_a = r[5] if D.cmp(wb_a, ">=", r[5]) else wb_a
+    if wb_a <= 0 or D.cmp(extra_a, ">", extra_r1):
+        return False, 0
+    if D.cmp(wb, "<", 1) and not size_bound_ok(D, r[1], wb):
+        better = -INF
+        h23 = two_disks_maximize_height(D, r[1], r[2], wb)
+        if size_bound_ok(D, r[3], min(1 - h23, wb)):
+            better = CR * h23 * wb - (r[1] + r[2])
+        h234 = nd_maximize_covered(D, wb, [r[1], r[2], r[3]])
+        if size_bound_ok(D, r[4], min(1 - h234, wb)):
+            e = CR * h234 * wb - (r[1] + r[2] + r[3])
+            if better == -INF or D.cmp(e, ">", better):
+                better = e
+        extra_r1 = extra_r1 + better
+        if better == -INF or D.cmp(extra_r1, "<=", 0):
+            return False, 0
+    if D.cmp(WEIGHT_BOUND_WALL_BUILDING * max_a * max_a, "<", r[5]):
+        return False, 0
+    short = D.cmp(WEIGHT_BOUND_WALL_BUILDING * min_a * min_a, ">=", r[5])
+    side = min_a if short else max_a
+    cost_wall = CR * side * mpmath.sqrt(2 * r[5])
+    if D.cmp(cost_wall, ">", extra_a):
+        extra_a = cost_wall
+    if D.cmp(extra_a, ">", extra_r1):
+        return False, 0
+    if D.cmp(r[5], "<=", wb_a):
+        return True, 1
+    r6_weight = la * CR - sum(r[:5])
+    width_incomplete = side * WIDTH_FRACTION_INCOMPLETE
+    need_a = CR * w1 * ha
+    dropped_inf = CR * width_incomplete * mpmath.sqrt(2 * r[5]) / LB_W_INFTY_DENOM
+    if D.cmp(r6_weight - dropped_inf, ">=", need_a):
+        return True, 2
+    dropped, cur = need_a, r[5]
+    while D.cmp(cur, ">", wb_a):
+        dropped += CR * width_incomplete * mpmath.sqrt(2 * cur)
+        cur *= PHASE_FACTOR_WALL_BUILDING
+    return (True, 3) if D.cmp(r6_weight - dropped, ">=", need_a) else (False, 0)
+
+
+def _c9_cover_rest(D, rem, largest, w, h):
+    """A w x h rectangle with weight rem: at 0.61 per area when the largest
+    disk meets the size bound for it, else by can_recurse."""
+    if size_bound_ok(D, largest, min(w, h)):
+        return D.cmp(CR * w * h, "<=", rem)
+    return rsb_can_recurse(D, rem, largest, w, h)
+
+
+def _c9_one_horizontal(D, la, r, w, weight):
+    """l_shaped_recursion_two_disks_vertical_one_horizontal (l_shaped_recursion.cu:
+    27-122): r3 as a square in the corner of the w x 1 rest; the L around
+    it split top / bottom-right or top-left / right, or r3 across the whole
+    width and the rest above it."""
+    R4 = weight - r[2]
+    s3 = mpmath.sqrt(2 * r[2])
+    htop = 1 - s3
+    if size_bound_ok(D, r[3], min(w, htop)):
+        rem = R4 - (CR * htop * w + r[3])
+        if D.cmp(rem, ">", 0) and _c9_cover_rest(D, rem, r[4], w - s3, s3):
+            return 1
+    wright = la - s3
+    if D.cmp(wright, ">=", 1) or size_bound_ok(D, r[3], wright):
+        rem = R4 - (CR * wright + r[3])
+        if D.cmp(rem, ">", 0) and _c9_cover_rest(D, rem, r[4], s3, 1 - s3):
+            return 2
+    if not D.cmp(4 * r[2], "<=", w * w):
+        hrem = 1 - mpmath.sqrt(4 * r[2] - w * w)
+        if _c9_cover_rest(D, R4, r[3], w, hrem):
+            return 3
+    return 0
+
+
+def rsb_c9_l_shaped(D, la, r):
+    """C9, l_shaped_recursion (l_shaped_recursion.cu:124-155): r1 and r2
+    stacked in a full-height strip, then r3 (one horizontal) or r3 and r4
+    (two) in the rest; the two-horizontal form recurses with r6 as the
+    largest remaining disk, as the reference has it."""
+    weight = la * CR - (r[0] + r[1])
+    wrem = la - two_disks_maximize_height(D, r[0], r[1], mpf(1))
+    if not D.cmp(wrem * CR, ">", weight):
+        s = _c9_one_horizontal(D, la, r, wrem, weight)
+        if s:
+            return True, s
+    w12 = two_disks_maximize_height(D, r[0], r[1], mpf(1))
+    if w12 <= 0:  # exactly 0: the function decided that (its comparisons are recorded)
+        return False, 0
+    wrem = la - w12
+    h34 = two_disks_maximize_height(D, r[2], r[3], wrem)
+    if h34 <= 0:
+        return False, 0
+    R5 = la * CR - (r[0] + r[1] + r[2] + r[3])
+    return (True, 4) if rsb_can_recurse(D, R5, r[5], wrem, 1 - h34) else (False, 0)
+
+
+def _two_by_two_concrete(D, a, b, c, d):
+    """compute_two_by_two_cover_concrete (two_by_two_cover.cu:28-76): a over b
+    in the first column, c over d in the second, each column as wide as its
+    two disks stacked can cover at full height; a column that cannot is left
+    out (width 0, centres at the origin).  Returns (width, circles[0..3])."""
+    def column(top, bottom):
+        diff = top - bottom
+        h1, h2 = mpf("0.5") + 2 * diff, mpf("0.5") - 2 * diff
+        wsq = 2 * (top + bottom) - mpf("0.25") - 4 * diff * diff
+        if (D.cmp(top, "<=", h1 * h1 / 4) or D.cmp(bottom, "<=", h2 * h2 / 4)
+                or D.cmp(2 * (top + bottom), "<=", mpf("0.25") + 4 * diff * diff)):
+            return None
+        return mpmath.sqrt(wsq), h1, h2
+    circles = [None] * 4
+    width, w1 = mpf(0), mpf(0)
+    col = column(a, b)
+    if col is None:
+        circles[0], circles[1] = (mpf(0), mpf(0), a), (mpf(0), mpf(0), b)
+    else:
+        w1, h1, h2 = col
+        circles[0], circles[1] = (w1 / 2, h1 / 2, a), (w1 / 2, 1 - h2 / 2, b)
+        width = w1
+    col = column(c, d)
+    if col is None:
+        circles[3], circles[2] = (mpf(0), mpf(0), c), (mpf(0), mpf(0), d)
+    else:
+        w2, h1, h2 = col
+        x = w1 + w2 / 2
+        circles[3], circles[2] = (x, 1 - h1 / 2, c), (x, h2 / 2, d)
+        width += w2
+    return width, circles
+
+
+def two_by_two_cover(D, r):
+    """compute_two_by_two_cover (two_by_two_cover.cu:78-89): the widest of the
+    three pairings of r1..r4 (ties to the later pairing, as the reference
+    selects)."""
+    c1 = _two_by_two_concrete(D, r[0], r[1], r[2], r[3])
+    c2 = _two_by_two_concrete(D, r[0], r[2], r[1], r[3])
+    c3 = _two_by_two_concrete(D, r[0], r[3], r[1], r[2])
+
+    def wider(a, b):  # two pairings left out entirely tie at exactly 0 in every arithmetic
+        return False if a[0] == 0 and b[0] == 0 else D.cmp(a[0], ">", b[0])
+    if wider(c1, c2):
+        return c1 if wider(c1, c3) else c3
+    return c2 if wider(c2, c3) else c3
+
+
+def rsb_c8_two_by_two(D, la, r):
+    """C8 (rectangle_size_bound.cu:206-209, two_by_two_cover_with_strip_and_recursion.cu:
+    25-105): the 2x2 cover of r1..r4 covers the rectangle, or the strip it
+    leaves is covered by recursion from r5, r6 or r7 on, or r5 and r6
+    placed against its right end with the disk of the 2x2 cover."""
+    width, circles = two_by_two_cover(D, r)
+    if width <= 0:  # both columns left out (decided inside)
+        return False, 0
+    if D.cmp(width, ">", la):
+        return True, 1
+    R = max(rsb_R(la, r), mpf(0))
+    w = la - width
+    R5 = la * CR - sum(r[:4])
+    R6 = R5 - r[4]
+    if rsb_can_recurse(D, R5, r[4], w, mpf(1)):
+        return True, 2
+    if D.cmp(4 * r[4], "<=", w * w):
+        return False, 0
+    h5 = mpmath.sqrt(4 * r[4] - w * w)
+    hrem = 1 - h5
+    if D.cmp(h5, ">=", 1) or rsb_can_recurse(D, R6, r[5], w, hrem):
+        return True, 3
+    if not D.cmp(4 * r[5], "<=", w * w):
+        h6 = mpmath.sqrt(4 * r[5] - w * w)
+        if D.cmp(h6, ">=", hrem) or rsb_can_recurse(D, R, r[5], w, hrem - h6):
+            return True, 4
+    if not D.cmp(4 * r[5], "<=", hrem * hrem):
+        w6 = mpmath.sqrt(4 * r[5] - hrem * hrem)
+        if D.cmp(w6, ">=", w):
+            return True, 5
+        c5 = (la - w / 2, h5 / 2, r[4])
+        c6 = (la - w6 / 2, 1 - hrem / 2, r[5])
+        ok, p0, p1 = circle_intersection(D, c5, c6)
+        c4 = circles[3]
+        if ok and D.cmp(p0[0], "<", p1[0]) and D.cmp(dist2(p0, c4[:2]), "<=", c4[2]):
+            ok46, q0, q1 = circle_intersection(D, c6, c4)
+            if ok46:
+                lb_y = q0[1] if D.cmp(q0[1], ">", q1[1]) else q1[1]
+                if rsb_can_recurse(D, R, r[5], w - w6, 1 - lb_y):
+                    return True, 6
+    return False, 0
+
+
+RSB_STEPS = {6: rsb_c1_even_split, 7: rsb_c2_uneven_split, 8: rsb_c3_multi_disk_strip, 9: rsb_c4_wall_building, 10: rsb_c5_r1_in_corner, 14: rsb_c9_l_shaped, 15: rsb_c10_r1_corner_wall_building, 12: rsb_c7_vertical_strip, 13: rsb_c8_two_by_two}
 
 
 def rsb_steps(la, r):

@@ -235,7 +235,8 @@ def test_planted_slips_are_caught(pinned):
 
 # --------------------------------------------- rectangle_size_bound leaf steps
 
-RSB_STRATEGIES = {6: {1, 2}, 7: {1, 2}, 8: {1, 2, 3, 4}, 9: {1}, 10: {1, 2}, 12: {1, 3}}
+RSB_STRATEGIES = {6: {1, 2}, 7: {1, 2}, 8: {1, 2, 3, 4}, 9: {1}, 10: {1, 2}, 12: {1, 3}, 13: {1, 2, 3, 4},
+                  14: {1, 3, 4}, 15: {1, 2}}
 
 # Cells whose C7 (r1_r2_vertical_strip) is decided by its last recursion, with
 # all six disks placed (found by a seeded random search; radii beyond 9/64,
@@ -267,9 +268,11 @@ def rsb_point_cells():
 def test_rsb_leaf_steps_equal_exact_at_clear_slack():
     """The leaf-chain steps C1 (even split), C2 (uneven split), C3 (multi-disk
     strips), C4 (vertical wall building), C5 (r1 in a corner with explicit
-    recursion) and C7 (r1, r2 strip and greedy strips), each evaluated alone
-    on point cells by the oracle's per-step masks, against their exact
-    statement; planted slips in C2, C3, C4 and C5 are caught."""
+    recursion), C7 (r1, r2 strip and greedy strips), C8 (2x2 cover with strip
+    and recursion), C9 (L-shaped recursion) and C10 (r1 in a corner with wall
+    building), each evaluated alone on point cells by the oracle's per-step
+    masks, against their exact statement; a planted slip in each of C2, C3,
+    C4, C5, C8, C9 and C10 is caught."""
     P = rsb_point_cells()
     cells = np.zeros((len(P), 14))
     cells[:, 0] = cells[:, 1] = P[:, 0]
@@ -351,7 +354,32 @@ def test_rsb_leaf_steps_equal_exact_at_clear_slack():
         finally:
             ce._c5_max_remaining = saved
 
-    for k, f in ((7, c2_index), (9, _c4_sub), (8, c3_single_root), (10, c5_index)):
+    def c8_index(D, la, r):  # C8's recursion from r5 on with r6 as its largest disk
+        saved = ce.rsb_can_recurse
+        ce.rsb_can_recurse = lambda D_, rem, largest, w, h: saved(D_, rem, r[5] if largest is r[4] else largest, w, h)
+        try:
+            return ce.rsb_c8_two_by_two(D, la, r)
+        finally:
+            ce.rsb_can_recurse = saved
+
+    def c9_r5(D, la, r):  # C9's two-horizontal recursion with r5 as the largest disk (the quirk "fixed")
+        saved = ce.rsb_can_recurse
+        ce.rsb_can_recurse = lambda D_, rem, largest, w, h: saved(D_, rem, r[4] if largest is r[5] else largest, w, h)
+        try:
+            return ce.rsb_c9_l_shaped(D, la, r)
+        finally:
+            ce.rsb_can_recurse = saved
+
+    def c10_factor(D, la, r):  # C10 with the wall-building weight bound's factor 1/2 as 1/4
+        saved = ce.WEIGHT_BOUND_WALL_BUILDING
+        ce.WEIGHT_BOUND_WALL_BUILDING = saved / 2
+        try:
+            return ce.rsb_c10_r1_corner_wall_building(D, la, r)
+        finally:
+            ce.WEIGHT_BOUND_WALL_BUILDING = saved
+
+    for k, f in ((7, c2_index), (9, _c4_sub), (8, c3_single_root), (10, c5_index), (13, c8_index), (14, c9_r5),
+                 (15, c10_factor)):
         caught = False
         for i, p in enumerate(P):
             D = ce.Dec()
