@@ -695,7 +695,7 @@ def _chords_dec(D, d, x):
 def _sup_decreasing(f, a, b):
     """The point where the continuous non-increasing f (f(a) >= 0 > f(b))
     crosses zero, by bisection to far below any compared slack."""
-    for _ in range(220):
+    for _ in range(120):  # 2^-120 of the interval: far below any compared slack
         m = (a + b) / 2
         if f(m) >= 0:
             a = m
@@ -1031,7 +1031,325 @@ def rsb_c8_two_by_two(D, la, r):
     return False, 0
 
 
-RSB_STEPS = {6: rsb_c1_even_split, 7: rsb_c2_uneven_split, 8: rsb_c3_multi_disk_strip, 9: rsb_c4_wall_building, 10: rsb_c5_r1_in_corner, 14: rsb_c9_l_shaped, 15: rsb_c10_r1_corner_wall_building, 12: rsb_c7_vertical_strip, 13: rsb_c8_two_by_two}
+_SQRT_4_M_INV_CRSQ = mpmath.sqrt(4 - 1 / (CR * CR))
+
+
+def efficient_cover_weight_range(w, h):
+    """compute_efficient_rectangle_cover_weight_range (compute_efficient_rectangle_cover_weight_range.cu:
+    25-35): the squared radii of a disk that covers a w x h rectangle (h the
+    shorter side) at density 0.61 or better."""
+    lb = CR * CR * h * h * (2 - _SQRT_4_M_INV_CRSQ)
+    ub = min(CR * h * w, CR * CR * h * h * (2 + _SQRT_4_M_INV_CRSQ))
+    return lb, ub
+
+
+def _c6_size_bounds(D, la, r, wa, ha, wb, hb, start):
+    """r1_r2_opposite_corners_recursion_check_size_bounds (r1_r2_opposite_corners_recursion.cu:
+    181-247): the disks from `start` on fill the region of the larger
+    shorter side up to its weight, then the other; the extra cost of the
+    split on each.  Returns (works, cost for the larger, cost for the smaller)."""
+    min_a, min_b = min(wa, ha), min(wb, hb)
+    a_smaller = D.cmp(min_a, "<", min_b)
+    bigger_min, smaller_min = (min_b, min_a) if a_smaller else (min_a, min_b)
+    bigger_area, smaller_area = (wb * hb, wa * ha) if a_smaller else (wa * ha, wb * hb)
+    r7 = min(r[5], rsb_R(la, r))
+    if start < 6:
+        if not size_bound_ok(D, r[start], bigger_min):
+            return False, 0, 0
+        acc = mpf(0)
+        while start < 6:
+            acc += r[start]
+            if D.cmp(acc, ">=", CR * bigger_area):
+                break
+            start += 1
+    if start < 6:
+        if not size_bound_ok(D, r[start], smaller_min):
+            return False, 0, 0
+        cost_bigger = r[start]
+        acc = mpf(0)
+        while start < 6:
+            acc += r[start]
+            if D.cmp(acc, ">=", CR * smaller_area):
+                break
+            start += 1
+        return True, cost_bigger, (r[start] if start < 6 else r7)
+    if not size_bound_ok(D, r7, smaller_min):
+        return False, 0, 0
+    return True, r7, r7
+
+
+def rsb_c6_opposite_corners(D, la, r):
+    """C6, r1_r2_opposite_corners_recursion (r1_r2_opposite_corners_recursion.cu:
+    249-340): r1 and r2 as squares in opposite corners, not meeting and not
+    covering the height; the regions A and B beside them by recursion, the
+    gap C between them filled with disks that cover it efficiently, or by
+    worst-case recursion."""
+    w1, w2 = mpmath.sqrt(2 * r[0]), mpmath.sqrt(2 * r[1])
+    comb = w1 + w2
+    extra12 = (2 * CR - 1) * (r[0] + r[1])
+    if not D.cmp(comb, "<", 1):
+        return False, 0
+    wa, wb, ha, hb = la - w2, la - w1, 1 - w1, w1
+    wc, hc = w2, 1 - comb
+    shorter, longer = (wc, hc) if D.cmp(wc, "<=", hc) else (hc, wc)
+    lo, hi = efficient_cover_weight_range(longer, shorter)
+    md = 2
+    while md < 6:
+        if D.cmp(lo, ">", r[md]) or D.cmp(r[md], ">", hi):
+            break
+        if D.cmp(4 * r[md], "<=", shorter * shorter):
+            break
+        covered = mpmath.sqrt(4 * r[md] - shorter * shorter)
+        if D.cmp(longer, "<", covered):  # C is covered at density 0.61
+            ok, cost_big, _ = _c6_size_bounds(D, la, r, wa, ha, wb, hb, md + 1)
+            return (True, 1) if ok and D.cmp(cost_big, "<=", extra12) else (False, 0)
+        longer -= covered
+        if not D.cmp(shorter, "<=", longer):
+            shorter, longer = longer, shorter
+        lo, hi = efficient_cover_weight_range(longer, shorter)
+        md += 1
+    ok, cost_big, cost_small = _c6_size_bounds(D, la, r, wa, ha, wb, hb, md)
+    if ok and size_bound_ok(D, cost_small, shorter):
+        return (True, 2) if D.cmp(cost_small + cost_big, "<=", extra12) else (False, 0)
+    cr_c = bound_worst_case_ratio(D, wc, hc)
+    big = mpf(0)
+    while md < 6:
+        big += r[md]
+        if D.cmp(big, ">=", cr_c * wc * hc):
+            break
+        md += 1
+    ok, cost_big, cost_small = _c6_size_bounds(D, la, r, wa, ha, wb, hb, md)
+    if ok:
+        cost = cost_small + cost_big + (cr_c - CR) * wc * hc
+        return (True, 3) if D.cmp(cost, "<=", extra12) else (False, 0)
+    return False, 0
+
+
+def rsb_c11_advanced_strip(D, la, r):
+    """C11, advanced_multi_disk_strip_vertical (multi_disk_strip.cu:264-317):
+    any proper subset of r1..r6 stacked in a full-height strip of the width
+    its weight needs at 0.61, each disk spanning it; the rest for the
+    largest disk not in the subset."""
+    for mask in range(1, 63):
+        inside = [j for j in range(6) if mask >> j & 1]
+        w = sum(r[j] for j in inside) / CR
+        if any(D.cmp(4 * r[j], "<=", w * w) for j in inside):
+            continue
+        if D.cmp(sum(mpmath.sqrt(4 * r[j] - w * w) for j in inside), "<", 1):
+            continue
+        largest = r[min(j for j in range(6) if not mask >> j & 1)]
+        if _rem_width_ok(D, la, w, largest):
+            return True, 1
+    return False, 0
+
+
+def max_height_strip(D, la, r1, r2):
+    """two_disks_maximal_height_strip (maximize_covered.cu:132-150): r1 and r2
+    side by side at the bottom of a strip of width la, as high as both
+    reach; (height, circle 1, circle 2), height 0 if one cannot."""
+    wd = 2 * (r1 - r2) / la
+    w1, w2 = la / 2 + wd, la / 2 - wd
+    if D.cmp(r1, "<=", w1 * w1 / 4) or D.cmp(r2, "<=", w2 * w2 / 4):
+        return mpf(0), (mpf(0), mpf(0), r1), (mpf(0), mpf(0), r2)
+    h = mpmath.sqrt(r1 - w1 * w1 / 4)
+    return 2 * h, (w1 / 2, h, r1), (la - w2 / 2, h, r2)
+
+
+def _trap(D):
+    """A reference trap site: the cell is left out of the comparison."""
+    D.slack = mpf(0)
+    return False
+
+
+def _c12_center(D, la, r):
+    """five_disk_full_cover_3v2_recursion_center (five_disk_full_cover.cu:92-171)."""
+    R6 = r[5] + max(rsb_R(la, r), mpf(0))
+    height, c1, c2 = max_height_strip(D, la, r[0], r[1])
+    if height <= 0:
+        return 0
+    hrem = 1 - height
+    if D.cmp(4 * r[3], "<=", hrem * hrem) or D.cmp(4 * r[4], "<=", hrem * hrem):
+        return 0
+    w4, w5 = mpmath.sqrt(4 * r[3] - hrem * hrem), mpmath.sqrt(4 * r[4] - hrem * hrem)
+    y45 = 1 - hrem / 2
+    c4, c5 = (la - w4 / 2, y45, r[3]), (w5 / 2, y45, r[4])
+    wrem = la - (w4 + w5)
+    if D.cmp(4 * r[2], "<=", wrem * wrem):
+        return 0
+    h3 = mpmath.sqrt(4 * r[2] - wrem * wrem)
+    if D.cmp(h3, ">=", hrem):
+        return 1
+    c3 = (w5 + wrem / 2, 1 - h3 / 2, r[2])
+    ok15, p15, q15 = circle_intersection(D, c1, c5)
+    ok12, p12, q12 = circle_intersection(D, c1, c2)
+    ok24, p24, q24 = circle_intersection(D, c2, c4)
+    if not (ok12 and ok15 and ok24):
+        return _trap(D)
+    if D.cmp(p15[0], ">=", q15[0]) or D.cmp(p24[0], ">=", q24[0]):
+        return 0
+    top12 = p12 if D.cmp(p12[1], ">", q12[1]) else q12
+    in12 = D.cmp(dist2(top12, c3[:2]), "<=", r[2])
+    in15 = D.cmp(dist2(q15, c3[:2]), "<=", r[2])
+    in24 = D.cmp(dist2(p24, c3[:2]), "<=", r[2])
+    if in12 and in15 and in24:
+        return 2
+    if in15 and in24:
+        ok31, p31, q31 = circle_intersection(D, c1, c3)
+        ok32, p32, q32 = circle_intersection(D, c2, c3)
+        if ok31 and ok32 and D.cmp(p31[0], "<", q31[0]) and D.cmp(p32[0], "<", q32[0]):
+            pw = p32[0] - q31[0]
+            ph = max(q31[1], p32[1]) - top12[1]
+            return 3 if rsb_can_recurse(D, R6, r[5], pw, ph) else 0
+    return 0
+
+
+def _c12_border32(D, la, r, t1, t2, t3, a, b):
+    """five_disk_full_cover_border_cover32_concrete (five_disk_full_cover.cu:173-194)."""
+    htop = nd_maximize_covered(D, la, [r[t1], r[t2], r[t3]])
+    if htop <= 0:
+        return False
+    hrem = 1 - htop
+    wright = two_disks_maximize_height(D, r[a], r[b], hrem)
+    if wright <= 0:
+        return False
+    return rsb_can_recurse(D, la * CR - sum(r[:5]), r[5], la - wright, hrem)
+
+
+def rsb_c12_five_disk(D, la, r):
+    """C12, five_disk_full_cover (five_disk_full_cover.cu:200-206)."""
+    s = _c12_center(D, la, r)
+    if s:
+        return True, s
+    if _c12_border32(D, la, r, 0, 1, 2, 3, 4) or _c12_border32(D, la, r, 0, 1, 4, 2, 3):
+        return True, 4
+    return False, 0
+
+
+def can_recurse_worst_case(D, rem, w, h):
+    """can_recurse_worst_case (can_recurse.cu:237-261)."""
+    lng, sht = (h, w) if D.cmp(w, "<", h) else (w, h)
+    if D.cmp(lng / sht, "<=", TOL_LAMBDA_RSB):
+        return D.cmp(w * h * C_WORST, "<=", rem)
+    return D.cmp(sht * sht / 2 + lng * lng / 4, "<=", rem)
+
+
+def max_height_strip_wc(D, r1, r2, R, w):
+    """two_disks_maximal_height_strip_wc_recursion (maximize_covered.cu:239-278):
+    the greatest height h below 2 r2 (the reference's bound) at which r1 and
+    r2 side by side at the bottom of the w-wide strip leave a remainder that
+    worst-case recursion covers with weight R; (height, circle 1, circle 2),
+    height 0 if none.  The test passes, then fails, as h grows."""
+    def test(h, Dx):
+        if 4 * r1 < h * h or 4 * r2 < h * h:
+            return False
+        wrem = w - (mpmath.sqrt(4 * r1 - h * h) + mpmath.sqrt(4 * r2 - h * h))
+        return wrem <= 0 or can_recurse_worst_case(Dx, R, wrem, h)
+    lo, hi = mpf(0), 2 * r2
+    for _ in range(120):
+        m = (lo + hi) / 2
+        if test(m, Dec()):
+            lo = m
+        else:
+            hi = m
+    if lo <= 0 or D.cmp(lo, "<=", mpf(10) ** -40):
+        return mpf(0), (mpf(0), mpf(0), r1), (mpf(0), mpf(0), r2)
+    w1, w2 = mpmath.sqrt(4 * r1 - lo * lo), mpmath.sqrt(4 * r2 - lo * lo)
+    return lo, (w1 / 2, lo / 2, r1), (w - w2 / 2, lo / 2, r2)
+
+
+def _c13_r12_r456_r3(D, la, r):
+    """six_disk_full_cover_r12_r456_r3_recursion (six_disk_full_cover.cu:168-193)."""
+    w12 = two_disks_maximize_height(D, r[0], r[1], mpf(1))
+    if w12 <= 0:
+        return False
+    w456 = nd_maximize_covered(D, mpf(1), [r[3], r[4], r[5]])
+    if w456 <= 0:
+        return False
+    wrem = la - (w12 + w456)
+    if D.cmp(wrem, "<=", 0):
+        return True
+    if D.cmp(4 * r[2], "<=", wrem * wrem):
+        return False
+    hrem = 1 - mpmath.sqrt(4 * r[2] - wrem * wrem)
+    return rsb_can_recurse(D, max(rsb_R(la, r), mpf(0)), r[5], wrem, hrem)
+
+
+def _c13_4_plus_2(D, la, r):
+    """six_disk_full_cover_4_plus_2 (six_disk_full_cover.cu:27-104): r1, r2 at
+    the bottom, r3, r4 in the top corners, r5 and r6 over the two gaps."""
+    height, c1, c2 = max_height_strip(D, la, r[0], r[1])
+    if height <= 0:
+        return False
+    hrem = 1 - height
+    if D.cmp(r[3], "<=", hrem * hrem / 4) or D.cmp(r[2], "<=", hrem * hrem / 4):
+        return False
+    w4, w3 = mpmath.sqrt(r[3] - hrem * hrem / 4), mpmath.sqrt(r[2] - hrem * hrem / 4)
+    y = 1 - hrem / 2
+    c4, c3 = (w4, y, r[3]), (la - w3, y, r[2])
+    ok34, p34, q34 = circle_intersection(D, c3, c4)
+    if not ok34:
+        return False
+    up_top, low_top = (q34, p34) if D.cmp(p34[1], "<", q34[1]) else (p34, q34)
+    ok12, p12, q12 = circle_intersection(D, c1, c2)
+    if not ok12:
+        return _trap(D)
+    up_bottom = q12 if D.cmp(p12[1], "<", q12[1]) else p12
+    ok14, p14, q14 = circle_intersection(D, c1, c4)
+    ok23, p23, q23 = circle_intersection(D, c2, c3)
+    if not (ok14 and ok23):
+        return False
+    top_c = (up_top[0], (1 + up_top[1]) / 2)
+    top = [dist2(top_c, (2 * w4, mpf(1))), dist2(top_c, (la - 2 * w3, mpf(1))), dist2(top_c, up_top)]
+    bottom_c = ((low_top[0] + up_bottom[0]) / 2, (low_top[1] + up_bottom[1]) / 2)
+    bottom = [dist2(bottom_c, q14), dist2(bottom_c, p23), dist2(bottom_c, low_top), dist2(bottom_c, up_bottom)]
+    if all(D.cmp(d, "<=", r[5]) for d in top):
+        return all(D.cmp(d, "<=", r[4]) for d in bottom)
+    if all(D.cmp(d, "<=", r[4]) for d in top):
+        return all(D.cmp(d, "<=", r[5]) for d in bottom)
+    return False
+
+
+def _c13_2v3(D, la, r):
+    """six_disk_full_cover_2v3_with_recursion_r6_center (six_disk_full_cover.cu:106-166)."""
+    height, c1, c2 = max_height_strip_wc(D, r[0], r[1], max(rsb_R(la, r), mpf(0)), la)
+    if height <= 0:
+        return False
+    hrem = 1 - height
+    if D.cmp(4 * r[2], "<=", hrem * hrem) or D.cmp(4 * r[3], "<=", hrem * hrem):
+        return False
+    w3, w4 = mpmath.sqrt(4 * r[2] - hrem * hrem), mpmath.sqrt(4 * r[3] - hrem * hrem)
+    wrem = la - (w3 + w4)
+    if D.cmp(wrem, "<=", 0):
+        return True
+    if D.cmp(r[4], "<=", wrem * wrem / 4):
+        return False
+    dy5 = mpmath.sqrt(r[4] - wrem * wrem / 4)
+    if D.cmp(2 * dy5 + height, ">=", 1):
+        return True
+    c5 = (w4 + wrem / 2, 1 - dy5, r[4])
+    ok15, p15_, q15 = circle_intersection(D, c1, c5)
+    ok25, p25, q25_ = circle_intersection(D, c2, c5)
+    if not (ok15 and ok25):
+        return False
+    if D.cmp(q15[1], "<=", height) and D.cmp(p25[1], "<=", height):
+        return True
+    wrem6 = p25[0] - q15[0]
+    hrem6 = max(q15[1], p25[1]) - height
+    return D.cmp(wrem6 * wrem6 + hrem6 * hrem6, "<=", 4 * r[5])
+
+
+def rsb_c13_six_disk(D, la, r):
+    """C13, six_disk_full_cover (six_disk_full_cover.cu:195-200)."""
+    if _c13_r12_r456_r3(D, la, r):
+        return True, 1
+    if _c13_4_plus_2(D, la, r):
+        return True, 2
+    if _c13_2v3(D, la, r):
+        return True, 3
+    return False, 0
+
+
+RSB_STEPS = {6: rsb_c1_even_split, 7: rsb_c2_uneven_split, 8: rsb_c3_multi_disk_strip, 9: rsb_c4_wall_building, 10: rsb_c5_r1_in_corner, 11: rsb_c6_opposite_corners, 14: rsb_c9_l_shaped, 15: rsb_c10_r1_corner_wall_building, 16: rsb_c11_advanced_strip, 17: rsb_c12_five_disk, 18: rsb_c13_six_disk, 12: rsb_c7_vertical_strip, 13: rsb_c8_two_by_two}
 
 
 def rsb_steps(la, r):

@@ -235,8 +235,8 @@ def test_planted_slips_are_caught(pinned):
 
 # --------------------------------------------- rectangle_size_bound leaf steps
 
-RSB_STRATEGIES = {6: {1, 2}, 7: {1, 2}, 8: {1, 2, 3, 4}, 9: {1}, 10: {1, 2}, 12: {1, 3}, 13: {1, 2, 3, 4},
-                  14: {1, 3, 4}, 15: {1, 2}}
+RSB_STRATEGIES = {6: {1, 2}, 7: {1, 2}, 8: {1, 2, 3, 4}, 9: {1}, 10: {1, 2}, 11: {1, 2, 3}, 12: {1, 3},
+                  13: {1, 2, 3, 4}, 14: {1, 3, 4}, 15: {1, 2}, 16: {1}, 17: {1, 4}, 18: {1, 3}}
 
 # Cells whose C7 (r1_r2_vertical_strip) is decided by its last recursion, with
 # all six disks placed (found by a seeded random search; radii beyond 9/64,
@@ -266,13 +266,15 @@ def rsb_point_cells():
 
 
 def test_rsb_leaf_steps_equal_exact_at_clear_slack():
-    """The leaf-chain steps C1 (even split), C2 (uneven split), C3 (multi-disk
-    strips), C4 (vertical wall building), C5 (r1 in a corner with explicit
-    recursion), C7 (r1, r2 strip and greedy strips), C8 (2x2 cover with strip
-    and recursion), C9 (L-shaped recursion) and C10 (r1 in a corner with wall
-    building), each evaluated alone on point cells by the oracle's per-step
-    masks, against their exact statement; a planted slip in each of C2, C3,
-    C4, C5, C8, C9 and C10 is caught."""
+    """The leaf-chain steps C1..C13 (C14 excepted), each evaluated alone on
+    point cells by the oracle's per-step masks, against their exact
+    statement: C1 even split, C2 uneven split, C3 multi-disk strips, C4
+    vertical wall building, C5 r1 in a corner with explicit recursion, C6 r1
+    and r2 in opposite corners, C7 r1, r2 strip and greedy strips, C8 2x2
+    cover with strip and recursion, C9 L-shaped recursion, C10 r1 in a
+    corner with wall building, C11 advanced multi-disk strip, C12 five-disk
+    and C13 six-disk full covers; a planted slip in each of C2..C13 but C7
+    is caught."""
     P = rsb_point_cells()
     cells = np.zeros((len(P), 14))
     cells[:, 0] = cells[:, 1] = P[:, 0]
@@ -378,8 +380,47 @@ def test_rsb_leaf_steps_equal_exact_at_clear_slack():
         finally:
             ce.WEIGHT_BOUND_WALL_BUILDING = saved
 
-    for k, f in ((7, c2_index), (9, _c4_sub), (8, c3_single_root), (10, c5_index), (13, c8_index), (14, c9_r5),
-                 (15, c10_factor)):
+    def c6_index(D, la, r):  # C6's regions A and B filled from one disk too far on
+        saved = ce._c6_size_bounds
+        ce._c6_size_bounds = lambda D_, la_, r_, wa, ha, wb, hb, start: saved(D_, la_, r_, wa, ha, wb, hb,
+                                                                              min(start + 1, 6))
+        try:
+            return ce.rsb_c6_opposite_corners(D, la, r)
+        finally:
+            ce._c6_size_bounds = saved
+
+    def c11_all_six(D, la, r):  # C11 with the set of all six disks allowed (r6 taken as the rest's largest)
+        v, st = ce.rsb_c11_advanced_strip(D, la, r)
+        if v:
+            return v, st
+        w = sum(r) / ce.CR
+        if any(D.cmp(4 * x, "<=", w * w) for x in r):
+            return False, 0
+        if D.cmp(sum(mpmath.sqrt(4 * x - w * w) for x in r), "<", 1):
+            return False, 0
+        return (True, 1) if ce._rem_width_ok(D, la, w, r[5]) else (False, 0)
+
+    def c12_weight(D, la, r):  # C12's border covers with r5 left in the remaining weight
+        saved = ce.rsb_can_recurse
+
+        def slipped(D_, rem, largest, w, h):
+            return saved(D_, rem + r[4] if largest is r[5] else rem, largest, w, h)
+        ce.rsb_can_recurse = slipped
+        try:
+            return ce.rsb_c12_five_disk(D, la, r)
+        finally:
+            ce.rsb_can_recurse = saved
+
+    def c13_index(D, la, r):  # C13's recursions with r5 as the largest disk
+        saved = ce.rsb_can_recurse
+        ce.rsb_can_recurse = lambda D_, rem, largest, w, h: saved(D_, rem, r[4] if largest is r[5] else largest, w, h)
+        try:
+            return ce.rsb_c13_six_disk(D, la, r)
+        finally:
+            ce.rsb_can_recurse = saved
+
+    for k, f in ((7, c2_index), (9, _c4_sub), (8, c3_single_root), (10, c5_index), (11, c6_index), (13, c8_index),
+                 (14, c9_r5), (15, c10_factor), (16, c11_all_six), (17, c12_weight), (18, c13_index)):
         caught = False
         for i, p in enumerate(P):
             D = ce.Dec()
