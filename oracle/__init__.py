@@ -53,6 +53,8 @@ def _load(name: str) -> ctypes.CDLL:
     L.or_tr_bisect.restype = None
     L.or_tr_merge_adjacent_r1.argtypes = [_vp, _sz, _vp]
     L.or_tr_merge_adjacent_r1.restype = _sz
+    L.or_tr_leaf_masks.argtypes = [_vp, _sz, _vp]
+    L.or_tr_leaf_masks.restype = None
     return L
 
 
@@ -147,6 +149,15 @@ def tr_bisect(cells: np.ndarray) -> np.ndarray:
     cells = np.ascontiguousarray(cells, dtype=np.float64).reshape(-1, 10)
     out = np.empty((2 * cells.shape[0], 10))
     lib.or_tr_bisect(cells.ctypes.data, cells.shape[0], out.ctypes.data)
+    return out
+
+
+def tr_leaf_masks(cells: np.ndarray) -> np.ndarray:
+    """Leaf-criterion bits of tight_rectangle leaves ((n, 10) records): see
+    or_tr_leaf_masks."""
+    cells = np.ascontiguousarray(cells, dtype=np.float64)
+    out = np.zeros(cells.shape[0], dtype=np.uint32)
+    lib.or_tr_leaf_masks(cells.ctypes.data, cells.shape[0], out.ctypes.data)
     return out
 
 

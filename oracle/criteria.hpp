@@ -41,6 +41,7 @@ bool r1_r2_r3_strategies(const IV& la, const IV& r1, const IV& r2, const IV& r3,
 bool r1_r2_r3_strategies(const Cell& c, const IV& R4);
 bool two_by_two_strategies(const Cell& c, const IV& R);
 bool r1_r2_large_r3_r4_gaps_strategy(const Cell& c, const IV& R);
+uint32_t leaf_strategy_bits(const Cell& c, const IV& R4, const IV& R);
 IV r1_range(const IV& la);
 IV r2_range(const IV& la, const IV& r1);
 IV r3_range(const IV& la, const IV& r1, const IV& r2);

@@ -453,6 +453,28 @@ bool r1_r2_large_r3_r4_gaps_strategy(const Cell& c, const IV& R) {
   return can_recurse(wrem, 1.0, ub_r5, R.lb);
 }
 
+// The leaf criteria of one tight_rectangle leaf, each evaluated on its own,
+// with the strategy inside each that decides (test support, no reference
+// counterpart): bit 0 two_by_two_strategies, 1 r1_r2_r3_strategies(vars, R4),
+// 2 r1_r2_large_r3_r4_gaps_strategy (tight_rectangle.cu:247-249); bit 8 the
+// two-by-two lambda_small placement, 9 lambda_big; 10 the three-disk big
+// strip, 11 small corner, 12 r1-r3 pocket, 13 manual three-disk pocket.
+uint32_t leaf_strategy_bits(const Cell& c, const IV& R4, const IV& R) {
+  const double la = c.la.ub, r1 = c.r[0].lb, r2 = c.r[1].lb, r3 = c.r[2].lb;
+  const double ub_r4 = c.r[3].ub, lb_R4 = R4.lb;
+  uint32_t m = 0;
+  if (two_by_two_lambda_small(c, R)) m |= 1u << 8;
+  if (two_by_two_lambda_big(c, R)) m |= 1u << 9;
+  if (three_disk_recursion_big_strip(la, r1, r2, r3, ub_r4, lb_R4)) m |= 1u << 10;
+  if (three_disk_recursion_small_corner(la, r1, r2, r3, ub_r4, lb_R4)) m |= 1u << 11;
+  if (three_disk_recursion_r1_r3_pocket(c.la, c.r[0], c.r[1], c.r[2], ub_r4, lb_R4)) m |= 1u << 12;
+  if (manually_proved_three_disk_pocket(c.la, c.r[0], c.r[1], c.r[2])) m |= 1u << 13;
+  if (two_by_two_strategies(c, R)) m |= 1u;
+  if (r1_r2_r3_strategies(c, R4)) m |= 2u;
+  if (r1_r2_large_r3_r4_gaps_strategy(c, R)) m |= 4u;
+  return m;
+}
+
 // Valid squared-radius ranges (tight_rectangle.cu:72-102).
 IV r1_range(const IV& la) {
   double f = div_rd(critical_ratio(la).lb, kRsbRatio);

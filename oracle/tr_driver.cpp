@@ -148,6 +148,28 @@ int or_tr_find_critical(const double* cells, size_t ncells, const int* n, double
   return 0;
 }
 
+// The leaf criteria of n tight_rectangle leaves (80-byte records), as the
+// enumeration forms them before its leaf test (tight_rectangle.cu:217-246):
+// R4 = weight - r1 - r2 - r3 and R = weight - r1 - .. - r4 (lower bounds
+// raised to 0) from the leaf's radii with their lower bounds tightened
+// downwards from r4.  out[i] = tr::leaf_strategy_bits, with bit 31 set when R
+// is empty (the enumeration skips such a leaf).
+void or_tr_leaf_masks(const double* cells, size_t n, uint32_t* out) {
+  for (size_t i = 0; i < n; ++i) {
+    tr::Cell c;
+    std::memcpy(&c, cells + 10 * i, sizeof c);
+    IV weight = tr::required_weight_for(c.la);
+    IV R4 = weight - c.r[0] - c.r[1] - c.r[2];
+    R4.tighten_lb(0.0);
+    c.r[2].tighten_lb(c.r[3].lb);
+    c.r[1].tighten_lb(c.r[2].lb);
+    c.r[0].tighten_lb(c.r[1].lb);
+    IV R = weight - c.r[0] - c.r[1] - c.r[2] - c.r[3];
+    R.tighten_lb(0.0);
+    out[i] = R.empty() || R4.empty() ? 1u << 31 : tr::leaf_strategy_bits(c, R4, R);
+  }
+}
+
 // The n pieces get_subinterval([lb, ub], i, n) as 2n doubles.
 void or_subintervals(double lb, double ub, int n, double* out) {
   for (int i = 0; i < n; ++i) {

@@ -1,6 +1,6 @@
-"""The ten criteria of the 4-D box set and the rectangle_size_bound leaf
-steps C1, C2, C3, C4, C5 and C7, decided in exact real arithmetic (test support,
-independent of oracle/ and of the kernels).
+"""The ten criteria of the 4-D box set, the rectangle_size_bound leaf
+steps C1..C13 and the tight_rectangle leaf criteria, decided in exact real
+arithmetic (test support, independent of oracle/ and of the kernels).
 
 Each function restates the mathematics of one reference criterion on a
 point cell (lambda, squared radii r1 >= r2 >= r3): the same placements,
@@ -27,7 +27,9 @@ compute_even_split_distance_bound.cu:25-76, even_split_recursion.cu:25-40,
 uneven_split_recursion.cu:25-76, vertical_wall_building_recursion.cu:25-78,
 r1_r2_vertical_strip.cu:43-95, multi_disk_strip.cu:31-262,
 r1_in_corner_explicit_recursion.cu:35-262, maximize_covered.cu:28-104,
-171-237; operations.cu:30-83.
+171-237; operations.cu:30-83; for the tight_rectangle leaf criteria
+tight_rectangle.cu:217-249, two_by_two_strategies.cu:29-115,
+r1_r2_r3_strategies.cu:101-260, r1_r2_large_r3_r4_gaps_strategy.cu:29-98.
 """
 from __future__ import annotations
 
@@ -1361,5 +1363,242 @@ def rsb_steps(la, r):
     for k, f in RSB_STEPS.items():
         D = Dec()
         v, s = f(D, la, r)
+        out[k] = (bool(v), s, D.slack)
+    return out
+
+
+# ------------------------------------------ tight_rectangle leaf criteria
+# Point leaves (lambda, r1 >= r2 >= r3 >= r4) as the enumeration forms them
+# (tight_rectangle.cu:217-246): W = critical_ratio(lambda) * lambda,
+# R4 = W - r1 - r2 - r3 and R = R4 - r4, both at least 0.
+
+def tr_leaf_weights(la, r):
+    W = tr_critical_ratio(la) * la
+    R4 = W - r[0] - r[1] - r[2]
+    return R4, R4 - r[3]
+
+
+def _tbt_r1_large(D, la, r, R):
+    """two_by_two_strategies.cu:29-51: r1 spans the height at width
+    sqrt(4 r1 - 1) on the left; r2, r3, r4 stacked in the strip to its right
+    cover heights sqrt(4 ri - wrem^2); recursion on the rest of that strip."""
+    w1 = mpmath.sqrt(4 * r[0] - 1)
+    wrem = la - w1
+    if D.cmp(4 * r[3], "<=", wrem * wrem):
+        return False
+    if min(r[1], r[2]) < r[3]:  # unsorted radii: the reference's square root is NaN, its tests false
+        return False
+    htot = sum(mpmath.sqrt(4 * x - wrem * wrem) for x in (r[1], r[2], r[3]))
+    if D.cmp(htot, ">=", 1):
+        return True
+    return tr_can_recurse(D, wrem, 1 - htot, min(r[3], R), R)
+
+
+def tbt_lambda_small(D, la, r, R):
+    """two_by_two_strategies.cu:53-83: r1's square in a corner (r1_large when
+    its side reaches the height), r2 in the strip to its right, r4 in the strip
+    above it, r3 covers the rectangle left in the far corner."""
+    side = mpmath.sqrt(2 * r[0])
+    if D.cmp(side, ">=", 1):
+        return _tbt_r1_large(D, la, r, R)
+    wrem, hrem = la - side, 1 - side
+    a, b = 4 * r[1] - wrem * wrem, 4 * r[3] - hrem * hrem
+    if D.cmp(a, "<=", 0) or D.cmp(b, "<=", 0):
+        return False
+    h2, w4 = min(mpmath.sqrt(a), side), min(mpmath.sqrt(b), side)
+    h, w = 1 - h2, la - w4
+    return D.cmp(4 * r[2], ">=", h * h + w * w)
+
+
+def tbt_lambda_big(D, la, r, R):
+    """two_by_two_strategies.cu:85-111: two stacked pairs side by side, the
+    best of the three pairings, recursion on the rest of the width."""
+    def th(a, b):
+        return two_disks_maximize_height(D, a, b, mpf(1))
+    w = max(th(r[0], r[1]) + th(r[2], r[3]), th(r[0], r[3]) + th(r[1], r[2]), th(r[0], r[2]) + th(r[1], r[3]))
+    if w == 0:  # every pairing decided 0 (its comparisons are recorded)
+        return False
+    return tr_can_recurse(D, la - w, mpf(1), min(R, r[3]), R)
+
+
+def tr_two_by_two(D, la, r, R):
+    """two_by_two_strategies (:113-115).  Strategies 1 lambda_small, 2 lambda_big."""
+    if tbt_lambda_small(D, la, r, R):
+        return True, 1
+    return (True, 2) if tbt_lambda_big(D, la, r, R) else (False, 0)
+
+
+def three_disk_big_strip(D, la, r1, r2, r3, ub_r4, R4):
+    """r1_r2_r3_strategies.cu:101-121: r2 and r3 cover a full-height strip
+    of width S (S^2 = 2 (r2 + r3) - 1/4 - 4 (r2 - r3)^2), r1 spans the
+    remaining width, recursion on the rest of the height."""
+    s_sq = 2 * r2 + 2 * r3 - mpf(1) / 4 - 4 * (r2 - r3) ** 2
+    if D.cmp(s_sq, "<=", 0):
+        return False
+    la_rem = la - mpmath.sqrt(s_sq)
+    h1_sq = 4 * r1 - la_rem * la_rem
+    if D.cmp(h1_sq, "<=", 0):
+        return False
+    if D.cmp(h1_sq, ">=", 1):
+        return True
+    return tr_can_recurse(D, la_rem, 1 - mpmath.sqrt(h1_sq), ub_r4, R4)
+
+
+def three_disk_small_corner(D, la, r1, r2, r3, ub_r4, R4):
+    """r1_r2_r3_strategies.cu:123-165: r3's square in a corner, r1 above it
+    and r2 beside it (or the other way round when r2 cannot reach half the
+    remaining width), recursion on the rectangle left.  The square root of
+    a negative difference is NaN in the reference, and a NaN width makes its
+    can_recurse false."""
+    c3 = mpmath.sqrt(2 * r3)
+    if D.cmp(c3, ">=", 1):
+        return False
+    la_rem, one_rem = la - c3, 1 - c3
+    if D.cmp(la_rem / 2, "<=", mpmath.sqrt(r2)):
+        a = 4 * r1 - one_rem * one_rem
+        if D.cmp(a, "<", 0):
+            return False
+        b = 4 * r2 - la_rem * la_rem
+        if D.cmp(b, "<", 0):
+            return False
+        return tr_can_recurse(D, la - mpmath.sqrt(a), 1 - mpmath.sqrt(b), ub_r4, R4)
+    if D.cmp(la_rem / 2, ">=", mpmath.sqrt(r1)):
+        return False
+    if D.cmp(one_rem, ">=", 2 * mpmath.sqrt(r2)):
+        return False
+    c2 = mpmath.sqrt(4 * r2 - one_rem * one_rem)
+    c1 = mpmath.sqrt(4 * r1 - la_rem * la_rem)
+    return tr_can_recurse(D, la - c2, 1 - c1, ub_r4, R4)
+
+
+def _pocket_X(D, la, r1, r2, r3):
+    """The three-disk pocket placement (r1_r2_r3_strategies.cu:167-186 and
+    :228-244): r1 spans the height at the left, r2 below-right of it, r3 in
+    the corner above r2; returns (S1, S3, X = lambda - S1 - S3) or None when a
+    disk cannot reach."""
+    if D.cmp(r1, "<=", mpf(1) / 4):
+        return None
+    s1 = 2 * mpmath.sqrt(r1 - mpf(1) / 4)
+    h2_rest = (la - s1) ** 2 / 4
+    if D.cmp(r2, "<=", h2_rest):
+        return None
+    h2 = 2 * mpmath.sqrt(r2 - h2_rest)
+    s3_rest = (1 - h2) ** 2 / 4
+    if D.cmp(r3, "<=", s3_rest):
+        return None
+    s3 = 2 * mpmath.sqrt(r3 - s3_rest)
+    return s1, s3, la - s1 - s3
+
+
+def three_disk_r1_r3_pocket(D, la, r1, r2, r3, ub_r4, R4):
+    """r1_r2_r3_strategies.cu:167-206: the pocket placement; the X x Y
+    rectangle between r1 and r3 is left to recursion."""
+    p = _pocket_X(D, la, r1, r2, r3)
+    if p is None:
+        return False
+    s1, s3, X = p
+    if D.cmp(X, "<=", 0):
+        return True
+    y_rest = (la - s1 / 2 - s3) ** 2
+    if D.cmp(r1, "<=", y_rest):
+        return False
+    Y = mpf(1) / 2 - mpmath.sqrt(r1 - y_rest)
+    if D.cmp(Y, "<=", 0):
+        return True
+    return tr_can_recurse(D, X, Y, ub_r4, R4)
+
+
+def manual_three_disk_pocket(D, la, r1, r2, r3):
+    """r1_r2_r3_strategies.cu:208-247: the manually proved neighbourhood of
+    the three-disk worst case (lambda <= 1.05, each ri within 0.001, 0.002,
+    0.003 of r*(lambda) = lambda^2/16 + 5/32 + 9/(256 lambda^2), the pocket's
+    X at most 0.047); its constants are the doubles the reference uses."""
+    if D.cmp(la, ">", mpf(1.0499999999999998)):
+        return False
+    lasq = la * la
+    rstar = lasq / 16 + mpf(0.15625) + mpf(0.03515625) / lasq
+    for x, eps in ((r1, mpf(0.0009999999999999998)), (r2, mpf(0.0019999999999999996)),
+                   (r3, mpf(0.0029999999999999996))):
+        d = rstar - x
+        if D.cmp(d, "<", -eps) or D.cmp(d, ">", eps):
+            return False
+    p = _pocket_X(D, la, r1, r2, r3)
+    if p is None:
+        return False
+    return not D.cmp(p[2], ">", mpf(0.046999999999999993))
+
+
+def tr_three_disk(D, la, r, R4):
+    """r1_r2_r3_strategies(vars, R4) (:249-260).  Strategies: 1 big strip, 2
+    small corner, 3 r1-r3 pocket, 4 the manual pocket."""
+    r1, r2, r3, ub_r4 = r
+    fns = (lambda: three_disk_big_strip(D, la, r1, r2, r3, ub_r4, R4),
+           lambda: three_disk_small_corner(D, la, r1, r2, r3, ub_r4, R4),
+           lambda: three_disk_r1_r3_pocket(D, la, r1, r2, r3, ub_r4, R4),
+           lambda: manual_three_disk_pocket(D, la, r1, r2, r3))
+    for s, f in enumerate(fns, 1):
+        if f():
+            return True, s
+    return False, 0
+
+
+def _gap_ok(D, gap, r1, w1, r2, w2, r4):
+    """can_realize_gap_width (r1_r2_large_r3_r4_gaps_strategy.cu:29-57): r1
+    and r2 span the height side by side at widths w1, w2 with a gap between
+    them; r4, centred on the gap at the bottom edge's height sqrt(r4 - gap^2/4),
+    reaches the lower intersection point of r1 and r2."""
+    y4_sq = r4 - gap * gap / 4
+    if D.cmp(y4_sq, "<=", 0):
+        return False
+    d2 = (w2 / 2 + gap + w1 / 2) ** 2
+    rd = r1 - r2
+    if D.cmp(2 * (r1 + r2) / d2, "<", 1 + rd * rd / (d2 * d2)):
+        return False
+    # the circles' centres are level, so the first point is the lower one
+    ok, p, _ = circle_intersection(Dec(), (w1 / 2, mpf(1) / 2, r1), (w2 / 2 + gap + w1, mpf(1) / 2, r2))
+    return D.cmp(dist2((gap / 2 + w1, mpmath.sqrt(y4_sq)), p), "<=", r4)
+
+
+def tr_gaps(D, la, r, R):
+    """r1_r2_large_r3_r4_gaps_strategy (:59-98): r1 and r2 span the height
+    with the widest gap between them that r4 closes (the reference's
+    bisection over [0, sqrt(r1) + sqrt(r2)], on exact midpoints; the tests of
+    its first levels, where the bracket is still wide, count towards the
+    slack), recursion on the rest of the width."""
+    r1, r2, r4 = r[0], r[1], r[3]
+    if D.cmp(4 * r1, "<=", 1) or D.cmp(4 * r2, "<=", 1):
+        return False, 0
+    w1, w2 = mpmath.sqrt(4 * r1 - 1), mpmath.sqrt(4 * r2 - 1)
+    lo, hi = mpf(0), mpmath.sqrt(r1) + mpmath.sqrt(r2)
+    wide = hi * mpf("1e-6")
+    for _ in range(140):
+        mid = (lo + hi) / 2
+        if _gap_ok(D if hi - lo > wide else Dec(), mid, r1, w1, r2, w2, r4):
+            lo = mid
+        else:
+            hi = mid
+    wrem = la - (w1 + lo + w2)
+    if D.cmp(wrem, "<=", 0):
+        return True, 1
+    return (True, 2) if tr_can_recurse(D, wrem, mpf(1), min(R, r[3]), R) else (False, 0)
+
+
+TR_LEAF = {0: "two_by_two", 1: "three_disk", 2: "gaps"}
+
+
+def tr_leaf(la, r):
+    """The three leaf criteria on the point leaf: {bit: (verdict, strategy, slack)}
+    with the bit numbering of the oracle's leaf masks."""
+    la = mpf(la)
+    r = [mpf(x) for x in r]
+    R4, R = tr_leaf_weights(la, r)
+    out = {}
+    for k, f in ((0, lambda D: tr_two_by_two(D, la, r, R)), (1, lambda D: tr_three_disk(D, la, r, R4)),
+                 (2, lambda D: tr_gaps(D, la, r, R))):
+        D = Dec()
+        try:
+            v, s = f(D)
+        except ZeroDivisionError:
+            v, s, D.slack = False, 0, mpf(0)
         out[k] = (bool(v), s, D.slack)
     return out

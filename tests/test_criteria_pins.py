@@ -429,3 +429,165 @@ def test_rsb_leaf_steps_equal_exact_at_clear_slack():
                 caught = True
                 break
         assert caught, f"planted slip in step {k} not caught"
+
+
+# --------------------------------------------- tight_rectangle leaf criteria
+
+def tr_leaf_points():
+    """Point leaves (lambda, r1 >= r2 >= r3 >= r4) with weight left over:
+    the radii share a fraction of critical_ratio(lambda) * lambda, in
+    proportions from seeded Dirichlet draws aimed at the two-by-two, three-
+    disk and gap placements, plus leaves around the three-disk worst case
+    r*(lambda) that the manually proved pocket covers."""
+    rng = np.random.default_rng(0xC1C1EC + 2)
+
+    def weight(la):
+        return np.array([float(ce.tr_critical_ratio(mpmath.mpf(x))) for x in la]) * la
+
+    def leaves(n, la_rng, frac, conc):
+        la = rng.uniform(*la_rng, n)
+        r = np.sort(rng.dirichlet(conc, n), axis=1)[:, ::-1] * (weight(la) * rng.uniform(*frac, n))[:, None]
+        return np.column_stack([la, r])
+
+    la = rng.uniform(1.0, 1.06, 150)
+    rs = la * la / 16 + 0.15625 + 0.03515625 / (la * la)
+    d = rng.uniform(0, 1, (150, 3)) * np.array([0.0012, 0.0024, 0.0036])
+    r4 = rng.uniform(0, 1, 150) * (weight(la) - 3 * rs + d.sum(axis=1))
+    pocket = np.column_stack([la, np.sort(np.column_stack([rs[:, None] - d, r4]), axis=1)[:, ::-1]])
+    P = np.concatenate([leaves(150, (1.0, 2.1), (0.6, 0.999), (1, 1, 1, 1)),
+                        leaves(150, (1.0, 2.1), (0.8, 0.999), (3, 2, 1, 1)),
+                        leaves(150, (1.0, 1.5), (0.85, 0.999), (4, 4, 4, 1)),
+                        leaves(150, (1.0, 1.6), (0.85, 0.999), (6, 6, 2, 2)),
+                        leaves(100, (1.0, 1.3), (0.9, 0.999), (8, 8, 8, 0.5)), pocket])
+    return P[weight(P[:, 0]) - P[:, 1:].sum(axis=1) > 1e-6]
+
+
+def _tr_leaf_fns(la, r, R4, R, over=None):
+    f = {8: lambda D: ce.tbt_lambda_small(D, la, r, R),
+         9: lambda D: ce.tbt_lambda_big(D, la, r, R),
+         10: lambda D: ce.three_disk_big_strip(D, la, r[0], r[1], r[2], r[3], R4),
+         11: lambda D: ce.three_disk_small_corner(D, la, r[0], r[1], r[2], r[3], R4),
+         12: lambda D: ce.three_disk_r1_r3_pocket(D, la, r[0], r[1], r[2], r[3], R4),
+         13: lambda D: ce.manual_three_disk_pocket(D, la, r[0], r[1], r[2]),
+         2: lambda D: ce.tr_gaps(D, la, r, R)[0]}
+    if over:
+        for k, g in over.items():
+            f[k] = (lambda g_: lambda D: g_(D, la, r, R4, R))(g)
+    return f
+
+
+def tr_leaf_exact(P, over=None, only=None):
+    """{bit: (verdicts, slacks)} of the exact leaf strategies on the points."""
+    out = {}
+    for i, p in enumerate(P):
+        la, r = mpmath.mpf(p[0]), [mpmath.mpf(x) for x in p[1:]]
+        R4, R = ce.tr_leaf_weights(la, r)
+        for k, f in _tr_leaf_fns(la, r, R4, R, over).items():
+            if only is not None and k not in only:
+                continue
+            D = ce.Dec()
+            v = bool(f(D))
+            vs, ss = out.setdefault(k, (np.zeros(len(P), bool), np.zeros(len(P))))
+            vs[i], ss[i] = v, float(D.slack)
+    return out
+
+
+def tr_leaf_oracle(P):
+    cells = np.zeros((len(P), 10))
+    cells[:, 0] = cells[:, 1] = P[:, 0]
+    for k in range(4):
+        cells[:, 2 + 2 * k] = cells[:, 3 + 2 * k] = P[:, 1 + k]
+    return oracle.tr_leaf_masks(cells)
+
+
+def test_tr_leaf_criteria_equal_exact_at_clear_slack():
+    """The tight_rectangle leaf criteria (tight_rectangle.cu:247-249), each
+    strategy on its own: two-by-two with lambda small (r1's square, r2 and
+    r4 beside it, r3 in the far corner; r1 large: r2..r4 stacked beside r1's
+    full-height strip) and lambda big (two stacked pairs), the three-disk big
+    strip, small corner, r1-r3 pocket and the manually proved pocket, and the
+    gaps strategy (with its bisection); the oracle's bits against the exact
+    statement, both ways, where the slack is clear.  The leaves' criteria
+    and strategies each prove and reject at least 15 leaves, and a planted
+    slip in each is caught."""
+    P = tr_leaf_points()
+    m = tr_leaf_oracle(P)
+    assert not (m >> 31).any()
+    ex = tr_leaf_exact(P)
+    for k, (v, sl) in ex.items():
+        clear = sl >= CLEAR
+        o = ((m >> k) & 1).astype(bool)
+        bad = np.flatnonzero(clear & (o != v))
+        assert bad.size == 0, (k, [list(P[i]) for i in bad[:3]])
+        assert min((clear & v).sum(), (clear & ~v).sum()) >= 15, f"bit {k}: {(clear & v).sum()} proven, " \
+                                                                  f"{(clear & ~v).sum()} rejected"
+    # the criteria are the strategies' disjunctions
+    assert (((m >> 8) | (m >> 9)) & 1 == m & 1).all()
+    assert (((m >> 10) | (m >> 11) | (m >> 12) | (m >> 13)) & 1 == (m >> 1) & 1).all()
+
+    def small_r3_r4_swapped(D, la, r, R4, R):  # r3 in the strip above r1's square, r4 in the corner
+        return ce.tbt_lambda_small(D, la, [r[0], r[1], r[3], r[2]], R)
+
+    def big_one_pairing(D, la, r, R4, R):  # only the pairing (r1, r2) + (r3, r4)
+        def th(a, b):
+            return ce.two_disks_maximize_height(D, a, b, mpmath.mpf(1))
+        w = th(r[0], r[1]) + th(r[2], r[3])
+        return w != 0 and ce.tr_can_recurse(D, la - w, mpmath.mpf(1), min(R, r[3]), R)
+
+    def strip_sign(D, la, r, R4, R):  # + 4 (r2 - r3)^2 in the strip width
+        s_sq = 2 * r[1] + 2 * r[2] - mpmath.mpf(1) / 4 + 4 * (r[1] - r[2]) ** 2
+        if D.cmp(s_sq, "<=", 0):
+            return False
+        la_rem = la - mpmath.sqrt(s_sq)
+        h1 = 4 * r[0] - la_rem * la_rem
+        if D.cmp(h1, "<=", 0):
+            return False
+        return D.cmp(h1, ">=", 1) or ce.tr_can_recurse(D, la_rem, 1 - mpmath.sqrt(h1), r[3], R4)
+
+    def corner_weight(D, la, r, R4, R):  # the recursion given R (r4 already spent) instead of R4
+        return ce.three_disk_small_corner(D, la, r[0], r[1], r[2], r[3], R)
+
+    def pocket_no_half(D, la, r, R4, R):  # la - S1 - S3 for the pocket's height term (S1 not halved)
+        p = ce._pocket_X(D, la, r[0], r[1], r[2])
+        if p is None:
+            return False
+        s1, s3, X = p
+        if D.cmp(X, "<=", 0):
+            return True
+        y_rest = X ** 2
+        if D.cmp(r[0], "<=", y_rest):
+            return False
+        Y = mpmath.mpf(1) / 2 - mpmath.sqrt(r[0] - y_rest)
+        return D.cmp(Y, "<=", 0) or ce.tr_can_recurse(D, X, Y, r[3], R4)
+
+    def manual_eps(D, la, r, R4, R):  # r3's tolerance 0.002 instead of 0.003
+        lasq = la * la
+        rstar = lasq / 16 + mpmath.mpf(0.15625) + mpmath.mpf(0.03515625) / lasq
+        d = rstar - r[2]
+        if D.cmp(d, "<", -0.002) or D.cmp(d, ">", 0.002):
+            return False
+        return ce.manual_three_disk_pocket(D, la, r[0], r[1], r[2])
+
+    def gaps_centre(D, la, r, R4, R):  # r4's centre at the gap's far end instead of its middle
+        saved = ce._gap_ok
+
+        def slipped(D_, gap, r1, w1, r2, w2, r4):
+            y4 = r4 - gap * gap / 4
+            if D_.cmp(y4, "<=", 0):
+                return False
+            ok, p, _ = ce.circle_intersection(ce.Dec(), (w1 / 2, mpmath.mpf(1) / 2, r1),
+                                              (w2 / 2 + gap + w1, mpmath.mpf(1) / 2, r2))
+            return ok and D_.cmp(ce.dist2((gap + w1, mpmath.sqrt(y4)), p), "<=", r4)
+        ce._gap_ok = slipped
+        try:
+            return ce.tr_gaps(D, la, r, R)[0]
+        finally:
+            ce._gap_ok = saved
+
+    slips = {8: small_r3_r4_swapped, 9: big_one_pairing, 10: strip_sign, 11: corner_weight, 12: pocket_no_half,
+             13: manual_eps, 2: gaps_centre}
+    sample = np.arange(0, len(P), 2)
+    for k, f in slips.items():
+        v, sl = tr_leaf_exact(P[sample], {k: f}, only={k})[k]
+        o = ((m[sample] >> k) & 1).astype(bool)
+        assert ((sl >= CLEAR) & (v != o)).any(), f"slip in bit {k} not caught"
