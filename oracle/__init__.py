@@ -54,6 +54,8 @@ def _load(name: str) -> ctypes.CDLL:
     L.or_tr_merge_adjacent_r1.argtypes = [_vp, _sz, _vp]
     L.or_tr_merge_adjacent_r1.restype = _sz
     L.or_tr_leaf_masks.argtypes = [_vp, _sz, _vp]
+    L.or_rsb_c14_probe.argtypes = [_vp, _sz, _vp, _i, _vp, _vp]
+    L.or_rsb_c14_probe.restype = None
     L.or_tr_leaf_masks.restype = None
     return L
 
@@ -175,6 +177,18 @@ def rsb_grid(outer, nla: int, nr1: int, nr2: int) -> np.ndarray:
     out = np.empty((nla * nr1 * nr2, 14))
     m = lib.or_rsb_grid(outer.ctypes.data, nla, nr1, nr2, out.ctypes.data)
     return out[:m].copy()
+
+
+def rsb_c14_probe(cells: np.ndarray, pieces: np.ndarray):
+    """C14's stages on (n, 14) cells with (n, np, 2) r7 pieces: (n, 7) widths
+    and ranges, (n, np) piece-strategy bits (or_rsb_c14_probe)."""
+    cells = np.ascontiguousarray(cells, dtype=np.float64)
+    pieces = np.ascontiguousarray(pieces, dtype=np.float64)
+    n, npc = cells.shape[0], pieces.shape[1]
+    out = np.zeros((n, 7))
+    bits = np.zeros((n, npc), dtype=np.uint32)
+    lib.or_rsb_c14_probe(cells.ctypes.data, n, pieces.ctypes.data, npc, out.ctypes.data, bits.ctypes.data)
+    return out, bits
 
 
 def rsb_bnb(cells: np.ndarray, max_depth: int, threads: int = 0, cap: int = 1 << 20):

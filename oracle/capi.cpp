@@ -262,6 +262,18 @@ int or_rsb_masks(const double* cells, size_t n, uint64_t* out, int threads) {
   return 0;
 }
 
+// C14's stages on n point-or-interval cells (14 doubles each) with np r7
+// pieces per cell (pieces: n * np (lb, ub) pairs): out[7 * i ..] and
+// bits[np * i ..] as rsb::seven_disk_probe.
+void or_rsb_c14_probe(const double* cells, size_t n, const double* pieces, int np, double* out, uint32_t* bits) {
+  for (size_t i = 0; i < n; ++i) {
+    rsb::Cell c;
+    c.la = IV(cells[14 * i], cells[14 * i + 1]);
+    for (int k = 0; k < 6; ++k) c.r[k] = IV(cells[14 * i + 2 + 2 * k], cells[14 * i + 3 + 2 * k]);
+    rsb::seven_disk_probe(c, pieces + 2 * np * i, np, out + 7 * i, bits + np * i);
+  }
+}
+
 int or_rsb_verdicts(const double* cells, size_t n, int32_t* out, uint8_t* trap, int threads) {
   if (threads <= 0) threads = (int)std::thread::hardware_concurrency();
   if (threads < 1) threads = 1;

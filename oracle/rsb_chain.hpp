@@ -49,6 +49,7 @@ bool advanced_multi_disk_strip_vertical(const Cell& c, const Vals& v);          
 bool five_disk_full_cover(const Cell& c, const Vals& v);                        // C12
 bool six_disk_full_cover(const Cell& c, const Vals& v);                         // C13
 bool seven_disk_strategies(const Cell& c, const Vals& v);                       // C14
+void seven_disk_probe(const Cell& in, const double* pieces, int np, double* out, uint32_t* bits);
 
 // Verdict of one 7-D cell exactly as the reference's enumeration kernel
 // treats a leaf (rectangle_size_bound.cu:162-229): cell-level shortcuts S1,S2,

@@ -933,6 +933,47 @@ bool seven_disk_strategies(const Cell& c, const Vals& v) {
   return true;
 }
 
+// C14's stages on one cell, for the tests (no reference counterpart):
+// out[0] the width r1..r6 cover, out[1..6] the r7 range (lb, ub) after
+// compute_r7_range, the T-shaped restriction and the four-disk strip, and
+// for each of the np r7 pieces (lb, ub) in pieces, bits[k] = which of the
+// six piece strategies of seven_disk_strategies.cu:1430-1475 cover it, each
+// evaluated on its own: 1 r7 in the corner, 2 three-disk top strip, 4 the
+// 2-2-2 placement, 8 the r4-r3-r5 strip, 16 the different-height cover, 32
+// the five-disk height (with h34 > 0).
+void seven_disk_probe(const Cell& in, const double* pieces, int np, double* out, uint32_t* bits) {
+  Cell c = tighten(in);
+  Vals v = make_vals(c);
+  double covered = six_disks_maximize_covered_width(c);
+  out[0] = covered;
+  IV w_rem = c.la - covered;
+  w_rem.tighten_lb(0.0);
+  IV range = compute_r7_range(c, v, w_rem.ub);
+  out[1] = range.lb;
+  out[2] = range.ub;
+  if (!range.empty()) r7_t_shaped(c, v, range);
+  out[3] = range.lb;
+  out[4] = range.ub;
+  if (!range.empty()) r7_four_disk_strip_r1_square(c, v, range);
+  out[5] = range.lb;
+  out[6] = range.ub;
+  double h34 = two_disks_maximize_height(c.r[2], c.r[3], c.la.ub);
+  for (int k = 0; k < np; ++k) {
+    IV r7(pieces[2 * k], pieces[2 * k + 1]);
+    uint32_t b = 0;
+    if (r7_in_corner_and_recurse(v.R, r7, w_rem.ub)) b |= 1;
+    if (three_disk_top_strip(c, v.R, r7)) b |= 2;
+    if (six_disks_222(c, v, r7)) b |= 4;
+    if (r435_strip(c, r7, v)) b |= 8;
+    if (different_height_r12_2x2(c, v, r7)) b |= 16;
+    if (h34 > 0.0) {
+      double h5 = five_disks_maximize_height_23(c.la.ub, c.r[0].lb, c.r[1].lb, c.r[4].lb, c.r[5].lb, r7.lb);
+      if (add_rd(h34, h5) >= 1.0) b |= 32;
+    }
+    bits[k] = b;
+  }
+}
+
 // ------------------------------------------------------------------ cell verdict
 // rectangle_size_bound.cu:162-229 for one 7-D cell.
 int cell_verdict(const Cell& in) {

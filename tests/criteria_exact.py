@@ -740,8 +740,7 @@ def nd_maximize_covered(D, w, d):
     if n == 3:
         return side
     stacked = _chords_dec(D, d, w)  # stacked: the chords at width w
-    best = side if D.cmp(side, ">=", stacked) else stacked
-    return hi if D.cmp(best, ">=", hi) else best
+    return min(max(side, stacked), hi)  # max and min: continuous, no decision to record
 
 
 def _c5_backtrack(D, d, reached, target, cross):
@@ -1601,4 +1600,787 @@ def tr_leaf(la, r):
         except ZeroDivisionError:
             v, s, D.slack = False, 0, mpf(0)
         out[k] = (bool(v), s, D.slack)
+    return out
+
+
+# ------------------------------- rectangle_size_bound leaf step C14 (seven disks)
+# seven_disk_strategies.cu: r1..r6 and an unknown seventh disk r7 whose range
+# [lower bound, min(r6, R)] is cut down by recursions and then checked in 16
+# pieces.  The reference's bisections become bisections on exact midpoints;
+# the tests of their first levels, where the bracket is still wide, count
+# towards the slack.  r7 ranges are [lo, hi] lists; an empty range has lo > hi.
+
+C14_ITER = 80        # 2^-80 of a bracket: far below any compared slack
+C14_WIDE = mpf("1e-8")
+
+
+def _bisect(D, test, lo, hi, record=True):
+    """The reference's `mid = (lo + hi) / 2; test(mid) ? lo = mid : hi = mid`
+    on exact midpoints; returns (lo, the value test returned at the last
+    passing midpoint or None)."""
+    wide = (hi - lo) * C14_WIDE
+    last = None
+    for _ in range(C14_ITER):
+        mid = (lo + hi) / 2
+        got = test(D if record and hi - lo > wide else Dec(), mid)
+        if got:
+            lo, last = mid, got
+        else:
+            hi = mid
+    return lo, last
+
+
+def three_disks_height(D, w, t):
+    """three_disks_maximize_height(t1, t2, t3, w) (maximize_covered.cu:198-237):
+    the greatest height at which the three disks side by side span w, below
+    2 sqrt(t3), the bound the reference's bisection starts from."""
+    hi = 2 * mpmath.sqrt(t[2])
+    if D.cmp(_chords(t, mpf(0)), "<", w):
+        return mpf(0)
+    if D.cmp(_chords(t, hi), ">=", w):
+        return hi
+    lo, _ = _bisect(D, lambda D_, h: D_.cmp(_chords(t, h), ">=", w), mpf(0), hi)
+    return lo
+
+
+def bound_critical_ratio(D, rem, largest, w, h):
+    """bound_critical_ratio (can_recurse.cu:90-160): the best covering ratio
+    the remaining weight reaches on a w x h rectangle (size-bounded with a
+    square or strip of the required side, the large size-bound corollary, the
+    worst case), INF if none."""
+    if D.cmp(rem, "<", 0):
+        return INF
+    if D.cmp(w, "<=", 0) or D.cmp(h, "<=", 0):
+        return mpf(0)
+    smin, smax = min(w, h), max(w, h)
+    best = INF
+    area = w * h
+    req = mpmath.sqrt(largest / UB_DISK_WEIGHT)
+    if D.cmp(smin, ">=", req):
+        return CR if D.cmp(CR * area, "<=", rem) else INF
+    if D.cmp(req, ">", smax):
+        if D.cmp(mpmath.sqrt(rem / CR), ">=", req):
+            best = req * req * CR / area
+    else:
+        need = CR * req * smax
+        if D.cmp(need, "<=", rem):
+            best = need / area
+    sig = (largest / (smin * smin)) ** 2
+    cor = mpmath.sqrt(mpmath.sqrt(sig + 1) + 1) / 2
+    if D.cmp(cor, "<", C_WORST):
+        cor = C_WORST
+    if D.cmp(cor, "<", best) and D.cmp(cor * area, "<=", rem):
+        best = cor
+    cr = tr_critical_ratio(smax / smin)
+    if D.cmp(cr, "<", best) and D.cmp(cr * area, "<=", rem):
+        best = cr
+    return best
+
+
+def recursion_bound_size(D, rem, w, h):
+    """recursion_bound_size (can_recurse.cu:162-219): the largest disk weight
+    recursion on a w x h rectangle tolerates with the weight rem (INF: any)."""
+    if D.cmp(rem, "<=", 0):
+        if rem < 0:
+            return mpf(0)
+        return INF if (w <= 0 or h <= 0) else mpf(0)
+    if D.cmp(w, "<=", 0) or D.cmp(h, "<=", 0):
+        return INF
+    smin, smax = min(w, h), max(w, h)
+    max_cr = rem / (w * h)
+    if D.cmp(tr_critical_ratio(smax / smin), "<=", max_cr):
+        return INF
+    sb_area = rem / CR
+    side = sb_area / smax
+    if D.cmp(side, "<", smin):
+        return mpf(0)
+    if D.cmp(side, ">", smax):
+        side = mpmath.sqrt(sb_area)
+    best = UB_DISK_WEIGHT * side * side
+    if D.cmp(max_cr, ">=", C_WORST):
+        lam = 2 * max_cr + mpmath.sqrt(4 * max_cr * max_cr - 2)
+        sig = max_cr * (lam - 2 / lam) * smin * smin
+        if D.cmp(sig, ">=", best):
+            best = sig
+    return best
+
+
+def _rest(x, y):
+    """The lower bound of a remaining weight x - y, where y is r7's upper
+    bound.  Where that bound is R itself (the last r7 piece) the difference is
+    0 in exact arithmetic, but below 0 in the reference, which subtracts R's
+    upper bound from R's lower bound: a negative infinitesimal stands in."""
+    d = x - y
+    return d if abs(d) > mpf("1e-40") * max(abs(x), abs(y)) else mpf("-1e-40") * max(abs(x), abs(y))
+
+
+def _empty(D, r7):
+    return D.cmp(r7[0], ">", r7[1])
+
+
+def _tighten_lb(r7, x):
+    if x > r7[0]:
+        r7[0] = x
+
+
+def _split_largest_r7(D, r7, rem, wf, hf, ws, hs):
+    """restrict_r7_t_shaped_recursion_split_recursion_largest_r7 (:116-153):
+    r7 as the largest disk of both parts; the range keeps only the r7 for
+    which the split does not work."""
+    def works(D_, x):
+        cr = bound_critical_ratio(D_, rem, x, wf, hf)
+        if cr == INF:
+            return False
+        return rsb_can_recurse(D_, rem - (cr * wf * hf + x), x, ws, hs)
+    if works(D, r7[1]):
+        r7[0] = INF
+        return
+    lo, _ = _bisect(D, works, r7[0], r7[1])
+    _tighten_lb(r7, lo)
+
+
+def _split_recursion(D, r7, r, rem, largest, wf, hf, ws, hs):
+    """restrict_r7_t_shaped_recursion_split_recursion (:155-215)."""
+    if largest > 5:
+        _split_largest_r7(D, r7, rem, wf, hf, ws, hs)
+        return
+    cr = bound_critical_ratio(D, rem, r[largest], wf, hf)
+    if cr == INF:
+        return
+    need = cr * wf * hf
+    win = sum(r[largest:6])
+    if D.cmp(win, ">", need):
+        _tighten_lb(r7, recursion_bound_size(D, rem - win, ws, hs))
+        return
+    _tighten_lb(r7, recursion_bound_size(D, rem - (need + r7[1]), ws, hs))
+    if not _empty(D, r7):
+        lo, _ = _bisect(D, lambda D_, x: not D_.cmp(recursion_bound_size(D_, rem - (need + x), ws, hs), "<", x),
+                        r7[0], r7[1])
+        r7[0] = lo
+
+
+def _place_r6(D, q, x1, x2, lower, upper):
+    """restrict_r7_t_shaped_recursion_compute_r6 (:225-258): the disk q
+    centred right of the two points, spanning their height; (centre, the x
+    up to which it covers the strip, 0 if the placement fails)."""
+    left = min(x1[0], x2[0])
+    top, bot = max(x1[1], x2[1]), min(x1[1], x2[1])
+    dx = q - (top - bot) ** 2 / 4
+    if q == 0 or D.cmp(dx, "<=", 0):  # q = 0: r7 at its lower bound 0
+        return None, mpf(0)
+    p = (left + mpmath.sqrt(dx), (bot + top) / 2)
+    dy = max((p[1] - lower) ** 2, (upper - p[1]) ** 2)
+    sw = q - dy
+    if D.cmp(sw, "<=", 0):
+        return p, mpf(0)
+    return p, p[0] + mpmath.sqrt(sw)
+
+
+def _meets(D, a, b):
+    """The two intersection points of two circles in no particular order (the
+    caller picks by coordinate), None unless they definitely intersect; only
+    the intersection test is recorded (operations.cu:39-51)."""
+    d2 = dist2(a, b)
+    rd = a[2] - b[2]
+    if d2 == 0 or D.cmp(2 * (a[2] + b[2]) / d2, "<", 1 + rd * rd / (d2 * d2)):
+        return None
+    _, p, q = circle_intersection(Dec(), a, b)
+    return p, q
+
+
+def _intersect_or_trap(D, a, b):
+    ok, p, q = circle_intersection(D, a, b)
+    if not ok:
+        _trap(D)
+    return ok, p, q
+
+
+def _t_shaped_r5_large(D, r7, la, r, c1, c3, c5, R6, w12, wrem, hrem, strip_w5, lower, upper, p43, p21):
+    """restrict_r7_t_shaped_recursion_r5_large (:260-314)."""
+    ok15, _, x15 = _intersect_or_trap(D, c1, c5)
+    ok35, _, x35 = _intersect_or_trap(D, c3, c5)
+    if not (ok15 and ok35):
+        return
+    left = min(x15[0], x35[0])
+    whr = w12 - left
+    if D.cmp(whr, "<=", 0):
+        if rsb_can_recurse(D, R6, r[5], wrem, mpf(1)):
+            r7[0] = INF
+        return
+    _split_recursion(D, r7, r, R6, 5, whr, hrem, wrem, mpf(1))
+    _split_recursion(D, r7, r, R6, 5, wrem, mpf(1), whr, hrem)
+    if _empty(D, r7):
+        return
+    R7 = R6 - r[5]
+    if D.cmp(R7, "<=", 0):
+        return
+    x1, x2 = x15, x35
+    if (D.cmp(x1[1], ">=", upper) or D.cmp(x1[1], "<=", lower) or D.cmp(x2[1], ">=", upper)
+            or D.cmp(x2[1], "<=", lower)):
+        x1, x2 = (strip_w5, upper), (strip_w5, lower)
+    _, reach = _place_r6(D, r[5], x1, x2, lower, upper)
+    if reach == 0 or D.cmp(reach, "<=", 0):  # 0: the placement failed
+        return
+    if D.cmp(reach, ">=", w12):
+        _tighten_lb(r7, recursion_bound_size(D, R7, wrem, mpf(1)))
+    else:
+        whr = w12 - left  # the reference recomputes this from r5's intersections
+        _split_recursion(D, r7, r, R7, 6, whr, hrem, wrem, mpf(1))
+        _split_recursion(D, r7, r, R7, 6, wrem, mpf(1), whr, hrem)
+
+
+def _t_shaped(D, r7, la, r, R):
+    """restrict_r7_t_shaped_recursion (:316-510): r2's square in the lower
+    left corner with r1 beside it, r3 and r4 stacked in the strip above, r5
+    (and r6, r7) in the gap between; recursion on the rest."""
+    w2 = mpmath.sqrt(2 * r[1])
+    h12 = w2
+    w1 = mpmath.sqrt(4 * r[0] - h12 * h12)
+    w12 = w1 + w2
+    h34 = two_disks_maximize_height(D, r[2], r[3], w12)
+    if h34 == 0:
+        return
+    R5 = la * CR - (r[0] + r[1] + r[2] + r[3])
+    hrem = 1 - (h12 + h34)
+    wrem = la - w12
+    if D.cmp(hrem, "<=", 0):
+        if D.cmp(wrem, "<=", 0) or rsb_can_recurse(D, R5, r[4], wrem, mpf(1)):
+            r7[0] = INF
+        return
+    if D.cmp(wrem, "<=", 0):
+        if rsb_can_recurse(D, R5, r[4], la, hrem):
+            r7[0] = INF
+        return
+    _split_recursion(D, r7, r, R5, 4, w12, hrem, wrem, mpf(1))
+    _split_recursion(D, r7, r, R5, 4, wrem, mpf(1), w12, hrem)
+    if _empty(D, r7):
+        return
+    R6 = R5 - r[4]
+    if D.cmp(R6, "<=", 0):
+        return
+    x5 = r[4] - hrem * hrem / 4
+    if D.cmp(x5, "<=", 0):
+        return
+    x5 = mpmath.sqrt(x5)
+    strip_w5 = 2 * x5
+    p5 = (x5, h12 + hrem / 2)
+    p2 = (w2 / 2, h12 / 2)
+    p1 = (w2 + w1 / 2, h12 / 2)
+    x4 = mpmath.sqrt(r[3] - h34 * h34 / 4)
+    y4 = 1 - h34 / 2
+    p4 = (x4, y4)
+    w3 = w12 - 2 * x4
+    p3 = (w12 - w3 / 2, y4)
+    c1, c2, c3, c4, c5 = (p1 + (r[0],)), (p2 + (r[1],)), (p3 + (r[2],)), (p4 + (r[3],)), (p5 + (r[4],))
+    p43, p21 = (2 * x4, 1 - h34), (w2, h12)
+    lower, upper = h12, 1 - h34
+    if D.cmp(dist2(p5, p43), "<=", r[4]) and D.cmp(dist2(p5, p21), "<=", r[4]):
+        _t_shaped_r5_large(D, r7, la, r, c1, c3, c5, R6, w12, wrem, hrem, strip_w5, lower, upper, p43, p21)
+        return
+    ok25, a25, x25 = _intersect_or_trap(D, c2, c5)
+    ok45, a45, x45 = _intersect_or_trap(D, c4, c5)
+    if not (ok25 and ok45):
+        return
+    if D.cmp(a25[0], ">=", x25[0]) or D.cmp(a45[0], ">=", x45[0]):
+        return
+    x1, x2 = x25, x45
+    if (D.cmp(x1[1], ">=", upper) or D.cmp(x1[1], "<=", lower) or D.cmp(x2[1], ">=", upper)
+            or D.cmp(x2[1], "<=", lower)):
+        x1, x2 = (strip_w5, upper), (strip_w5, lower)
+    left = min(x1[0], x2[0])
+    whr = w12 - left
+    _split_recursion(D, r7, r, R6, 5, whr, hrem, wrem, mpf(1))
+    _split_recursion(D, r7, r, R6, 5, wrem, mpf(1), whr, hrem)
+    if _empty(D, r7):
+        return
+    R7 = R6 - r[5]
+    if D.cmp(R7, "<=", 0):
+        return
+    p6, reach = _place_r6(D, r[5], x1, x2, lower, upper)
+    if reach == 0 or D.cmp(reach, "<=", 0):  # 0: the placement failed
+        return
+    c6 = p6 + (r[5],)
+    ok16, _, x16 = _intersect_or_trap(D, c1, c6)
+    ok36, _, x36 = _intersect_or_trap(D, c3, c6)
+    if not (ok16 and ok36):
+        return
+    r6_right = False
+    if D.cmp(dist2(p43, p6), "<=", r[5]) and D.cmp(dist2(p21, p6), "<=", r[5]):
+        left = min(x16[0], x36[0])
+        d71, d72 = x36, x16
+        r6_right = True
+    else:
+        left = reach
+        # the reference builds these points with the strip borders as x and
+        # the covered x as y (a quirk kept on purpose)
+        d71, d72 = (upper, left), (lower, left)
+    whr = w12 - left
+    if D.cmp(whr, "<=", 0):
+        _tighten_lb(r7, recursion_bound_size(D, R7, wrem, mpf(1)))
+    else:
+        _split_recursion(D, r7, r, R7, 6, whr, hrem, wrem, mpf(1))
+        _split_recursion(D, r7, r, R7, 6, wrem, mpf(1), whr, hrem)
+    if _empty(D, r7):
+        return
+    R8 = _rest(R7, r7[1])
+    if D.cmp(R8, "<=", 0):
+        return
+    if r6_right:
+        ok13, a13, x13 = circle_intersection(D, c1, c3)
+        if (ok13 and D.cmp(dist2(a13, p6), "<=", r[5]) and D.cmp(x13[1], ">=", lower)
+                and D.cmp(x13[1], "<=", upper)):
+            d71 = d72 = x13
+    _, reach7 = _place_r6(D, r7[0], d71, d72, lower, upper)
+    if reach7 == 0 or D.cmp(reach7, "<=", 0):
+        return
+    if D.cmp(reach7, ">=", la):
+        _split_largest_r7(D, r7, R8, wrem, h12, wrem, h34)
+        _split_largest_r7(D, r7, R8, wrem, h34, wrem, h12)
+    if D.cmp(reach7, ">=", w12):
+        _tighten_lb(r7, recursion_bound_size(D, R8, wrem, mpf(1)))
+    else:
+        whr = w12 - reach7
+        _split_largest_r7(D, r7, R8, whr, hrem, wrem, mpf(1))
+        _split_largest_r7(D, r7, R8, wrem, mpf(1), whr, hrem)
+
+
+def _four_disk_strip_r1_square(D, r7, la, r, R):
+    """restrict_r7_four_disk_strip_r1_square (:1222-1265): r2..r5 cover a
+    strip along the top, r1's square sits below it in the left corner, r6
+    covers what is left beside the square."""
+    h = nd_maximize_covered(D, la, [r[1], r[2], r[3], r[4]])
+    if h == 0:
+        return
+    wh1 = mpmath.sqrt(2 * r[0])
+    hrem = 1 - h
+    hbl = 1 - (h + wh1)
+    if D.cmp(hbl, "<=", 0):
+        w1 = mpmath.sqrt(4 * r[0] - hrem * hrem)
+        R6 = la * CR - sum(r[:5])
+        if rsb_can_recurse(D, R6, r[5], la - w1, hrem):
+            r7[0] = INF
+        return
+    wrem = la - wh1
+    if D.cmp((wh1 * wh1 + hbl * hbl) / 4, "<=", r[5]):
+        _tighten_lb(r7, recursion_bound_size(D, R, wrem, hrem))
+        return
+    w6 = 4 * r[5] - hbl * hbl
+    if D.cmp(w6, "<=", 0):
+        return
+    _split_largest_r7(D, r7, R, wrem, hrem, wh1 - mpmath.sqrt(w6), hbl)
+
+
+def _r7_range(D, la, r, R, Rub, wrem):
+    """compute_r7_range (:1150-1190): [0, min(r6, R)], cut where the rest
+    could be covered size-bounded or by the (large size-bound) worst case."""
+    r7 = [mpf(0), min(r[5], Rub)]
+    eff = R / wrem
+    if D.cmp(eff, ">=", CR):
+        msw = R / CR
+        if D.cmp(msw, ">=", 1):
+            return [mpf(1), mpf(0)]
+        _tighten_lb(r7, UB_DISK_WEIGHT * msw * msw)
+    if D.cmp(eff, ">=", C_WORST) and not _empty(D, r7):
+        if D.cmp(tr_critical_ratio(1 / wrem), "<=", eff):
+            return [mpf(1), mpf(0)]
+        lam = 2 * eff + mpmath.sqrt(4 * eff * eff - 2)
+        _tighten_lb(r7, eff * (lam - 2 / lam) * wrem * wrem)
+    return r7
+
+
+def _r7_in_corner(D, R, r7, w):
+    """r7_in_corner_and_recurse (:38-47)."""
+    h = 4 * r7[0] - w * w
+    if D.cmp(h, "<", 0):
+        return False
+    return rsb_can_recurse(D, _rest(R, r7[1]), r7[1], w, 1 - mpmath.sqrt(h))
+
+
+def _top_strip_can_cover(D, la, th, rem4, largest, R7):
+    """three_disk_top_strip_can_cover (:49-114): three disks cover a strip
+    of height th along the top; recursion (size-bounded, else worst case with
+    the tolerated skew) covers part of the rest's width, two pairs of the
+    remaining four disks the other part."""
+    if th == 0:
+        return False
+    hrem = 1 - th
+    if D.cmp(hrem, "<=", 0):
+        return True
+    wsb = R7 / CR / hrem
+    if size_bound_ok(D, largest, min(wsb, hrem)):
+        wrem = la - wsb
+    else:
+        wwc = R7 / C_WORST / hrem
+        if wwc > 0 and D.cmp(max(wwc, hrem) / min(wwc, hrem), ">", TOL_LAMBDA_RSB):
+            if D.cmp(wwc, ">", hrem):
+                wwc = hrem * TOL_LAMBDA_RSB
+            else:
+                x = 2 * R7 - hrem * hrem / 2
+                wwc = mpf(0) if D.cmp(x, "<=", 0) else mpmath.sqrt(x)
+        wrem = la - wwc
+    bot = two_disks_maximize_height(D, rem4[1], rem4[2], wrem)
+    if bot == 0:
+        return False
+    if D.cmp(bot, ">=", hrem):
+        return True
+    h2 = two_disks_maximize_height(D, rem4[0], rem4[3], wrem)
+    if h2 == 0:
+        return False
+    return D.cmp(h2, ">=", hrem - bot)
+
+
+def _three_disk_top_strip(D, la, r, R, r7, heights):
+    """three_disk_top_strip (:512-546) over every triple of the seven disks
+    (r7 at its lower bound); heights caches the triples without r7."""
+    disks = list(r) + [r7[0]]
+    R7 = _rest(R, r7[1])
+    for i in range(5):
+        for j in range(i + 1, 6):
+            for k in range(j + 1, 7):
+                rest = [disks[g] for g in range(7) if g not in (i, j, k)]
+                key = (i, j, k)
+                if k < 6 and key in heights:
+                    th = heights[key]
+                else:
+                    th = three_disks_height(D, la, [disks[i], disks[j], disks[k]])
+                    if k < 6:
+                        heights[key] = th
+                if _top_strip_can_cover(D, la, th, rest, r7[1], R7):
+                    return True
+    return False
+
+
+def _six_222(D, la, r, R, r7):
+    """six_disks_222_with_recursion (:1192-1246): two stacked pairs side by
+    side, r5 and r6 stacked in the rest of the width; the two rectangles left
+    above r6 to recursion (r7 covering one of them, or the smaller group of a
+    greedy split)."""
+    c12 = two_disks_maximize_height(D, r[0], r[1], mpf(1))
+    c34 = two_disks_maximize_height(D, r[2], r[3], mpf(1))
+    if c34 == 0:
+        return False
+    wrem = la - (c12 + c34)
+    h2 = 4 * r[5] - wrem * wrem
+    if D.cmp(h2, "<=", 0):
+        return False
+    h1, h2 = mpmath.sqrt(4 * r[4] - wrem * wrem), mpmath.sqrt(h2)
+    hrem = 1 - (h1 + h2)
+    if D.cmp(hrem, "<=", 0):
+        return True
+    wtop = r[5] - (1 - (h1 + h2 / 2)) ** 2
+    if D.cmp(wtop, "<=", 0):
+        return False
+    wrect = (wrem - 2 * mpmath.sqrt(wtop)) / 2
+    if D.cmp((wrect * wrect + hrem * hrem) / 4, "<=", r7[0]):
+        return rsb_can_recurse(D, _rest(R, r7[1]), r7[1], wrect, hrem)
+    return rsb_can_recurse(D, _rest(R / 2, r7[1] / 2), r7[1], wrect, hrem)
+
+
+def _r435_strip(D, la, r, R, r7):
+    """r435_strip_covering_r12_gap_r67_right (:548-718): r3, r4, r5 cover a
+    vertical strip, r1 above r2 the widest strip next to it whose left
+    intersection r3 still covers, r6 and r7 stacked at the right."""
+    c435 = three_disks_height(D, mpf(1), [r[2], r[3], r[4]])
+    if c435 == 0:
+        return False
+    sq = c435 * c435
+    h3, h4 = 4 * r[2] - sq, 4 * r[3] - sq
+    if D.cmp(h3, "<=", 0) or D.cmp(h4, "<=", 0):
+        return False
+    h3, h4 = mpmath.sqrt(h3), mpmath.sqrt(h4)
+    pos3 = (c435 / 2, h3 / 2 + h4)
+    s1, s2 = mpmath.sqrt(r[0]), mpmath.sqrt(r[1])
+    if D.cmp(s1 + s2, "<=", mpf(1) / 2):
+        return False
+    ub = 2 * s2
+    wr = la - c435
+    if D.cmp(wr, "<", ub):
+        p1y, p2y = r[0] - wr * wr / 4, r[1] - wr * wr / 4
+        if D.cmp(p1y, ">", 0) and D.cmp(p2y, ">", 0):
+            x = wr / 2 + c435
+            ok, a, b = circle_intersection(D, (x, 1 - mpmath.sqrt(p1y), r[0]), (x, mpmath.sqrt(p2y), r[1]))
+            if ok and D.cmp(a[0], "<", b[0]) and D.cmp(dist2(pos3, a), "<=", r[2]):
+                rem_h = 1 - (2 * mpmath.sqrt(p1y) + 2 * mpmath.sqrt(p2y))
+                # la + x (not la - x) as the reference writes it
+                if rsb_can_recurse(D, R + r[5], r[5], la + b[0], rem_h):
+                    return True
+        ub = wr
+
+    def test(D_, mid):
+        hh2 = r[1] - mid * mid / 4
+        if D_.cmp(hh2, "<=", 0):
+            return None
+        hh1 = r[0] - mid * mid / 4
+        if hh1 < 0:  # NaN in the reference: no intersection
+            return None
+        x = mid / 2 + c435
+        ok, a, b = circle_intersection(D_, (x, 1 - mpmath.sqrt(hh1), r[0]), (x, mpmath.sqrt(hh2), r[1]))
+        if ok and D_.cmp(a[0], "<", b[0]) and D_.cmp(dist2(pos3, a), "<=", r[2]):
+            return b
+        return None
+    w12, best = _bisect(D, test, mpf(0), ub)
+    if best is None:
+        return False
+    w67 = la - (c435 + w12)
+    h7, h6 = 4 * r7[0] - w67 * w67, 4 * r[5] - w67 * w67
+    if D.cmp(h7, ">", 0) and D.cmp(h6, ">", 0):
+        h7, h6 = mpmath.sqrt(h7), mpmath.sqrt(h6)
+        pos6 = (la - w67 / 2, 1 - h7 - h6 / 2)
+        if D.cmp(dist2(best, pos6), "<=", r[5]):
+            if rsb_can_recurse(D, _rest(R, r7[1]), r7[1], w67, 1 - (h7 + h6)):
+                return True
+    wide = la - best[0]
+    h7, h6 = 4 * r7[0] - wide * wide, 4 * r[5] - wide * wide
+    if D.cmp(h7, ">", 0) and D.cmp(h6, ">", 0):
+        hr = 1 - (mpmath.sqrt(h6) + mpmath.sqrt(h7))
+        wrec = w67 if D.cmp(hr, "<=", best[1]) else wide
+        if rsb_can_recurse(D, _rest(R, r7[1]), r7[1], wrec, hr):
+            return True
+    return False
+
+
+def _different_height(D, la, r, R, r7):
+    """different_height_r12_and_two_by_two (:1248-1303): r2's square with r1
+    above it (4 r1 - w12, unsquared, as the reference writes it: Q7), pairs
+    r3/r4 and r5/r6 stacked beside; r7 across the higher gap, the rest priced
+    by the large size-bound corollary."""
+    w12 = mpmath.sqrt(2 * r[1])
+    if D.cmp(4 * r[0], "<", w12):  # NaN in the reference, every test false
+        return False
+    h12 = mpmath.sqrt(4 * r[0] - w12) + w12
+    wrem = la - w12
+    h3456 = two_disks_maximize_height(D, r[2], r[3], wrem) + two_disks_maximize_height(D, r[4], r[5], wrem)
+    a, b = 1 - h12, 1 - h3456
+    if D.cmp(a, "<", b):
+        hhigh, hlow, whigh, wlow = b, a, wrem, w12
+    else:
+        hhigh, hlow, whigh, wlow = a, b, w12, wrem
+    wr7 = 4 * r7[0] - hhigh * hhigh
+    if D.cmp(wr7, "<=", 0):
+        return False
+    whigh -= mpmath.sqrt(wr7)
+    R8 = _rest(R, r7[1])
+    if D.cmp(R8, "<=", 0):
+        return False
+    if D.cmp(whigh, "<=", 0):
+        return rsb_can_recurse(D, R8, r7[1], whigh + wlow, hlow)
+    s = min(hlow, wlow, whigh)
+    sig = (r7[1] / (s * s)) ** 2
+    cr = mpmath.sqrt(mpmath.sqrt(sig + 1) + 1) / 2
+    if D.cmp(cr, "<", C_WORST):
+        cr = C_WORST
+    area = r7[1] / (cr * hhigh) * (hhigh - hlow) + whigh * hhigh + wlow * hlow
+    return D.cmp(R8, ">=", area * cr)
+
+
+def _five_disks_23(D, la, t1, t2, m1, m2, m3):
+    """five_disks_maximize_height_23 (:975-1063): t1, t2 side by side at the
+    bottom, m2, m1, m3 in a row above them as high as possible with m1
+    covering the bottom pair's upper intersection and the middle row's lower
+    intersections inside the bottom disks; 0 if none."""
+    bot, b1, b2 = max_height_strip(D, la, t1, t2)
+    if bot == 0:
+        return mpf(0)
+    pq = _meets(D, b1, b2)
+    if pq is None:
+        return _trap(D) or mpf(0)
+    up = pq[1] if D.cmp(pq[0][1], "<=", pq[1][1]) else pq[0]
+
+    def lower(D_, a, b):
+        pq_ = _meets(D_, a, b)
+        if pq_ is None:
+            return None
+        return pq_[0] if D_.cmp(pq_[0][1], "<=", pq_[1][1]) else pq_[1]
+
+    def test(D_, hm):
+        sq = hm * hm
+        w = [4 * m - sq for m in (m1, m2, m3)]
+        if any(D_.cmp(x, "<=", 0) for x in w):
+            return False
+        w1, w2, w3 = (mpmath.sqrt(x) for x in w)
+        if D_.cmp(w1 + w2 + w3, ">=", la):
+            return True
+        wr = la - (w2 + w3)
+        h1 = m1 - wr * wr / 4
+        if D_.cmp(h1, "<=", 0):
+            return False
+        h1 = mpmath.sqrt(h1)
+        y23 = bot + hm / 2
+        c1 = ((w2 + la - w3) / 2, bot + hm - h1, m1)
+        if D_.cmp(dist2(c1, up), ">", m1):
+            return False
+        l12 = lower(D_, c1, (w2 / 2, y23, m2))
+        l13 = lower(D_, c1, (la - w3 / 2, y23, m3))
+        if l12 is None or l13 is None:
+            return False
+        return D_.cmp(dist2(l12, b1), "<=", t1) and D_.cmp(dist2(l13, b2), "<=", t2)
+    lo, _ = _bisect(D, test, mpf(0), 2 * mpmath.sqrt(m3))
+    return mpf(0) if lo == 0 else bot + lo
+
+
+PAIRINGS_3x2 = [((0, 1), (2, 3), (4, 5)), ((0, 1), (2, 4), (3, 5)), ((0, 1), (2, 5), (3, 4)),
+                ((0, 2), (1, 3), (4, 5)), ((0, 2), (1, 4), (3, 5)), ((0, 2), (1, 5), (3, 4)),
+                ((0, 3), (1, 2), (4, 5)), ((0, 3), (1, 4), (2, 5)), ((0, 3), (1, 5), (2, 4)),
+                ((0, 4), (1, 2), (3, 5)), ((0, 4), (1, 3), (2, 5)), ((0, 4), (1, 5), (2, 3)),
+                ((0, 5), (1, 2), (3, 4)), ((0, 5), (1, 3), (2, 4)), ((0, 5), (1, 4), (2, 3))]
+
+
+def _r56_center(D, w, r):
+    """six_disks_can_cover_width_r56_center (six_disks_cover_width.cu:29-90):
+    pairs r1/r4 at the bottom and r2/r3 at the top of a strip of width w,
+    r5 and r6 in the band between at its left and right."""
+    if w == 0:  # 2 / w is infinite in the reference: the first pair's height is 0
+        return False
+    h14 = two_disks_maximize_height(D, r[0], r[3], w)
+    h23 = two_disks_maximize_height(D, r[1], r[2], w)
+    if h14 == 0 or h23 == 0:
+        return False
+    hrem = 1 - (h14 + h23)
+    q = hrem * hrem / 4
+    x5, x6 = r[4] - q, r[5] - q
+    if D.cmp(x5, "<=", 0) or D.cmp(x6, "<=", 0):
+        return False
+    x5, x6 = mpmath.sqrt(x5), mpmath.sqrt(x6)
+    if D.cmp(2 * (x5 + x6), ">=", w):
+        return True
+    x6 = w - x6
+    y56, y14, y23 = h14 + hrem / 2, h14 / 2, 1 - h23 / 2
+    a1, a2 = r[0] - h14 * h14 / 4, r[1] - h23 * h23 / 4
+    if a1 < 0 or a2 < 0:  # NaN in the reference
+        return False
+    w1, w2 = mpmath.sqrt(a1), mpmath.sqrt(a2)
+    p1, p2 = (w - w1, y14), (w - w2, y23)
+    ps1, ps2 = (w - 2 * w1, h14), (w - 2 * w2, 1 - h23)
+    ok, a, b = circle_intersection(D, p1 + (r[0],), p2 + (r[1],))
+    if not ok or not D.cmp(a[0], "<", b[0]):
+        return False
+    p5, p6 = (x5, y56), (x6, y56)
+    return (D.cmp(dist2(p5, ps1), "<=", r[4]) and D.cmp(dist2(p5, ps2), "<=", r[4])
+            and D.cmp(dist2(p5, a), "<=", r[4]) and D.cmp(dist2(p6, b), "<=", r[5]))
+
+
+def six_disks_covered_width(D, la, r):
+    """six_disks_maximize_covered_width (six_disks_cover_width.cu:186-234):
+    the widest full-height strip r1..r6 cover as three stacked pairs, two
+    three-disk columns or the r5/r6-centre placement (by bisection up to
+    lambda), and whether the placement already works at lambda."""
+    w = mpf(0)
+    for (a, b), (c, d), (e, f) in PAIRINGS_3x2:
+        w = max(w, sum(two_disks_maximize_height(D, r[x], r[y], mpf(1)) for x, y in ((a, b), (c, d), (e, f))))
+    for first in ((0, 1, 2), (0, 1, 3), (0, 1, 4), (0, 1, 5), (0, 2, 3), (0, 2, 4), (0, 2, 5), (0, 3, 4),
+                  (0, 3, 5), (0, 4, 5)):
+        second = tuple(i for i in range(6) if i not in first)
+        w = max(w, three_disks_height(D, mpf(1), [r[i] for i in first])
+                + three_disks_height(D, mpf(1), [r[i] for i in second]))
+    if _r56_center(D, w, r):
+        # the reference bisects up to the double above lambda: a placement
+        # that works at lambda itself ends the search at lambda
+        if _r56_center(D, la, r):
+            return la, True
+        w, _ = _bisect(D, lambda D_, x: _r56_center(D_, x, r), w, la)
+    return w, False
+
+
+def rsb_c14_seven_disk(D, la, r):
+    """C14, seven_disk_strategies (:1305-1405).  Strategies: 1 r1..r6 cover
+    the width, 2 the r7 range is empty from the start, 3 after the T-shaped
+    restriction, 4 after the four-disk strip, 5 every one of the 16 r7 pieces
+    is covered (by r7 in the corner, a three-disk top strip, the 2-2-2
+    placement, the r4-r3-r5 strip, the different-height cover or the
+    five-disk height)."""
+    R = max(rsb_R(la, r), mpf(0))
+    Rub = rsb_R(la, r)
+    wc, reaches = six_disks_covered_width(D, la, r)
+    if reaches or D.cmp(wc, ">=", la):
+        return True, 1
+    wrem = max(la - wc, mpf(0))
+    r7 = _r7_range(D, la, r, R, Rub, wrem)
+    if _empty(D, r7):
+        return True, 2
+    _t_shaped(D, r7, la, r, R)
+    if _empty(D, r7):
+        return True, 3
+    _four_disk_strip_r1_square(D, r7, la, r, R)
+    if _empty(D, r7):
+        return True, 4
+    h34 = two_disks_maximize_height(D, r[2], r[3], la)
+    heights = {}
+    used = set()
+    for i in range(16):
+        a = r7[0] + (r7[1] - r7[0]) * i / 16
+        b = r7[0] + (r7[1] - r7[0]) * (i + 1) / 16
+        piece = [a, b]
+        checks = (lambda: _r7_in_corner(D, R, piece, wrem),
+                  lambda: _three_disk_top_strip(D, la, r, R, piece, heights),
+                  lambda: _six_222(D, la, r, R, piece),
+                  lambda: _r435_strip(D, la, r, R, piece),
+                  lambda: _different_height(D, la, r, R, piece))
+        for s, f in enumerate(checks, 1):
+            if f():
+                used.add(s)
+                break
+        else:
+            if h34 > 0:
+                h5 = _five_disks_23(D, la, r[0], r[1], r[4], r[5], a)
+                if D.cmp(h34 + h5, ">=", 1):
+                    used.add(6)
+                    continue
+            return False, 0
+    return True, 5 + 10 * sum(1 << (s - 1) for s in used)
+
+
+C14_PIECE_STRATEGIES = ("r7 in the corner", "three-disk top strip", "2-2-2 placement", "r4-r3-r5 strip",
+                        "different heights", "five-disk height")
+
+
+def c14_stages(la, r, pieces, stages=True, strategies=range(6)):
+    """C14's stages on the point cell, each with the slack of its own
+    decisions: ((width r1..r6 cover, reaches lambda, slack), [(r7 range after
+    compute_r7_range / the T-shaped restriction / the four-disk strip, slack)]
+    (left out unless `stages`), [[(verdict, slack) of each of the six piece
+    strategies] per r7 piece] (those not in `strategies` with slack 0))."""
+    la = mpf(la)
+    r = [mpf(x) for x in r]
+    R, Rub = max(rsb_R(la, r), mpf(0)), rsb_R(la, r)
+    D = Dec()
+    wc, reaches = six_disks_covered_width(D, la, r)
+    width = (wc, reaches, D.slack)
+    wrem = max(la - wc, mpf(0))
+    ranges = []
+    if reaches or wc >= la or not stages:  # C14 ends here; no r7 range is formed
+        # (tighten_lb(0) leaves the remaining width's upper bound la - wc)
+        return width, ranges, _c14_piece_rows(la, r, R, la - wc, pieces, D, strategies)
+    r7 = _r7_range(D, la, r, R, Rub, wrem)
+    ranges.append((list(r7), D.slack))
+    if not (r7[0] > r7[1]):
+        _t_shaped(D, r7, la, r, R)
+    ranges.append((list(r7), D.slack))
+    if not (r7[0] > r7[1]):
+        _four_disk_strip_r1_square(D, r7, la, r, R)
+    ranges.append((list(r7), D.slack))
+    return width, ranges, _c14_piece_rows(la, r, R, wrem, pieces, D, strategies)
+
+
+def _c14_piece_rows(la, r, R, wrem, pieces, D, strategies):
+    out = []
+    for a, b in pieces:
+        piece = [mpf(a), mpf(b)]
+        row = [(False, mpf(0))] * 6  # strategies not evaluated: no slack
+        for k in strategies:
+            Dk = Dec()
+            if k == 0:
+                v = _r7_in_corner(Dk, R, piece, wrem)
+            elif k == 1:
+                v = _three_disk_top_strip(Dk, la, r, R, piece, {})
+            elif k == 2:
+                v = _six_222(Dk, la, r, R, piece)
+            elif k == 3:
+                v = _r435_strip(Dk, la, r, R, piece)
+            elif k == 4:
+                v = _different_height(Dk, la, r, R, piece)
+            else:
+                h34 = two_disks_maximize_height(Dk, r[2], r[3], la)
+                v = h34 > 0 and Dk.cmp(h34 + _five_disks_23(Dk, la, r[0], r[1], r[4], r[5], piece[0]), ">=", 1)
+            row[k] = (bool(v), min(Dk.slack, D.slack if k == 0 else INF))
+        out.append(row)
     return out

@@ -17,6 +17,10 @@ slack is as small as 1e-8.  A mutation check plants plausible slips in the
 exact statement and requires each to be caught.  (The GPU kernels are
 compared with the oracle bit for bit by tests/test_c2_batch.py.)
 """
+import json
+import multiprocessing
+import os
+
 import mpmath
 import numpy as np
 import pytest
@@ -591,3 +595,185 @@ def test_tr_leaf_criteria_equal_exact_at_clear_slack():
         v, sl = tr_leaf_exact(P[sample], {k: f}, only={k})[k]
         o = ((m[sample] >> k) & 1).astype(bool)
         assert ((sl >= CLEAR) & (v != o)).any(), f"slip in bit {k} not caught"
+
+
+# ------------------------------------------- rectangle_size_bound leaf step C14
+
+C14_FIXTURE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "c14_cells.json")
+C14_RANGE_TOL = 1e-7  # relative; the range bounds are bisection results of the reference's recursions
+
+
+def c14_fixture():
+    with open(C14_FIXTURE) as f:
+        d = json.load(f)
+    return tuple(np.array([[float.fromhex(x) for x in row] for row in d[k]])
+                 for k in ("bnb", "different_heights", "two_two_two"))
+
+
+def _cells14(P):
+    cells = np.zeros((len(P), 14))
+    cells[:, 0] = cells[:, 1] = P[:, 0]
+    for k in range(6):
+        cells[:, 2 + 2 * k] = cells[:, 3 + 2 * k] = P[:, 1 + k]
+    return cells
+
+
+def _c14_verdict(p):
+    D = ce.Dec()
+    v, s = ce.rsb_c14_seven_disk(D, mpmath.mpf(p[0]), [mpmath.mpf(x) for x in p[1:]])
+    return bool(v), s, float(D.slack)
+
+
+def _c14_stages(args):
+    """Exact stages of one cell; None where a planted slip leaves the reals
+    (a square root of a negative number): that cell is then left out."""
+    p, pieces, stages, strategies = args if len(args) == 4 else (*args, True, range(6))
+    try:
+        (w, reaches, wsl), ranges, rows = ce.c14_stages(p[0], p[1:], pieces, stages, strategies)
+    except (TypeError, ValueError, ZeroDivisionError):
+        return None
+    return ((float(w), reaches, float(wsl)), [([float(x) for x in rg], float(sl)) for rg, sl in ranges],
+            [[(v, float(sl)) for v, sl in row] for row in rows])
+
+
+def _pool_map(fn, items):
+    """fn over items on forked workers (which see the exact statement as the
+    parent has it, planted slips included)."""
+    with multiprocessing.get_context("fork").Pool(min(8, os.cpu_count() or 1)) as pool:
+        return pool.map(fn, items, chunksize=2)
+
+
+def c14_pieces(P, n, seed):
+    """n seeded r7 pieces [a, b] in [0, min(r6, R)] per cell."""
+    rng = np.random.default_rng(seed)
+    top = np.minimum(P[:, 6], P[:, 0] * 0.61 - P[:, 1:].sum(axis=1))
+    a = rng.uniform(0, 1, (len(P), n)) * top[:, None]
+    b = a + rng.uniform(0, 1, (len(P), n)) * (top[:, None] - a)
+    return np.stack([a, b], axis=2)
+
+
+def c14_stage_inputs():
+    bnb, dh, t222 = c14_fixture()
+    P = np.concatenate([bnb[:96], dh[:, :7], t222[:, :7]])
+    pieces = np.concatenate([c14_pieces(bnb[:96], 4, 0xC14), np.repeat(dh[:, None, 7:9], 4, axis=1),
+                             np.repeat(t222[:, None, 7:9], 4, axis=1)])
+    return P, pieces
+
+
+def c14_stage_mismatches(P, pieces, res, out, bits):
+    """(width/range mismatches, piece mismatches, per-strategy [rejected,
+    covered] counts at clear slack) of exact stages against the oracle's."""
+    stage_bad, piece_bad = [], []
+    cnt = np.zeros((6, 2), int)
+    for i, r in enumerate(res):
+        if r is None:
+            continue
+        (w, reaches, wsl), ranges, rows = r
+        if wsl >= CLEAR and not reaches and abs(w - out[i, 0]) > 1e-9 * w:
+            stage_bad.append((i, "width", w, out[i, 0]))
+        for s, (rg, sl) in enumerate(ranges):
+            if sl < CLEAR:
+                continue
+            lo, hi = out[i, 1 + 2 * s], out[i, 2 + 2 * s]
+            if (rg[0] > rg[1]) != (lo > hi):
+                stage_bad.append((i, s, rg, lo, hi))
+            elif rg[0] <= rg[1] and (abs(rg[0] - lo) > C14_RANGE_TOL * rg[1] or abs(rg[1] - hi) > 1e-12 * rg[1]):
+                stage_bad.append((i, s, rg, lo, hi))
+        for j, row in enumerate(rows):
+            for k, (v, sl) in enumerate(row):
+                if sl < CLEAR:
+                    continue
+                if bool(bits[i, j] >> k & 1) != v:
+                    piece_bad.append((i, j, k))
+                cnt[k, int(v)] += 1
+    return stage_bad, piece_bad, cnt
+
+
+def test_c14_verdicts_equal_exact_at_clear_slack():
+    """C14 (seven_disk_strategies.cu:1305-1405) on the centres of cells that
+    reach it in the branch-and-bound: the oracle's per-step bit 19 against
+    the exact statement, both ways, at clear slack; it proves and rejects at
+    least 15 cells, and each way it proves (r1..r6 spanning lambda, an r7
+    range empty from the start, after the T-shaped restriction, after the
+    four-disk strip, every r7 piece covered) decides some cell."""
+    bnb = c14_fixture()[0]
+    mask = oracle.rsb_masks(_cells14(bnb))
+    res = _pool_map(_c14_verdict, list(bnb))
+    seen, cnt = set(), [0, 0]
+    for i, (v, s, sl) in enumerate(res):
+        if sl < CLEAR:
+            continue
+        assert bool(mask[i] >> 19 & 1) == v, (list(bnb[i]), v, s, sl)
+        cnt[int(v)] += 1
+        if v:
+            seen.add(min(s, 5))
+    assert min(cnt) >= 15, cnt
+    assert seen == {1, 2, 3, 4, 5}, seen
+
+
+def test_c14_stages_equal_exact():
+    """C14's stages, each on its own: the width r1..r6 cover, the r7 range
+    after compute_r7_range, the T-shaped restriction and the four-disk strip
+    (emptiness, and bounds within 1e-7 of the range), and each of the six
+    piece strategies on seeded r7 pieces (r7 in the corner, three-disk top
+    strip, 2-2-2, r4-r3-r5 strip, different heights, five-disk height), the
+    oracle's against the exact statement at clear slack; every piece strategy
+    covers some pieces and fails on others."""
+    P, pieces = c14_stage_inputs()
+    out, bits = oracle.rsb_c14_probe(_cells14(P), pieces)
+    res = _pool_map(_c14_stages, list(zip(P, pieces)))
+    assert all(r is not None for r in res)
+    stage_bad, piece_bad, cnt = c14_stage_mismatches(P, pieces, res, out, bits)
+    assert not stage_bad, stage_bad[:3]
+    assert not piece_bad, piece_bad[:5]
+    assert (cnt[:, 1] >= 3).all() and (cnt[:, 0] >= 15).all(), cnt.tolist()
+
+
+def _swap(r, i, j):
+    r = list(r)
+    r[i], r[j] = r[j], r[i]
+    return r
+
+
+C14_SLIPS = {
+    # name: (module attribute, replacement factory, "stages" or the piece strategy compared)
+    "r7 in the corner: remaining height 1 + sqrt(h)": (
+        "_r7_in_corner", lambda f: (lambda D, R, r7, w: (lambda h: False if D.cmp(h, "<", 0) else ce.rsb_can_recurse(
+            D, ce._rest(R, r7[1]), r7[1], w, 1 + mpmath.sqrt(h)))(4 * r7[0] - w * w)), 0),
+    "top strip: r7's lower bound taken for its upper bound": (
+        "_three_disk_top_strip", lambda f: (lambda D, la, r, R, r7, heights: f(D, la, r, R, [r7[0], r7[0]],
+                                                                               heights)), 1),
+    "2-2-2: pairs r1/r3 and r2/r4 for r1/r2 and r3/r4": ("_six_222", lambda f: (lambda D, la, r, R, r7: f(
+        D, la, _swap(r, 1, 2), R, r7)), 2),
+    "r4-r3-r5 strip: r3 and r4 swapped": ("_r435_strip", lambda f: (lambda D, la, r, R, r7: f(D, la, _swap(r, 2, 3),
+                                                                                             R, r7)), 3),
+    "different heights: r1 and r2 swapped": ("_different_height", lambda f: (lambda D, la, r, R, r7: f(
+        D, la, _swap(r, 0, 1), R, r7)), 4),
+    "five-disk height: m1 and m2 swapped": ("_five_disks_23", lambda f: (lambda D, la, t1, t2, m1, m2, m3: f(
+        D, la, t1, t2, m2, m1, m3)), 5),
+    "r7 range: efficiency over lambda instead of the remaining width": (
+        "_r7_range", lambda f: (lambda D, la, r, R, Rub, wrem: f(D, la, r, R, Rub, la)), "stages"),
+    "T-shaped restriction: r1 and r2 swapped": ("_t_shaped", lambda f: (lambda D, r7, la, r, R: f(
+        D, r7, la, _swap(r, 0, 1), R)), "stages"),
+    "four-disk strip: r1 and r2 swapped": ("_four_disk_strip_r1_square", lambda f: (lambda D, r7, la, r, R: f(
+        D, r7, la, _swap(r, 0, 1), R)), "stages"),
+    "six-disk width: r5 and r6 swapped": ("six_disks_covered_width", lambda f: (lambda D, la, r: f(
+        D, la, _swap(r, 4, 5))), "stages"),
+}
+
+
+def test_c14_planted_slips_are_caught():
+    """A plausible slip planted in each of C14's parts disagrees with the
+    oracle at clear slack (widths and ranges, or piece strategies)."""
+    P, pieces = c14_stage_inputs()
+    out, bits = oracle.rsb_c14_probe(_cells14(P), pieces)
+    for name, (attr, make, what) in C14_SLIPS.items():
+        saved = getattr(ce, attr)
+        setattr(ce, attr, make(saved))
+        try:
+            stages = what == "stages"
+            res = _pool_map(_c14_stages, [(p, pc, stages, () if stages else (what,)) for p, pc in zip(P, pieces)])
+        finally:
+            setattr(ce, attr, saved)
+        stage_bad, piece_bad, _ = c14_stage_mismatches(P, pieces, res, out, bits)
+        assert (stage_bad if stages else piece_bad), f"slip not caught: {name}"
