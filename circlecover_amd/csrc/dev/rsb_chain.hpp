@@ -18,6 +18,12 @@
 #define CC_EXACT_SHORTCUTS 0
 #endif
 #include "rsb.hpp"
+// The leaf chain's larger steps are out-of-line functions (smaller code for
+// the kernels that run few leaves); CC_CHAIN_FN inlines them where a kernel
+// runs the chain on very many leaves.
+#ifndef CC_CHAIN_FN
+#define CC_CHAIN_FN static __device__ __noinline__
+#endif
 
 namespace ccd {
 
@@ -501,7 +507,7 @@ CCD bool stacked_34_horizontal(const cell7& c) {
   return rem >= 1.0 || disk_fits(c.r[4].hi, rem);
 }
 // multi_disk_strip.cu:95-145
-static __device__ __noinline__ bool stacked_34_56_horizontal(const cell7& c) {
+CC_CHAIN_FN bool stacked_34_56_horizontal(const cell7& c) {
   double wgt = 0.0;
   for (int j = 0; j < 6; ++j) wgt = add_ru(wgt, c.r[j].hi);
   double w = div_ru(wgt, kRsbRatio);
@@ -527,7 +533,7 @@ static __device__ __noinline__ bool stacked_34_56_horizontal(const cell7& c) {
   return rem >= 1.0 || disk_fits(c.r[5].hi, rem);
 }
 // multi_disk_strip.cu:147-240 (lb_h14 / lb_h23 take sqrt twice, as the reference)
-static __device__ __noinline__ bool six_strip_three_rows(const cell7& c, const vals7& v, Trap& tp) {
+CC_CHAIN_FN bool six_strip_three_rows(const cell7& c, const vals7& v, Trap& tp) {
   double wgt = 0.0;
   for (int i = 0; i < 6; ++i) wgt = add_ru(wgt, c.r[i].hi);
   double wn = div_ru(wgt, kRsbRatio);
@@ -612,7 +618,7 @@ CCD bool c4_wall_building(const Konst& K, const cell7& c, const vals7& v) {
 // r1_in_corner_explicit_recursion.cu:36-187: r1 in the corner, the other
 // disks stacked above it (right = false) or beside it (right = true) in
 // backtracked groups.
-static __device__ __noinline__ bool corner_explicit(const cell7& c, const vals7& v, int nd, bool right) {
+CC_CHAIN_FN bool corner_explicit(const cell7& c, const vals7& v, int nd, bool right) {
   double lbs[6];
 #pragma unroll
   for (int i = 0; i < 6; ++i) lbs[i] = c.r[i].lo;
@@ -764,7 +770,7 @@ CCD size_check opposite_sizes(const cell7& c, const vals7& v, iv wA, iv hA, iv w
   return size_check{r7, r7, true};
 }
 // r1_r2_opposite_corners_recursion.cu:254-340
-static __device__ __noinline__ bool c6_opposite_corners(const cell7& c, const vals7& v, Trap& tp) {
+CC_CHAIN_FN bool c6_opposite_corners(const cell7& c, const vals7& v, Trap& tp) {
   iv w1{sqrt_rd(mul_rd(2.0, c.r[0].lo)), sqrt_ru(mul_ru(2.0, c.r[0].hi))};
   iv w2{sqrt_rd(mul_rd(2.0, c.r[1].lo)), sqrt_ru(mul_ru(2.0, c.r[1].hi))};
   iv both = w1 + w2;

@@ -45,7 +45,7 @@ CCD cover22 two_by_two_concrete(double r1, double r2, double r3, double r4, Trap
   return res;
 }
 // two_by_two_cover.cu:78-88
-static __device__ __noinline__ cover22 two_by_two(const cell7& c, Trap& tp) {
+CC_CHAIN_FN cover22 two_by_two(const cell7& c, Trap& tp) {
   cover22 a = two_by_two_concrete(c.r[0].lo, c.r[1].lo, c.r[2].lo, c.r[3].lo, tp);
   cover22 b = two_by_two_concrete(c.r[0].lo, c.r[2].lo, c.r[1].lo, c.r[3].lo, tp);
   cover22 d = two_by_two_concrete(c.r[0].lo, c.r[3].lo, c.r[1].lo, c.r[2].lo, tp);
@@ -55,7 +55,7 @@ static __device__ __noinline__ cover22 two_by_two(const cell7& c, Trap& tp) {
 
 // ------------------------------------------------------------------ C8
 // two_by_two_cover_with_strip_and_recursion.cu:26-105
-static __device__ __noinline__ bool c8_two_by_two_strip(const Konst& K, const cell7& c, const vals7& v, Trap& tp) {
+CC_CHAIN_FN bool c8_two_by_two_strip(const Konst& K, const cell7& c, const vals7& v, Trap& tp) {
   const double r5 = c.r[4].lo, r6 = c.r[5].lo, R = v.R.lo;
   double ub_w = add_ru(c.la.hi, -v.cover_width);
   double R5 = mul_rd(kRsbRatio, c.la.lo);
@@ -96,7 +96,7 @@ static __device__ __noinline__ bool c8_two_by_two_strip(const Konst& K, const ce
 
 // ------------------------------------------------------------------ C9
 // l_shaped_recursion.cu:28-120
-static __device__ __noinline__ bool l_one_horizontal(const Konst& K, const cell7& c, double ub_w, double lb_weight) {
+CC_CHAIN_FN bool l_one_horizontal(const Konst& K, const cell7& c, double ub_w, double lb_weight) {
   double R4 = add_rd(lb_weight, -c.r[2].hi);
   double side3 = sqrt_rd(mul_rd(2.0, c.r[2].lo));
   iv h_top = pt(1.0) - side3;
@@ -159,7 +159,7 @@ CCD bool c9_l_shaped(const Konst& K, const cell7& c) {
 
 // ------------------------------------------------------------------ C10
 // r1_in_corner_wall_building_recursion.cu:28-149
-static __device__ __noinline__ bool c10_corner_wall(const Konst& K, const cell7& c, Trap& tp) {
+CC_CHAIN_FN bool c10_corner_wall(const Konst& K, const cell7& c, Trap& tp) {
   iv w1 = sqrt(2.0 * c.r[0], tp);
   double extra_r1 = mul_rd(add_rd(mul_rd(2.0, kRsbRatio), -1.0), c.r[0].lo);
   iv wB = c.la - w1;
@@ -217,7 +217,7 @@ static __device__ __noinline__ bool c10_corner_wall(const Konst& K, const cell7&
 
 // ------------------------------------------------------------------ C11
 // multi_disk_strip.cu:264-317: all 62 nonempty proper disk subsets as one strip.
-static __device__ __noinline__ bool c11_advanced_strip(const cell7& c) {
+CC_CHAIN_FN bool c11_advanced_strip(const cell7& c) {
   for (unsigned s = 1; s < 63; ++s) {
     double wgt = 0.0;
     for (int j = 0; j < 6; ++j)
@@ -248,7 +248,7 @@ static __device__ __noinline__ bool c11_advanced_strip(const cell7& c) {
 
 // ------------------------------------------------------------------ C12
 // five_disk_full_cover.cu:91-177
-static __device__ __noinline__ bool five_3v2_center(const Konst& K, const cell7& c, const vals7& v, Trap& tp) {
+CC_CHAIN_FN bool five_3v2_center(const Konst& K, const cell7& c, const vals7& v, Trap& tp) {
   const double la = c.la.hi;
   iv R6 = c.r[5] + v.R;
   strip2 bot = max_height_strip(la, c.r[0].lo, c.r[1].lo, tp);
@@ -313,7 +313,7 @@ CCD bool c12_five_disk(const Konst& K, const cell7& c, const vals7& v, Trap& tp)
 
 // ------------------------------------------------------------------ C13
 // six_disk_full_cover.cu:29-108
-static __device__ __noinline__ bool six_4_plus_2(const cell7& c, Trap& tp) {
+CC_CHAIN_FN bool six_4_plus_2(const cell7& c, Trap& tp) {
   const double la = c.la.hi;
   strip2 bot = max_height_strip(la, c.r[0].lo, c.r[1].lo, tp);
   if (bot.h <= 0.0) return false;
@@ -356,7 +356,7 @@ static __device__ __noinline__ bool six_4_plus_2(const cell7& c, Trap& tp) {
   return false;
 }
 // six_disk_full_cover.cu:110-167
-static __device__ __noinline__ bool six_2v3_r6_center(const cell7& c, const vals7& v, Trap& tp) {
+CC_CHAIN_FN bool six_2v3_r6_center(const cell7& c, const vals7& v, Trap& tp) {
   strip2 bot = max_height_strip_wc(c.r[0].lo, c.r[1].lo, v.R.lo, c.la.hi);
   if (bot.h <= 0.0) return false;
   double hrem = add_ru(1.0, -bot.h);
