@@ -164,10 +164,9 @@ CCD point midpoint(const point& a, const point& b) { return point{a.x + 0.5 * (b
 // i-th of n near-equal pieces of x, the last snapped to x.hi
 // (reference Prover/subintervals.hpp:32-43, lb + diff*i fused).
 CCD iv subinterval(iv x, int i, int n) {
-  double d = rn_span_div(x.hi, x.lo, (double)n);
-  double l = rn_fma(d, (double)i, x.lo);
-  double u = (i + 1 == n) ? x.hi : rn_fma(d, (double)(i + 1), x.lo);
-  return iv{l, u};
+  double l, u;
+  rn_piece(x.hi, x.lo, (double)n, (double)i, (double)(i + 1), l, u);
+  return iv{l, (i + 1 == n) ? x.hi : u};
 }
 
 }  // namespace ccd

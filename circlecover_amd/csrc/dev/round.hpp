@@ -130,6 +130,54 @@ CCD double rn_span_div(double hi, double lo, double n) {
   return r;
 }
 
+// fma(d, i, lo) and fma(d, j, lo) rounded to nearest-even under one mode
+// switch: the two bounds of the reference's `lb + diff*i`, `lb + diff*(i+1)`.
+CCD void rn_fma2(double d, double i, double j, double lo, double& l, double& u) {
+  asm volatile(
+      "s_nop 1\n\t"
+      "s_setreg_imm32_b32 hwreg(HW_REG_MODE, 0, 4), 0\n\t"
+      "s_nop 1\n\t"
+      "v_fma_f64 %0, %2, %3, %5\n\t"
+      "v_fma_f64 %1, %2, %4, %5\n\t"
+      "s_nop 1\n\t"
+      "s_setreg_imm32_b32 hwreg(HW_REG_MODE, 0, 4), 10\n\t"
+      "s_nop 1"
+      : "=&v"(l), "=&v"(u)
+      : "v"(d), "v"(i), "v"(j), "v"(lo));
+}
+// get_subinterval's three nearest-even operations under one mode switch:
+// diff = (hi - lo) / n (the scaled Newton division of rn_span_div), then
+// l = fma(diff, i, lo) and u = fma(diff, i + 1, lo).
+CCD void rn_piece(double hi, double lo, double n, double i, double j, double& l, double& u) {
+  double t0, t1, t2, t3, t4, t5;
+  asm volatile(
+      "s_nop 1\n\t"
+      "s_setreg_imm32_b32 hwreg(HW_REG_MODE, 0, 4), 0\n\t"
+      "s_nop 1\n\t"
+      "v_add_f64 %[a], %[hi], -%[lo]\n\t"
+      "v_div_scale_f64 %[d], vcc, %[n], %[n], %[a]\n\t"
+      "v_rcp_f64 %[y], %[d]\n\t"
+      "v_div_scale_f64 %[s], vcc, %[a], %[n], %[a]\n\t"
+      "v_fma_f64 %[t], -%[d], %[y], 1.0\n\t"
+      "v_fma_f64 %[y], %[y], %[t], %[y]\n\t"
+      "v_fma_f64 %[t], -%[d], %[y], 1.0\n\t"
+      "v_fma_f64 %[y], %[y], %[t], %[y]\n\t"
+      "v_mul_f64 %[t], %[s], %[y]\n\t"
+      "v_fma_f64 %[d], -%[d], %[t], %[s]\n\t"
+      "s_nop 4\n\t"
+      "v_div_fmas_f64 %[d], %[d], %[y], %[t]\n\t"
+      "v_div_fixup_f64 %[q], %[d], %[n], %[a]\n\t"
+      "v_fma_f64 %[l], %[q], %[i], %[lo]\n\t"
+      "v_fma_f64 %[u], %[q], %[j], %[lo]\n\t"
+      "s_nop 1\n\t"
+      "s_setreg_imm32_b32 hwreg(HW_REG_MODE, 0, 4), 10\n\t"
+      "s_nop 1"
+      : [l] "=&v"(l), [u] "=&v"(u), [a] "=&v"(t0), [d] "=&v"(t1), [y] "=&v"(t2), [s] "=&v"(t3), [t] "=&v"(t4),
+        [q] "=&v"(t5)
+      : [hi] "v"(hi), [lo] "v"(lo), [n] "v"(n), [i] "v"(i), [j] "v"(j)
+      : "vcc");
+}
+
 // ------------------------------------------------------------ neighbours
 // Next representable double toward +inf / -inf for finite nonzero x.
 CCD double next_up(double x) {

@@ -132,9 +132,9 @@ extern "C" __global__ void __launch_bounds__(kRowsBlock) ccrd_rsb_enum_rows(cons
   int open_prefix = -1;
   if (live && d6 > 2.0 * (next_up(r6.hi) - r6.hi)) {
     auto open_at = [&](int i) {
-      const double l6 = rn_fma(d6, (double)i, r6.lo);
-      const double u6 = i + 1 == n6 ? r6.hi : rn_fma(d6, (double)(i + 1), r6.lo);
-      v.r[5] = iv{l6, u6};
+      double l6, u6;
+      rn_fma2(d6, (double)i, (double)(i + 1), r6.lo, l6, u6);
+      v.r[5] = iv{l6, i + 1 == n6 ? r6.hi : u6};
       const cell7 c = rsbc::tighten(v);
       return !(rsbc::is_empty(c) || empty(rsbc::remaining_weight(c)));
     };
