@@ -80,6 +80,49 @@ CCD cell7 leaf_of(const rsb_rec* __restrict__ cells, const rsb_s3& s, int i4, in
   return rsbc::tighten(v);
 }
 
+// Estimate (any rounding) of the number of open leaves in a row: 0 when the
+// row's own part is empty, else the first r6 piece index whose lower bound
+// reaches the least of the thresholds on l6 at which the tightened leaf
+// becomes empty (l6 >= min(hi of r2..r5)) or its remaining weight's upper
+// bound la.hi * 0.61 - r1.lo - sum of the tightened lower bounds of r2..r6
+// falls below 0.  v holds the row's la, r1..r5 (r5 and r4 as their pieces).
+// A CPU check against the oracle's emptiness verdicts matched all of 400
+// sampled rows of the proof grid.
+CCD int estimate_open_prefix(const cell7& v, double lo6, double d6, int n6) {
+  double H = v.r[1].hi;
+  for (int k = 2; k < 5; ++k) H = v.r[k].hi < H ? v.r[k].hi : H;
+  // m[k]: the tightened lower bound of r(k+2) for l6 below it, non-increasing in k
+  double m[4];
+  double run = 0.0;
+  for (int k = 3; k >= 0; --k) {
+    run = v.r[k + 1].lo > run ? v.r[k + 1].lo : run;
+    m[k] = run;
+  }
+  // a row whose own part is empty (r1, or r2..r5 with the row's tightened
+  // lower bounds) has no open leaf
+  if (v.r[0].lo >= v.r[0].hi) return 0;
+  for (int k = 0; k < 4; ++k)
+    if (m[k] >= v.r[k + 1].hi) return 0;
+  double rest = v.la.hi * kRsbRatio - v.r[0].lo;
+  double T = H;
+  // l6 in [m[k], m[k-1]]: r2..r(k+1) keep m, the other 4 - k and r6 take l6
+  double fixed = 0.0;
+  for (int k = 0; k <= 4; ++k) {
+    const double root = (rest - fixed) / (double)(5 - k);
+    const double lo_seg = k < 4 ? m[k] : -1e300;
+    if (root >= lo_seg) {
+      T = root < T ? root : T;
+      break;
+    }
+    if (k < 4) fixed += m[k];
+  }
+  if (!(T > lo6)) return 0;
+  const double x = (T - lo6) / d6;
+  if (!(x < (double)n6)) return n6;
+  const int g = (int)ceil(x);
+  return g < 0 ? 0 : g > n6 ? n6 : g;
+}
+
 // Rows of the level-A records [rec0, rec0 + nrec) (n4, n5, n6 <= 256), in
 // workgroups of 1024 lanes.  The list has room for every leaf of those
 // records.  One atomic per workgroup claims its part of the list: nearly
@@ -139,6 +182,12 @@ extern "C" __global__ void __launch_bounds__(kRowsBlock) ccrd_rsb_enum_rows(cons
       return !(rsbc::is_empty(c) || empty(rsbc::remaining_weight(c)));
     };
     int a = 0, b = n6;  // open on [0, a), closed on [b, n6)
+    // The prefix's end estimated from the two thresholds on l6 (the least
+    // upper bound H of r2..r5, and where R.hi, a piecewise linear function
+    // of l6, reaches 0), then confirmed by the full test on the pieces
+    // either side of it; an unconfirmed estimate falls back to the search.
+    const int g = estimate_open_prefix(v, r6.lo, d6, n6);
+    if ((g == 0 || open_at(g - 1)) && (g == n6 || !open_at(g))) a = b = g;
     while (a < b) {
       const int mid = (a + b) >> 1;
       if (open_at(mid)) a = mid + 1;
