@@ -35,7 +35,10 @@ configurations measured in the same run (not part of `value`):
                plain cascade, ranked and ranked cascade (same proven set);
   full_proof   both of the reference's proofs end to end through the drop-in
                executable (`circlecover_prover --only tr|rsb`), wall time and
-               number of critical intervals left.
+               number of critical intervals left; then tight_rectangle in full
+               and rectangle_size_bound on lambda pieces 24..27 in the three
+               enumeration forms (batched, plain = the reference's kernel
+               shape and operation sequence, plain-exact), same reports.
 """
 from __future__ import annotations
 
@@ -430,6 +433,9 @@ def log(msg: str) -> None:
     print(f"bench.py: {msg}", file=sys.stderr, flush=True)
 
 
+RSB_SLICE = "24,4"  # lambda pieces 24..27 of the rectangle_size_bound proof grid (65,536 cells)
+
+
 def full_proof() -> dict:
     """Both of the reference's proofs, end to end, through the drop-in
     executable (reference constants: tight_rectangle 256 lambda launches of
@@ -437,19 +443,45 @@ def full_proof() -> dict:
     256x128x128 grid with 64^4 pieces per cell plus one generation).  Wall
     time from process start to exit, including context set-up; the critical
     intervals are counted from the stdout report, whose rows sit between the
-    banner and the closing rule."""
+    banner and the closing rule.
+
+    Then the same procedure in the enumeration's three forms
+    (`--enum-form`): batched (the default), plain (the reference's kernel
+    shape and operation sequence: one block per outer piece pair, no
+    compaction) and plain-exact (that shape with the exact shortcuts):
+    tight_rectangle in full, rectangle_size_bound on lambda pieces 24..27 of
+    its grid (the plain form would take about 25 minutes for the whole
+    grid); the reports must be identical."""
     import subprocess
 
     exe = os.path.join(ROOT, "circlecover_amd", "circlecover_prover")
+
+    def run(args):
+        t = time.perf_counter()
+        r = subprocess.run([exe, *args], stdout=subprocess.PIPE, stderr=subprocess.DEVNULL, text=True, timeout=600)
+        return time.perf_counter() - t, r
+
     out = {}
     for proof in ("tr", "rsb"):
-        log(f"full proof {proof} (the prover's progress follows on stderr)")
-        t = time.perf_counter()
-        r = subprocess.run([exe, "--only", proof], stdout=subprocess.PIPE, text=True, timeout=600)
-        wall = time.perf_counter() - t
+        log(f"full proof {proof}")
+        wall, r = run(["--only", proof])
         lines = [ln for ln in r.stdout.splitlines() if ln.strip()]
         rows = max(0, len(lines) - 2) if r.returncode == 0 else None
         out[proof] = {"wall_s": wall, "exit_status": r.returncode, "critical_intervals": rows}
+    forms = {}
+    for name, args in (("tr", ["--only", "tr"]),
+                       ("rsb_slice", ["--only", "rsb", "--rsb-la-range", RSB_SLICE, "--rsb-generations", "0"])):
+        times, reports = {}, {}
+        for form in ("batched", "plain_exact", "plain"):
+            log(f"{name} in the {form} enumeration form")
+            wall, r = run(args + ["--enum-form", form.replace("_", "-")])
+            times[form + "_s"] = wall if r.returncode == 0 else None
+            reports[form] = r.stdout
+        forms[name] = {**times, "args": " ".join(args),
+                       "identical_reports": len(set(reports.values())) == 1,
+                       "batched_vs_plain": times["plain_s"] / times["batched_s"],
+                       "batched_vs_plain_exact": times["plain_exact_s"] / times["batched_s"]}
+    out["forms"] = forms
     return out
 
 
@@ -555,7 +587,15 @@ def main() -> int:
                                                    if "plain_exact_s" in c3 else None),
                 "identical_unproven": c3["identical"]}
         if not args.no_full_proof:
-            workloads["full_proof"] = full_proof()
+            fp = full_proof()
+            workloads["full_proof"] = fp
+            # the metric's full-proof wall time: batched against the plain
+            # (reference-shaped) enumeration, same reports
+            if "comparison" in out:
+                out["comparison"]["full_proof"] = {
+                    k: {f: v.get(f) for f in ("batched_s", "plain_s", "plain_exact_s", "batched_vs_plain",
+                                              "batched_vs_plain_exact", "identical_reports", "args")}
+                    for k, v in fp["forms"].items()}
         out["workloads"] = workloads
     if not args.no_cpu:
         # every CPU this process may use, up to the 16 a one-GPU job is

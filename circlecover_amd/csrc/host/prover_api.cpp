@@ -315,6 +315,14 @@ int tr_launches(cc_ctx* ctx, const std::vector<cc_tr_cell>& launches, bool init,
 
 extern "C" {
 
+int cc_set_enumeration_form(cc_ctx* ctx, int form) {
+  if (!ctx) return CC_E_ARG;
+  if (form != CC_ENUM_BATCHED && form != CC_ENUM_PLAIN && form != CC_ENUM_PLAIN_EXACT)
+    return fail(ctx, CC_E_ARG, "unknown enumeration form");
+  ctx->enum_form = form;
+  return CC_OK;
+}
+
 int cc_tr_prove_shape(cc_ctx* ctx, const int init_counts[4], const int gen_counts[4]) {
   if (!ctx) return CC_E_ARG;
   static const int kInit[4] = {128, 128, 128, 128}, kGen[4] = {128, 128, 128, 64};

@@ -70,6 +70,36 @@ int read_counts(cc_ctx* ctx, int k) {
   return CC_OK;
 }
 
+// The reference's kernel shape (rectangle_size_bound.cu:161-251): a (64, 64)
+// grid of 16x16 blocks per cell, the cells of the batch along the grid's
+// third dimension; leaves in scratch().leaves as enumerate_dev leaves them
+// (counter slots 1 leaves, 2 trap), relaunched once if the buffer was short.
+int enumerate_plain(cc_ctx* ctx, const void* d_cells, size_t ncells, const int n[4], size_t* n_leaves, int* trap) {
+  Scratch& S = scratch(ctx);
+  unsigned long long* cnt = static_cast<unsigned long long*>(ctx->d_counts);
+  int* d_trap = reinterpret_cast<int*>(cnt + 2);
+  hipFunction_t f;
+  CC_HIP(ctx, kernel(ctx, ctx->enum_form == CC_ENUM_PLAIN ? "ccrd_rsb_plain" : "ccrd_rsb_plain_x", &f));
+  if (ncells > 65535) return fail(ctx, CC_E_ARG, "plain enumeration: at most 65535 cells a call");
+  if (!S.leaves.p) CC_HIP(ctx, S.leaves.ensure((size_t)1 << 20));
+  uint64_t cb = 0;
+  int rc;
+  for (int pass = 0; pass < 2; ++pass) {
+    const uint64_t cap = S.leaves.cap / kRec;
+    CC_HIP(ctx, hipMemsetAsync(cnt + 1, 0, sizeof(unsigned long long), ctx->stream));
+    CC_HIP(ctx, launch(f, dim3(64, 64, (unsigned)ncells), dim3(16, 16), 0, ctx->stream, d_cells, n[0], n[1], n[2],
+                       n[3], ctx->d_konst, S.leaves.p, cap, cnt + 1, d_trap));
+    if ((rc = read_counts(ctx, 3))) return rc;
+    cb = ctx->h_counts[1];
+    if (cb <= cap) break;
+    if (pass == 1) return fail(ctx, CC_E_OVERFLOW, "leaf count grew between identical launches");
+    CC_HIP(ctx, S.leaves.ensure(cb * kRec));
+  }
+  *trap = (int)(ctx->h_counts[2] & 0xffffffffu);
+  *n_leaves = cb;
+  return CC_OK;
+}
+
 // Enumeration of the device cells with counts n[4] (r3..r6); leaves land in
 // scratch().leaves.  Counter slots: 0 level A, 1 leaves, 2 trap (int), 3
 // listed non-empty leaves, 4 leaves listed for C14 (both lists 8 bytes per
@@ -90,6 +120,7 @@ int enumerate_dev(cc_ctx* ctx, const void* d_cells, size_t ncells, const int n[4
   *trap = 0;
   CC_HIP(ctx, hipMemsetAsync(cnt, 0, 5 * sizeof(unsigned long long), ctx->stream));
   if (ncells == 0) return CC_OK;
+  if (ctx->enum_form != CC_ENUM_BATCHED) return enumerate_plain(ctx, d_cells, ncells, n, n_leaves, trap);
   for (int k = 1; k < 4; ++k)
     if (n[k] < 1 || n[k] > 256) return fail(ctx, CC_E_ARG, "r4..r6 piece counts must be 1..256");
   const uint64_t nc = ncells, ta = nc * (uint64_t)n[0];

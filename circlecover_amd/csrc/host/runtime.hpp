@@ -47,6 +47,7 @@ struct cc_ctx {
   int tr_init_counts[4] = {128, 128, 128, 128}, tr_gen_counts[4] = {128, 128, 128, 64};
   int rsb_init_counts[4] = {64, 64, 64, 64}, rsb_gen_counts[4] = {16, 16, 16, 32};
   int rsb_gen_grid[3] = {64, 32, 32};
+  int enum_form = 0;  // CC_ENUM_* of both proofs' enumerations (cc_set_enumeration_form)
   std::string err;
 };
 

@@ -28,6 +28,38 @@ int read_counts(cc_ctx* ctx, int k) {
   return CC_OK;
 }
 
+// The reference's kernel shape (tight_rectangle.cu:169-277): a (128, 128)
+// grid of 16x16 blocks per cell, the cells of the batch along the grid's
+// third dimension; same outputs and counter slots as enumerate_dev (3
+// leaves, 4 trap), the same relaunch when the leaf buffer was short.
+int enumerate_plain(cc_ctx* ctx, const void* d_cells, size_t ncells, const int n[4], size_t* n_leaves, int* trap) {
+  Scratch& S = scratch(ctx);
+  unsigned long long* cnt = static_cast<unsigned long long*>(ctx->d_counts);
+  int* d_trap = reinterpret_cast<int*>(cnt + 4);
+  hipFunction_t f;
+  CC_HIP(ctx, kernel(ctx, ctx->enum_form == CC_ENUM_PLAIN ? "ccrd_tr_plain" : "ccrd_tr_plain_x", &f));
+  if (ncells > 65535) return fail(ctx, CC_E_ARG, "plain enumeration: at most 65535 cells a call");
+  if (!S.leaves.p) CC_HIP(ctx, S.leaves.ensure((size_t)1 << 20));
+  uint64_t c4 = 0;
+  int rc;
+  for (int pass = 0; pass < 2; ++pass) {
+    const uint64_t cap = S.leaves.cap / kCell;
+    CC_HIP(ctx, hipMemsetAsync(cnt + 3, 0, sizeof(unsigned long long), ctx->stream));
+    CC_HIP(ctx, S.verdict.ensure(cap * sizeof(uint32_t)));
+    CC_HIP(ctx, S.tr_piece.ensure(cap * sizeof(uint32_t)));
+    CC_HIP(ctx, launch(f, dim3(128, 128, (unsigned)ncells), dim3(16, 16), 0, ctx->stream, d_cells, n[0], n[1], n[2],
+                       n[3], ctx->d_konst, S.leaves.p, cap, S.verdict.p, S.tr_piece.p, cnt + 3, d_trap));
+    if ((rc = read_counts(ctx, 5))) return rc;
+    c4 = ctx->h_counts[3];
+    if (c4 <= cap) break;
+    if (pass == 1) return fail(ctx, CC_E_OVERFLOW, "leaf count grew between identical launches");
+    CC_HIP(ctx, S.leaves.ensure(c4 * kCell));
+  }
+  *trap = (int)(ctx->h_counts[4] & 0xffffffffu);
+  *n_leaves = c4;
+  return CC_OK;
+}
+
 // Enumerates the device cells d_cells[0..ncells) with piece counts n[4];
 // the unproven leaves land in scratch().leaves, their number in *n_leaves.
 // Counter slots: 0..3 level counts, 4 trap flag (as int).
@@ -45,6 +77,7 @@ int enumerate_dev(cc_ctx* ctx, const void* d_cells, size_t ncells, const int n[4
   *trap = 0;
   CC_HIP(ctx, hipMemsetAsync(cnt, 0, 8 * sizeof(unsigned long long), ctx->stream));
   if (ncells == 0) return CC_OK;
+  if (ctx->enum_form != CC_ENUM_BATCHED) return enumerate_plain(ctx, d_cells, ncells, n, n_leaves, trap);
   const uint64_t nc = ncells;
   CC_HIP(ctx, S.aux.ensure(ncells * kAux));
   CC_HIP(ctx, launch(f0, dim3(blocks_for(nc, 256)), dim3(256), 0, ctx->stream, d_cells, nc, ctx->d_konst, S.aux.p));

@@ -17,6 +17,8 @@
 //   --checkpoint P       save progress to P.tr / P.rsb and resume from them
 //   --max-batches N      stop each proof after N batches, exit status 3
 //                        (resume later with the same --checkpoint)
+//   --enum-form F        batched (default), plain or plain-exact: the form of
+//                        the per-cell enumerations (cc_set_enumeration_form)
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -28,7 +30,7 @@
 
 int main(int argc, const char* argv[]) {
   bool tr = true, rsb = true, custom = false;
-  int tr_la = 0, tr_gen = -1, rsb_la = 0, rsb_r1 = 0, rsb_r2 = 0, rsb_gen = -1;
+  int tr_la = 0, tr_gen = -1, rsb_la = 0, rsb_r1 = 0, rsb_r2 = 0, rsb_gen = -1, enum_form = CC_ENUM_BATCHED;
   int tr_first = 0, tr_count = 0, rsb_first = 0, rsb_count = 0, max_batches = 0;
   const char* ckpt = nullptr;
   int tr_c[4], tr_gc[4], rsb_c[4], rsb_gc[4], rsb_gg[3];
@@ -78,6 +80,15 @@ int main(int argc, const char* argv[]) {
       ckpt = v, custom = true, ++i;
     } else if (!std::strcmp(a, "--max-batches") && v) {
       max_batches = std::atoi(v), custom = true, ++i;
+    } else if (!std::strcmp(a, "--enum-form") && v) {
+      if (!std::strcmp(v, "batched")) enum_form = CC_ENUM_BATCHED;
+      else if (!std::strcmp(v, "plain")) enum_form = CC_ENUM_PLAIN;
+      else if (!std::strcmp(v, "plain-exact")) enum_form = CC_ENUM_PLAIN_EXACT;
+      else {
+        std::cerr << "--enum-form expects batched, plain or plain-exact" << std::endl;
+        return 2;
+      }
+      custom = true, ++i;
     } else {
       std::cerr << "unknown argument " << a << std::endl;
       return 2;
@@ -93,7 +104,8 @@ int main(int argc, const char* argv[]) {
     std::cerr << "no usable gfx950 device" << std::endl;
     return 1;
   }
-  int rc = cc_tr_prove_shape(ctx, have_tr_c ? tr_c : nullptr, have_tr_gc ? tr_gc : nullptr);
+  int rc = cc_set_enumeration_form(ctx, enum_form);
+  if (rc == CC_OK) rc = cc_tr_prove_shape(ctx, have_tr_c ? tr_c : nullptr, have_tr_gc ? tr_gc : nullptr);
   if (rc == CC_OK)
     rc = cc_rsb_prove_shape(ctx, have_rsb_c ? rsb_c : nullptr, have_rsb_gg ? rsb_gg : nullptr,
                             have_rsb_gc ? rsb_gc : nullptr);

@@ -86,6 +86,7 @@ lib.cc_run_bnb.argtypes = [_vp, _vp, _P(_vp), _P(_sz), _vp]
 lib.cc_free.argtypes = [_vp]
 lib.cc_free.restype = None
 lib.cc_interrupt.argtypes = [_vp]
+lib.cc_set_enumeration_form.argtypes = [_vp, _i]
 lib.cc_rsb_autotune.argtypes = [_vp, _vp, _sz, _i, _vp, _P(ctypes.c_double)]
 RANKER_PARAMS = 5451
 RANKER_CLASSES = 11  # criteria 0..9 first proving a box, 10: none proves it
@@ -173,6 +174,7 @@ class BnbStats(ctypes.Structure):
 
 
 CC_BNB_PLAIN, CC_BNB_BATCHED, CC_BNB_LEARNED, CC_BNB_PLAIN_EXACT = 0, 1, 2, 3
+CC_ENUM_BATCHED, CC_ENUM_PLAIN, CC_ENUM_PLAIN_EXACT = 0, 1, 2
 
 
 class BnbConfig(ctypes.Structure):
@@ -336,6 +338,11 @@ class Prover:
         leaves = self._grow(lambda o, c, n: lib.cc_tr_bnb(self.ctx, _ptr(cells), cells.shape[0], _ptr(cnt), levels,
                                                           _ptr(o), c, ctypes.byref(n), _ptr(per)), cap, 10)
         return leaves, per
+
+    def set_enumeration_form(self, form: int) -> None:
+        """CC_ENUM_BATCHED (0), CC_ENUM_PLAIN (1) or CC_ENUM_PLAIN_EXACT (2) for
+        both proofs' enumerations (cc_set_enumeration_form)."""
+        self._check(lib.cc_set_enumeration_form(self.ctx, int(form)))
 
     # ---- rectangle_size_bound enumeration / branch-and-bound
     def rsb_find_critical_in(self, cell, is_init: bool, cap: int = 1 << 16) -> np.ndarray:

@@ -427,6 +427,23 @@ int cc_rsb_prove_ex(cc_ctx* ctx, int la_init, int la_first, int la_count, int r1
  * RSB/rectangle_size_bound.cu:36-62).  A checkpoint records the shape it was
  * made with. */
 int cc_tr_prove_shape(cc_ctx* ctx, const int init_counts[4], const int gen_counts[4]);
+/* The form of both proofs' per-cell enumerations (cc_tr_enumerate,
+ * cc_rsb_enumerate, the find_critical_in calls and cc_tr_prove /
+ * cc_rsb_prove on top of them).  No counterpart in the reference: an option
+ * for measuring the batched form against the reference's own kernel shape.
+ *   CC_ENUM_BATCHED      level kernels with compaction between levels and the
+ *                        exact shortcuts (the default);
+ *   CC_ENUM_PLAIN        the reference's kernel shape (TR/tight_rectangle.cu:
+ *                        169-277, RSB/rectangle_size_bound.cu:161-251): one
+ *                        block per outer piece pair, the inner pieces in a 16x16
+ *                        block, no compaction, one atomic per unproven leaf,
+ *                        the reference's operation sequence;
+ *   CC_ENUM_PLAIN_EXACT  that shape with the batched form's exact shortcuts.
+ * Every form gives the same leaves (as a set) and reports. */
+#define CC_ENUM_BATCHED 0
+#define CC_ENUM_PLAIN 1
+#define CC_ENUM_PLAIN_EXACT 2
+int cc_set_enumeration_form(cc_ctx* ctx, int form);
 int cc_rsb_prove_shape(cc_ctx* ctx, const int init_counts[4], const int gen_grid[3], const int gen_counts[4]);
 /* Asks the call running on ctx to stop at its next launch or level boundary
  * (reference: was_interrupted, Prover/interrupt.cpp:30-46, polled at
