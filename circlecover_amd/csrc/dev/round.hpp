@@ -33,36 +33,77 @@ CCD uint64_t bits(double x) { return (uint64_t)__double_as_longlong(x); }
 CCD double from_bits(uint64_t b) { return __longlong_as_double((long long)b); }
 
 // ---------------------------------------------------------------- add / mul
+// Operands the compiler knows as constants are not copied into VGPRs: the
+// inline constants 0.5, 1, 2 and 4 are written into the instruction, any
+// other constant goes in as an SGPR pair (one per instruction: the VALU reads
+// at most one scalar operand).  Same instruction, same rounding; a VGPR
+// operand would cost two moves per use.
+// A translation unit sets CC_CONST_OPERANDS 0 before including this header
+// to keep every operand in VGPRs: in the RSB enumeration's inlined leaf
+// chain, which already uses every SGPR, the constant operands made its
+// stage A 8% slower (SGPR spills through VGPR lanes, a different schedule).
+#ifndef CC_CONST_OPERANDS
+#define CC_CONST_OPERANDS 1
+#endif
+#define CC_K(x, v) (CC_CONST_OPERANDS && __builtin_constant_p(x) && (x) == (v))
+#define CC_KS(x) (CC_CONST_OPERANDS && __builtin_constant_p(x))
+// op %0, M1 a, M2 b with an inline constant, an SGPR or a VGPR for b, or an
+// SGPR for a (both operand orders are listed by the callers of commutative
+// operations).
+#define CC_BIN(OP, M1, M2, a, b, r)                                                            \
+  do {                                                                                         \
+    if (CC_K(b, 0.5)) asm(OP " %0, " M1 "%1, " M2 "0.5" : "=v"(r) : "v"(a));                   \
+    else if (CC_K(b, 1.0)) asm(OP " %0, " M1 "%1, " M2 "1.0" : "=v"(r) : "v"(a));              \
+    else if (CC_K(b, 2.0)) asm(OP " %0, " M1 "%1, " M2 "2.0" : "=v"(r) : "v"(a));              \
+    else if (CC_K(b, 4.0)) asm(OP " %0, " M1 "%1, " M2 "4.0" : "=v"(r) : "v"(a));              \
+    else if (CC_KS(b)) asm(OP " %0, " M1 "%1, " M2 "%2" : "=v"(r) : "v"(a), "s"(b));            \
+    else if (CC_KS(a)) asm(OP " %0, " M1 "%1, " M2 "%2" : "=v"(r) : "s"(a), "v"(b));            \
+    else asm(OP " %0, " M1 "%1, " M2 "%2" : "=v"(r) : "v"(a), "v"(b));                          \
+  } while (0)
+
 CCD double add_rd(double a, double b) {
   double r;
-  asm("v_add_f64 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+  if (CC_KS(a) && !CC_KS(b)) CC_BIN("v_add_f64", "", "", b, a, r);
+  else CC_BIN("v_add_f64", "", "", a, b, r);
   return r;
 }
 CCD double add_ru(double a, double b) {
   double r;
-  asm("v_add_f64 %0, -%1, -%2" : "=v"(r) : "v"(a), "v"(b));
+  if (CC_KS(a) && !CC_KS(b)) CC_BIN("v_add_f64", "-", "-", b, a, r);
+  else CC_BIN("v_add_f64", "-", "-", a, b, r);
   return -r;
 }
 CCD double mul_rd(double a, double b) {
   double r;
-  asm("v_mul_f64 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+  if (CC_KS(a) && !CC_KS(b)) CC_BIN("v_mul_f64", "", "", b, a, r);
+  else CC_BIN("v_mul_f64", "", "", a, b, r);
   return r;
 }
 CCD double mul_ru(double a, double b) {
   double r;
-  asm("v_mul_f64 %0, -%1, %2" : "=v"(r) : "v"(a), "v"(b));
+  // -RD((-a) * b) = -RD(a * (-b)): the sign goes on whichever operand stays a VGPR
+  if (CC_KS(a) && !CC_KS(b)) CC_BIN("v_mul_f64", "-", "", b, a, r);
+  else CC_BIN("v_mul_f64", "-", "", a, b, r);
   return -r;
 }
-// a*b + c rounded once, downward.
+// a*b + c rounded once, downward; c (the accumulator) may be a constant.
+#define CC_FMA(M1, a, b, c, r)                                                                     \
+  do {                                                                                             \
+    if (CC_K(c, 0.5)) asm("v_fma_f64 %0, " M1 "%1, %2, 0.5" : "=v"(r) : "v"(a), "v"(b));          \
+    else if (CC_K(c, 1.0)) asm("v_fma_f64 %0, " M1 "%1, %2, 1.0" : "=v"(r) : "v"(a), "v"(b));     \
+    else if (CC_KS(c)) asm("v_fma_f64 %0, " M1 "%1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "s"(c));   \
+    else if (CC_KS(b)) asm("v_fma_f64 %0, " M1 "%1, %2, %3" : "=v"(r) : "v"(a), "s"(b), "v"(c));   \
+    else asm("v_fma_f64 %0, " M1 "%1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));                 \
+  } while (0)
 CCD double fma_rd(double a, double b, double c) {
   double r;
-  asm("v_fma_f64 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+  CC_FMA("", a, b, c, r);
   return r;
 }
 // c - a*b rounded once, downward.
 CCD double fnma_rd(double a, double b, double c) {
   double r;
-  asm("v_fma_f64 %0, -%1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+  CC_FMA("-", a, b, c, r);
   return r;
 }
 

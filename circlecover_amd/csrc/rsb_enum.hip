@@ -11,6 +11,8 @@
 #define CC_EXACT_SHORTCUTS 1
 #define CC_INLINE_RECURSE 1
 #define CC_CHAIN_FN static __device__ __forceinline__
+// constants as VGPR operands (see round.hpp)
+#define CC_CONST_OPERANDS 0
 #include "dev/queue.hpp"
 #include "dev/rsb_rec.hpp"
 
