@@ -278,9 +278,72 @@ int max_distances(cc_ctx* ctx, const std::vector<cc_tr_cell>& crit, MaxDist* out
     CC_HIP(ctx, hipMemcpyAsync(d_cells.p, crit.data(), crit.size() * sizeof(cc_tr_cell), hipMemcpyHostToDevice,
                                ctx->stream));
   const uint64_t n = crit.size();
-  CC_HIP(ctx, launch(f, dim3(1), dim3(64), 0, ctx->stream, d_cells.p, n, ctx->d_konst, d_out.p));
+  // one workgroup of 256 lanes: the kernel's reduction assumes exactly that
+  CC_HIP(ctx, launch(f, dim3(1), dim3(256), 0, ctx->stream, d_cells.p, n, ctx->d_konst, d_out.p));
   CC_HIP(ctx, hipMemcpyAsync(out, d_out.p, sizeof(MaxDist), hipMemcpyDeviceToHost, ctx->stream));
   CC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  // the reference's interval sqrt traps on a possibly negative argument
+  // (ACU/interval.cuh:339-350)
+  if (out->trap) return fail(ctx, CC_E_TRAP, "a critical cell reached a sqrt of a possibly negative interval");
+  return CC_OK;
+}
+
+// The Max_distances lines of the final report (tight_rectangle.cu:498-536)
+// for the critical cells `current`.
+int write_tr_criticals(cc_ctx* ctx, const std::vector<cc_tr_cell>& current, std::ostream& o) {
+  if (current.empty()) return CC_OK;
+  MaxDist r;
+  if (const int rc = max_distances(ctx, current, &r)) return rc;
+  auto crit_line = [&](const cc_tr_cell& c) {
+    o << "\t\t";
+    put_cell(o, c);
+    o << std::endl;
+  };
+  if (r.have_2d) {  // tight_rectangle.cu:508-519
+    o << "Two disks: Range for lambda that contains 2-disk critical intervals: ";
+    put_iv(o, r.la_2d);
+    o << "; maximal distances from theoretical values: " << std::endl;
+    o << "\tr_1^2: ";
+    put_iv(o, r.d_r1_2d);
+    o << ", r_2^2: ";
+    put_iv(o, r.d_r2_2d);
+    o << ", R_3 <= " << r.ub_R3_2d << ", h_1 >= " << r.lb_h1_2d << ", lambda / S_1 <= " << r.ub_la_by_S1
+      << std::endl;
+    o << "\tDominating criticals for r_1: " << std::endl;
+    crit_line(r.d_r1_2d_crit[0]);
+    crit_line(r.d_r1_2d_crit[1]);
+    o << "\tDominating criticals for r_2: " << std::endl;
+    crit_line(r.d_r2_2d_crit[0]);
+    crit_line(r.d_r2_2d_crit[1]);
+  }
+  if (r.have_3d) {  // :521-533
+    o << "Three disks: Range for lambda that contains 3-disk critical intervals: ";
+    put_iv(o, r.la_3d);
+    o << "; maximal distances from theoretical values: " << std::endl;
+    o << "\tr_1^2: ";
+    put_iv(o, r.d_r1_3d);
+    o << ", r_2^2: ";
+    put_iv(o, r.d_r2_3d);
+    o << ", r_3^2: ";
+    put_iv(o, r.d_r3_3d);
+    o << ", R_4 <= " << r.ub_R4_3d << std::endl;
+    o << "\tS_1: ";
+    put_iv(o, r.S1);
+    o << ", h_2: ";
+    put_iv(o, r.h2);
+    o << ", S_3: ";
+    put_iv(o, r.S3);
+    o << ", pocket width <= " << r.pocket_3d << std::endl;
+    o << "\tDominating criticals for r_1: " << std::endl;
+    crit_line(r.d_r1_3d_crit[0]);
+    crit_line(r.d_r1_3d_crit[1]);
+    o << "\tDominating criticals for r_2: " << std::endl;
+    crit_line(r.d_r2_3d_crit[0]);
+    crit_line(r.d_r2_3d_crit[1]);
+    o << "\tDominating criticals for r_3: " << std::endl;
+    crit_line(r.d_r3_3d_crit[0]);
+    crit_line(r.d_r3_3d_crit[1]);
+  }
   return CC_OK;
 }
 
@@ -450,65 +513,26 @@ int cc_tr_prove_ex(cc_ctx* ctx, int la_init, int la_first, int la_count, int gen
       if (current.empty()) break;
     }
   }
-  if (!current.empty()) {
-    MaxDist r;
-    if ((rc = max_distances(ctx, current, &r))) return rc;
-    auto crit_line = [&](const cc_tr_cell& c) {
-      o << "\t\t";
-      put_cell(o, c);
-      o << std::endl;
-    };
-    if (r.have_2d) {  // tight_rectangle.cu:508-519
-      o << "Two disks: Range for lambda that contains 2-disk critical intervals: ";
-      put_iv(o, r.la_2d);
-      o << "; maximal distances from theoretical values: " << std::endl;
-      o << "\tr_1^2: ";
-      put_iv(o, r.d_r1_2d);
-      o << ", r_2^2: ";
-      put_iv(o, r.d_r2_2d);
-      o << ", R_3 <= " << r.ub_R3_2d << ", h_1 >= " << r.lb_h1_2d << ", lambda / S_1 <= " << r.ub_la_by_S1
-        << std::endl;
-      o << "\tDominating criticals for r_1: " << std::endl;
-      crit_line(r.d_r1_2d_crit[0]);
-      crit_line(r.d_r1_2d_crit[1]);
-      o << "\tDominating criticals for r_2: " << std::endl;
-      crit_line(r.d_r2_2d_crit[0]);
-      crit_line(r.d_r2_2d_crit[1]);
-    }
-    if (r.have_3d) {  // :521-533
-      o << "Three disks: Range for lambda that contains 3-disk critical intervals: ";
-      put_iv(o, r.la_3d);
-      o << "; maximal distances from theoretical values: " << std::endl;
-      o << "\tr_1^2: ";
-      put_iv(o, r.d_r1_3d);
-      o << ", r_2^2: ";
-      put_iv(o, r.d_r2_3d);
-      o << ", r_3^2: ";
-      put_iv(o, r.d_r3_3d);
-      o << ", R_4 <= " << r.ub_R4_3d << std::endl;
-      o << "\tS_1: ";
-      put_iv(o, r.S1);
-      o << ", h_2: ";
-      put_iv(o, r.h2);
-      o << ", S_3: ";
-      put_iv(o, r.S3);
-      o << ", pocket width <= " << r.pocket_3d << std::endl;
-      o << "\tDominating criticals for r_1: " << std::endl;
-      crit_line(r.d_r1_3d_crit[0]);
-      crit_line(r.d_r1_3d_crit[1]);
-      o << "\tDominating criticals for r_2: " << std::endl;
-      crit_line(r.d_r2_3d_crit[0]);
-      crit_line(r.d_r2_3d_crit[1]);
-      o << "\tDominating criticals for r_3: " << std::endl;
-      crit_line(r.d_r3_3d_crit[0]);
-      crit_line(r.d_r3_3d_crit[1]);
-    }
-  }
+  if ((rc = write_tr_criticals(ctx, current, o))) return rc;
   end_case(o);
   const std::string s = o.str();
   std::fwrite(s.data(), 1, s.size(), out ? out : stdout);
   std::fflush(out ? out : stdout);
   if (checkpoint && completed) std::remove(checkpoint);  // the proof is done
+  return CC_OK;
+}
+
+int cc_tr_report_criticals(cc_ctx* ctx, const cc_tr_cell* cells, size_t n, char* buf, size_t cap, size_t* len) {
+  if (!ctx || !len || (n && !cells)) return CC_E_ARG;
+  std::ostringstream o;
+  begin_case(o, "Critical intervals for rectangles (tight)");
+  const std::vector<cc_tr_cell> crit(cells, cells + n);
+  if (const int rc = write_tr_criticals(ctx, crit, o)) return rc;
+  end_case(o);
+  const std::string t = o.str();
+  *len = t.size();
+  if (!buf || cap <= t.size()) return CC_E_OVERFLOW;
+  std::memcpy(buf, t.c_str(), t.size() + 1);
   return CC_OK;
 }
 

@@ -313,10 +313,20 @@ extern "C" __global__ void __launch_bounds__(256) ccrd_tr_bisect(const cell5* __
 
 // Distances of the final critical cells from the theoretical worst cases
 // (kernel_compute_distance_from_theoretical_worst_cases,
-// tight_rectangle.cu:343-456): one lane walks the cells in order, as the
-// reference's <<<1,1>>> launch does, so the "dominating critical" records
-// are the same.  Output layout: struct tr_maxdist below (host mirror in
-// host/prover_api.cpp).
+// tight_rectangle.cu:343-456), as a reduction over one workgroup.  The
+// reference walks the cells in order on one thread.  Each of its running
+// extremes starts from a fixed value and takes a cell's value only if it is
+// strictly beyond (so NaN never enters), and its "dominating" cells are the
+// ones that set the final extremes: every extreme is therefore the pair
+// (value, first cell attaining it), position 0 standing for the start value,
+// and pairs combine by value and then by the lower position, which gives the
+// same value bits (also for zeros of either sign) and the same cell in any
+// order of combination.  S1, h2 and S3 restart from a cell's own values
+// whenever its pocket width exceeds a running maximum that is still negative;
+// the last restart is at the first cell whose width is >= 0, or, without
+// one, at the first cell with the largest width above -1.  A first pass finds
+// that cell, a second forms the three hulls from it on.  Output layout:
+// struct tr_maxdist below (host mirror in host/prover_api.cpp).
 struct tr_maxdist {
   int have_2d, have_3d;
   iv la_2d, la_3d;
@@ -337,84 +347,205 @@ struct tr_maxdist {
   int trap;
 };
 
-extern "C" __global__ void ccrd_tr_max_distances(const cell5* __restrict__ crit, uint64_t n,
-                                                 const Konst* __restrict__ kp, tr_maxdist* __restrict__ o) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+// Lanes of the workgroup (one launch of one workgroup).
+constexpr int kMdLanes = 256;
+constexpr uint64_t kMdNone = ~0ull;  // no value yet
+
+// (value, position): position 0 is the start value, cell i is i + 1.
+struct md_best {
+  double v;
+  uint64_t i;
+};
+// A lane meets its cells in increasing order: a strictly better value wins.
+CCD void md_lane_min(md_best& a, double v, uint64_t i) {
+  if (a.i == kMdNone || v < a.v) a = md_best{v, i};
+}
+CCD void md_lane_max(md_best& a, double v, uint64_t i) {
+  if (a.i == kMdNone || v > a.v) a = md_best{v, i};
+}
+// Two partial results: the better value, then the earlier position.
+template <bool Min>
+CCD void md_merge(md_best& a, md_best b) {
+  if (b.i == kMdNone) return;
+  const bool better = Min ? b.v < a.v : b.v > a.v;
+  if (a.i == kMdNone || better || (b.v == a.v && b.i < a.i)) a = b;
+}
+template <bool Min>
+CCD md_best md_reduce(md_best a, md_best* lds) {
+  for (int off = 32; off > 0; off >>= 1) {
+    md_best b;
+    b.v = __shfl_xor(a.v, off);
+    b.i = (uint64_t)__shfl_xor((unsigned long long)a.i, off);
+    md_merge<Min>(a, b);
+  }
+  __syncthreads();  // lds is reused by the next reduction
+  if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = a;
+  __syncthreads();
+  a = lds[0];
+  for (int w = 1; w < kMdLanes / 64; ++w) md_merge<Min>(a, lds[w]);
+  return a;
+}
+
+// The reference's per-cell terms (tight_rectangle.cu:372-387 and :389-414).
+struct md_two {
+  iv d1, d2;
+  double R3_hi, la_by_S1, h1;
+};
+CCD md_two md_two_disk(const Konst& K, const cell5& v) {
+  md_two t;
+  const iv r1t = 0.25 * (square(v.la) + 1.0);
+  t.d1 = v.r[0] - r1t;
+  t.d2 = v.r[1] - pt(0.25);
+  t.R3_hi = (weight_for(K, v.la) - v.r[0] - v.r[1]).hi;
+  const double S1 = mul_rd(2.0, sqrt_rd(add_rd(v.r[0].lo, -0.25)));
+  t.la_by_S1 = div_ru(v.la.hi, S1);
+  const double half = add_ru(v.la.hi, -mul_rd(0.5, S1));
+  t.h1 = mul_rd(2.0, sqrt_rd(add_rd(v.r[0].lo, -mul_ru(half, half))));
+  return t;
+}
+struct md_three {
+  iv d1, d2, d3;
+  double R4_hi;
+  iv S1, h2, S3;
+  double pw;
+};
+CCD md_three md_three_disk(const Konst& K, const cell5& v, Trap& tp) {
+  md_three t;
+  const iv la2 = square(v.la);
+  const iv la4 = square(la2);
+  const iv th = (16.0 * la4 + 40.0 * la2 + 9.0) / (256.0 * la2);
+  t.d1 = v.r[0] - th;
+  t.d2 = v.r[1] - th;
+  t.d3 = v.r[2] - th;
+  t.R4_hi = (weight_for(K, v.la) - v.r[0] - v.r[1] - v.r[2]).hi;
+  t.S1 = sqrt(4.0 * v.r[0] - 1.0, tp);
+  t.h2 = sqrt(4.0 * v.r[1] - square(v.la - t.S1), tp);
+  t.S3 = sqrt(4.0 * v.r[2] - square(1.0 - t.h2), tp);
+  t.pw = (v.la - t.S1 - t.S3).hi;
+  return t;
+}
+CCD bool md_is_two_disk(const cell5& v) { return v.r[2].hi < 0.1; }  // r1 the circumcircle, r2^2 = 1/4
+
+extern "C" __global__ void __launch_bounds__(kMdLanes)
+    ccrd_tr_max_distances(const cell5* __restrict__ crit, uint64_t n, const Konst* __restrict__ kp,
+                          tr_maxdist* __restrict__ o) {
+  __shared__ md_best lds[kMdLanes / 64];
   const Konst K = *kp;
+  const uint64_t lane = threadIdx.x;
   Trap tp{0};
-  tr_maxdist r;
-  r.have_2d = r.have_3d = 0;
-  r.la_3d = pt(1.0);
-  r.la_2d = pt(2.0);
-  r.d_r1_2d = r.d_r2_2d = pt(0.0);
-  r.ub_R3_2d = r.ub_R4_3d = 0.0;
-  r.d_r1_3d = r.d_r2_3d = r.d_r3_3d = pt(0.0);
-  r.pocket_3d = -1.0;
-  r.lb_h1_2d = 1.0;
-  r.ub_la_by_S1 = 1.0;
-  const cell5 zero{pt(0.0), {pt(0.0), pt(0.0), pt(0.0), pt(0.0)}};
-  for (int k = 0; k < 2; ++k)
-    r.d_r1_2d_crit[k] = r.d_r2_2d_crit[k] = r.d_r1_3d_crit[k] = r.d_r2_3d_crit[k] = r.d_r3_3d_crit[k] = zero;
-  r.S1 = r.h2 = r.S3 = pt(0.0);
-  for (uint64_t i = 0; i < n; ++i) {
+  // start values of the reference's running extremes, at position 0
+  md_best have2{0.0, 0}, la2_lo{2.0, 0}, la2_hi{2.0, 0}, d1_2_lo{0.0, 0}, d1_2_hi{0.0, 0}, d2_2_lo{0.0, 0},
+      d2_2_hi{0.0, 0}, R3{0.0, 0}, laS1{1.0, 0}, h1{1.0, 0};
+  md_best have3{0.0, 0}, la3_lo{1.0, 0}, la3_hi{1.0, 0}, d1_3_lo{0.0, 0}, d1_3_hi{0.0, 0}, d2_3_lo{0.0, 0},
+      d2_3_hi{0.0, 0}, d3_3_lo{0.0, 0}, d3_3_hi{0.0, 0}, R4{0.0, 0}, pocket{-1.0, 0};
+  md_best first_nonneg{0.0, kMdNone};  // first cell with pocket width >= 0 (value unused)
+  for (uint64_t i = lane; i < n; i += kMdLanes) {
     const cell5 v = crit[i];
-    if (v.r[2].hi < 0.1) {  // two-disk critical: r1 the circumcircle, r2^2 = 1/4
-      r.have_2d = 1;
-      join_into(r.la_2d, v.la);
-      const iv r1t = 0.25 * (square(v.la) + 1.0);
-      const iv d1 = v.r[0] - r1t;
-      if (d1.lo < r.d_r1_2d.lo) r.d_r1_2d_crit[0] = v;
-      if (d1.hi > r.d_r1_2d.hi) r.d_r1_2d_crit[1] = v;
-      join_into(r.d_r1_2d, d1);
-      const iv d2 = v.r[1] - pt(0.25);
-      if (d2.lo < r.d_r2_2d.lo) r.d_r2_2d_crit[0] = v;
-      if (d2.hi > r.d_r2_2d.hi) r.d_r2_2d_crit[1] = v;
-      join_into(r.d_r2_2d, d2);
-      const iv R3 = weight_for(K, v.la) - v.r[0] - v.r[1];
-      if (R3.hi > r.ub_R3_2d) r.ub_R3_2d = R3.hi;
-      const double S1 = mul_rd(2.0, sqrt_rd(add_rd(v.r[0].lo, -0.25)));
-      const double q = div_ru(v.la.hi, S1);
-      if (q > r.ub_la_by_S1) r.ub_la_by_S1 = q;
-      const double half = add_ru(v.la.hi, -mul_rd(0.5, S1));
-      const double h1 = mul_rd(2.0, sqrt_rd(add_rd(v.r[0].lo, -mul_ru(half, half))));
-      if (h1 < r.lb_h1_2d) r.lb_h1_2d = h1;
-    } else {  // three-disk critical
-      r.have_3d = 1;
-      join_into(r.la_3d, v.la);
-      const iv la2 = square(v.la);
-      const iv la4 = square(la2);
-      const iv th = (16.0 * la4 + 40.0 * la2 + 9.0) / (256.0 * la2);
-      const iv d1 = v.r[0] - th;
-      if (d1.lo < r.d_r1_3d.lo) r.d_r1_3d_crit[0] = v;
-      if (d1.hi > r.d_r1_3d.hi) r.d_r1_3d_crit[1] = v;
-      const iv d2 = v.r[1] - th;
-      if (d2.lo < r.d_r2_3d.lo) r.d_r2_3d_crit[0] = v;
-      if (d2.hi > r.d_r2_3d.hi) r.d_r2_3d_crit[1] = v;
-      const iv d3 = v.r[2] - th;
-      if (d3.lo < r.d_r3_3d.lo) r.d_r3_3d_crit[0] = v;
-      if (d3.hi > r.d_r3_3d.hi) r.d_r3_3d_crit[1] = v;
-      join_into(r.d_r1_3d, d1);
-      join_into(r.d_r2_3d, d2);
-      join_into(r.d_r3_3d, d3);
-      const iv R4 = weight_for(K, v.la) - v.r[0] - v.r[1] - v.r[2];
-      if (R4.hi > r.ub_R4_3d) r.ub_R4_3d = R4.hi;
-      const iv S1 = sqrt(4.0 * v.r[0] - 1.0, tp);
-      const iv h2 = sqrt(4.0 * v.r[1] - square(v.la - S1), tp);
-      const iv S3 = sqrt(4.0 * v.r[2] - square(1.0 - h2), tp);
-      const double pw = (v.la - S1 - S3).hi;
-      if (pw > r.pocket_3d) {
-        if (r.pocket_3d < 0.0) {
-          r.S1 = S1;
-          r.h2 = h2;
-          r.S3 = S3;
-        }
-        r.pocket_3d = pw;
-      }
-      join_into(r.S1, S1);
-      join_into(r.h2, h2);
-      join_into(r.S3, S3);
+    const uint64_t p = i + 1;
+    if (md_is_two_disk(v)) {
+      const md_two t = md_two_disk(K, v);
+      have2.v = 1.0;
+      md_lane_min(la2_lo, v.la.lo, p);
+      md_lane_max(la2_hi, v.la.hi, p);
+      md_lane_min(d1_2_lo, t.d1.lo, p);
+      md_lane_max(d1_2_hi, t.d1.hi, p);
+      md_lane_min(d2_2_lo, t.d2.lo, p);
+      md_lane_max(d2_2_hi, t.d2.hi, p);
+      md_lane_max(R3, t.R3_hi, p);
+      md_lane_max(laS1, t.la_by_S1, p);
+      md_lane_min(h1, t.h1, p);
+    } else {
+      const md_three t = md_three_disk(K, v, tp);
+      have3.v = 1.0;
+      md_lane_min(la3_lo, v.la.lo, p);
+      md_lane_max(la3_hi, v.la.hi, p);
+      md_lane_min(d1_3_lo, t.d1.lo, p);
+      md_lane_max(d1_3_hi, t.d1.hi, p);
+      md_lane_min(d2_3_lo, t.d2.lo, p);
+      md_lane_max(d2_3_hi, t.d2.hi, p);
+      md_lane_min(d3_3_lo, t.d3.lo, p);
+      md_lane_max(d3_3_hi, t.d3.hi, p);
+      md_lane_max(R4, t.R4_hi, p);
+      md_lane_max(pocket, t.pw, p);
+      if (t.pw >= 0.0 && first_nonneg.i == kMdNone) first_nonneg.i = p;
     }
   }
-  r.trap = tp.flag;
+  md_best trap{(double)tp.flag, 0};
+  trap = md_reduce<false>(trap, lds);
+  have2 = md_reduce<false>(have2, lds);
+  have3 = md_reduce<false>(have3, lds);
+  la2_lo = md_reduce<true>(la2_lo, lds);
+  la2_hi = md_reduce<false>(la2_hi, lds);
+  d1_2_lo = md_reduce<true>(d1_2_lo, lds);
+  d1_2_hi = md_reduce<false>(d1_2_hi, lds);
+  d2_2_lo = md_reduce<true>(d2_2_lo, lds);
+  d2_2_hi = md_reduce<false>(d2_2_hi, lds);
+  R3 = md_reduce<false>(R3, lds);
+  laS1 = md_reduce<false>(laS1, lds);
+  h1 = md_reduce<true>(h1, lds);
+  la3_lo = md_reduce<true>(la3_lo, lds);
+  la3_hi = md_reduce<false>(la3_hi, lds);
+  d1_3_lo = md_reduce<true>(d1_3_lo, lds);
+  d1_3_hi = md_reduce<false>(d1_3_hi, lds);
+  d2_3_lo = md_reduce<true>(d2_3_lo, lds);
+  d2_3_hi = md_reduce<false>(d2_3_hi, lds);
+  d3_3_lo = md_reduce<true>(d3_3_lo, lds);
+  d3_3_hi = md_reduce<false>(d3_3_hi, lds);
+  R4 = md_reduce<false>(R4, lds);
+  pocket = md_reduce<false>(pocket, lds);
+  first_nonneg.v = first_nonneg.i == kMdNone ? 0.0 : -(double)first_nonneg.i;
+  first_nonneg = md_reduce<false>(first_nonneg, lds);  // the lowest position
+  // The last restart of S1, h2, S3 (position 0: none, the hulls start at 0).
+  const uint64_t k = first_nonneg.i != kMdNone ? first_nonneg.i : (pocket.v > -1.0 ? pocket.i : 0);
+  md_best S_lo[3], S_hi[3];
+  for (int q = 0; q < 3; ++q) S_lo[q] = S_hi[q] = k ? md_best{0.0, kMdNone} : md_best{0.0, 0};
+  for (uint64_t i = lane; i < n; i += kMdLanes) {
+    if (i + 1 < k) continue;
+    const cell5 v = crit[i];
+    if (md_is_two_disk(v)) continue;
+    Trap unused{0};
+    const md_three t = md_three_disk(K, v, unused);
+    const iv S[3] = {t.S1, t.h2, t.S3};
+    for (int q = 0; q < 3; ++q) {
+      md_lane_min(S_lo[q], S[q].lo, i + 1);
+      md_lane_max(S_hi[q], S[q].hi, i + 1);
+    }
+  }
+  for (int q = 0; q < 3; ++q) {
+    S_lo[q] = md_reduce<true>(S_lo[q], lds);
+    S_hi[q] = md_reduce<false>(S_hi[q], lds);
+  }
+  if (lane != 0) return;
+  const cell5 zero{pt(0.0), {pt(0.0), pt(0.0), pt(0.0), pt(0.0)}};
+  auto cell_at = [&](uint64_t p) { return p ? crit[p - 1] : zero; };
+  tr_maxdist r;
+  r.have_2d = have2.v != 0.0;
+  r.have_3d = have3.v != 0.0;
+  r.la_2d = iv{la2_lo.v, la2_hi.v};
+  r.la_3d = iv{la3_lo.v, la3_hi.v};
+  r.d_r1_2d = iv{d1_2_lo.v, d1_2_hi.v};
+  r.d_r1_2d_crit[0] = cell_at(d1_2_lo.i);
+  r.d_r1_2d_crit[1] = cell_at(d1_2_hi.i);
+  r.d_r2_2d = iv{d2_2_lo.v, d2_2_hi.v};
+  r.d_r2_2d_crit[0] = cell_at(d2_2_lo.i);
+  r.d_r2_2d_crit[1] = cell_at(d2_2_hi.i);
+  r.lb_h1_2d = h1.v;
+  r.ub_la_by_S1 = laS1.v;
+  r.ub_R3_2d = R3.v;
+  r.d_r1_3d = iv{d1_3_lo.v, d1_3_hi.v};
+  r.d_r1_3d_crit[0] = cell_at(d1_3_lo.i);
+  r.d_r1_3d_crit[1] = cell_at(d1_3_hi.i);
+  r.d_r2_3d = iv{d2_3_lo.v, d2_3_hi.v};
+  r.d_r2_3d_crit[0] = cell_at(d2_3_lo.i);
+  r.d_r2_3d_crit[1] = cell_at(d2_3_hi.i);
+  r.d_r3_3d = iv{d3_3_lo.v, d3_3_hi.v};
+  r.d_r3_3d_crit[0] = cell_at(d3_3_lo.i);
+  r.d_r3_3d_crit[1] = cell_at(d3_3_hi.i);
+  r.ub_R4_3d = R4.v;
+  r.S1 = iv{S_lo[0].v, S_hi[0].v};
+  r.h2 = iv{S_lo[1].v, S_hi[1].v};
+  r.S3 = iv{S_lo[2].v, S_hi[2].v};
+  r.pocket_3d = pocket.v;
+  r.trap = trap.v != 0.0;
   *o = r;
 }

@@ -125,6 +125,7 @@ lib.cc_c2_f32_report.argtypes = [_vp, _vp, _sz, _sz, _vp, _vp]
 lib.cc_tr_find_critical_in.argtypes = [_vp, _vp, _i, _vp, _sz, _P(_sz)]
 lib.cc_tr_enumerate.argtypes = [_vp, _vp, _sz, _vp, _vp, _sz, _P(_sz)]
 lib.cc_tr_bnb.argtypes = [_vp, _vp, _sz, _vp, _i, _vp, _sz, _P(_sz), _vp]
+lib.cc_tr_report_criticals.argtypes = [_vp, _vp, _sz, ctypes.c_char_p, _sz, _P(_sz)]
 
 
 def _ptr(a: np.ndarray) -> int:
@@ -329,6 +330,22 @@ class Prover:
         cnt = np.ascontiguousarray(counts, dtype=np.int32)
         return self._grow(lambda o, c, n: lib.cc_tr_enumerate(self.ctx, _ptr(cells), cells.shape[0], _ptr(cnt),
                                                               _ptr(o), c, ctypes.byref(n)), cap, 10)
+
+    def tr_report_criticals(self, cells: np.ndarray) -> str:
+        """The tight_rectangle report for a final critical set
+        (tight_rectangle.cu:498-536), its Max_distances lines computed on the
+        device."""
+        cells = np.ascontiguousarray(cells, dtype=np.float64).reshape(-1, 10)
+        n = _sz(0)
+        cap = 1 << 14
+        while True:
+            buf = ctypes.create_string_buffer(cap)
+            rc = lib.cc_tr_report_criticals(self.ctx, _ptr(cells), cells.shape[0], buf, cap, ctypes.byref(n))
+            if rc == CC_E_OVERFLOW and n.value >= cap:
+                cap = n.value + 1
+                continue
+            self._check(rc)
+            return buf.value.decode("utf-8")
 
     def tr_bnb(self, cells: np.ndarray, counts, levels: int, cap: int = 1 << 16):
         """Returns (unproven leaves, unproven count after each level)."""

@@ -393,6 +393,14 @@ int cc_parse_interval(const char* text, size_t n, cc_iv* out, size_t* consumed);
  * stderr.  SIGINT during the generations ends the run with the last
  * complete generation, as the reference does. */
 int cc_tr_prove(cc_ctx* ctx, int la_init, int la_first, int la_count, int generations, FILE* out);
+/* The report that cc_tr_prove writes for a final critical set `cells`
+ * (tight_rectangle.cu:498-536: the case banner, the Max_distances lines of
+ * kernel_compute_distance_from_theoretical_worst_cases, :343-456, computed
+ * on the device, and the closing line), into buf as a NUL-terminated
+ * string; *len receives its length.  CC_E_OVERFLOW if buf is NULL or cap <=
+ * *len; CC_E_TRAP if a three-disk cell takes the square root of a possibly
+ * negative interval (the reference's kernel traps there). */
+int cc_tr_report_criticals(cc_ctx* ctx, const cc_tr_cell* cells, size_t n, char* buf, size_t cap, size_t* len);
 /* The rectangle_size_bound proof of rectangle_size_bound.cu:312-382:
  * initial grid la_init x r1_init x r2_init (<= 0: 256, 128, 128) over the
  * lambda pieces [la_first, la_first + la_count) (the reference's

@@ -11,7 +11,7 @@ import pytest
 import oracle
 from circlecover_amd import TR_LAMBDA_MAX, tr_initial_cells
 from circlecover_amd.report import compare_reports, read_report
-from report_format import begin_case, end_case, iv, merge_critical, rsb_report, tr_report
+from report_format import begin_case, end_case, iv, max_distances, merge_critical, rsb_report, tr_report
 
 EXE = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "circlecover_amd",
                    "circlecover_prover")
@@ -38,6 +38,22 @@ def run_prover(args):
     r = subprocess.run([EXE, *args], capture_output=True, text=True, timeout=600)
     assert r.returncode == 0, r.stderr[-2000:]
     return r.stdout
+
+
+def check_tr_run(args, criticals):
+    """A tight_rectangle run whose final critical set is `criticals` writes
+    the oracle's report; where a three-disk cell of the set takes the square
+    root of a possibly negative interval in the Max_distances step, the
+    reference's kernel traps there (ACU/interval.cuh:339-350), and the run
+    fails with the trap error and no report instead.  Returns whether it
+    trapped."""
+    r = subprocess.run([EXE, *args], capture_output=True, text=True, timeout=600)
+    if len(criticals) and max_distances(criticals).trap:
+        assert r.returncode == 1 and r.stdout == "" and "error 3" in r.stderr, r.stderr[-2000:]
+        return True
+    assert r.returncode == 0, r.stderr[-2000:]
+    assert r.stdout == tr_report(criticals)
+    return False
 
 
 @pytest.mark.gpu
@@ -237,13 +253,17 @@ def test_gpu_tr_prove_generations_match_oracle():
     """Generations 0..3 of the tight_rectangle proof on 8 initial lambda
     launches, with r1..r4 piece counts 32^4 for the initial launches and
     16,16,16,8 for the generation launches (the critical sets have 12, 8 and
-    32 records, then none): each report equals the oracle's, byte for byte."""
+    32 records, then none): each report equals the oracle's, byte for byte.
+    The initial launches' 12 wide cells include a three-disk cell with r1
+    possibly below 1/4, so that run ends at the reference's trap."""
     sets = _tr_oracle_generations(0, 8, (32, 32, 32, 32), (16, 16, 16, 8), 3)
     assert [len(s) for s in sets] == [12, 8, 32, 0]
     base = ["--only", "tr", "--tr-la-range", "0,8", "--tr-counts", "32,32,32,32", "--tr-gen-counts", "16,16,16,8"]
+    trapped = []
     for g, s in enumerate(sets):
         print(f"tr generations {g}", flush=True)
-        assert run_prover([*base, "--tr-generations", str(g)]) == tr_report(s)
+        trapped.append(check_tr_run([*base, "--tr-generations", str(g)], s))
+    assert trapped == [True, False, False, False]
 
 
 @pytest.mark.gpu
@@ -263,8 +283,7 @@ def test_gpu_tr_prove_generations_at_reference_counts():
     for g, recs in enumerate(gold["generations"]):
         s = np.array([[float.fromhex(x) for x in r] for r in recs]).reshape(-1, 10)
         print(f"tr generations {g} at reference counts", flush=True)
-        got = run_prover(["--only", "tr", "--tr-la-range", f"{first},{count}", "--tr-generations", str(g)])
-        assert got == tr_report(s)
+        assert not check_tr_run(["--only", "tr", "--tr-la-range", f"{first},{count}", "--tr-generations", str(g)], s)
 
 
 @pytest.mark.gpu
