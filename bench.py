@@ -353,10 +353,14 @@ def extended(p, soa: np.ndarray, mask: np.ndarray) -> dict:
 C3_FORMS = ("plain", "plain_exact", "batched", "learned_batched")
 
 
+C3_REPEATS = 3
+
+
 def c3_forms(p, forms=C3_FORMS) -> dict:
     """The rectangle_size_bound branch-and-bound on the 256x64x64 grid over
-    lambda in [1, 8], depth 24, in its forms, each timed once after a
-    warm-up run (wall time of the call, cells uploaded from the host):
+    lambda in [1, 8], depth 24, in its forms, each timed three times after a
+    warm-up run, the median reported (wall time of the call, cells uploaded
+    from the host; the runs of one form differ by a few percent):
     level-synchronous with the reference's operation sequence (plain), the
     same with the batched form's exact shortcuts (plain_exact), batched, and
     learned+batched.  The ranker of the learned form is trained on the GPU
@@ -388,9 +392,14 @@ def c3_forms(p, forms=C3_FORMS) -> dict:
         log(f"C3: {name} form")
         if name in ("batched", "learned_batched"):
             calls[name]()  # warm-up: graph capture
-        t = time.perf_counter()
-        u, st = calls[name]()
-        runs[name] = (time.perf_counter() - t, u, st)
+        reps = []
+        for _ in range(C3_REPEATS):
+            t = time.perf_counter()
+            u, st = calls[name]()
+            reps.append((time.perf_counter() - t, u, st))
+        reps.sort(key=lambda r: r[0])
+        runs[name] = reps[len(reps) // 2]
+        out.setdefault("runs_s", {})[name] = [r[0] for r in reps]
     out.update({f"{k}_s": v[0] for k, v in runs.items()})
     out["device_ms"] = {k: v[2].device_ms for k, v in runs.items()}
     for base in ("plain", "plain_exact"):
