@@ -273,10 +273,11 @@ CCD double div_rd_fast_pos(double a, double b) {
 static __device__ __noinline__ double div_rd_slow(double a, double b);
 
 CCD bool div_fast_ok(double a, double b) {
-  // both exponents within [-500, 499]: |x| in [2^-500, 2^500)
-  uint32_t ea = (uint32_t)((bits(a) >> 52) & 0x7ff);
-  uint32_t eb = (uint32_t)((bits(b) >> 52) & 0x7ff);
-  return (ea - (1023u - 500u)) < 1000u && (eb - (1023u - 500u)) < 1000u;
+  // |a| and |b| in [2^-500, 2^500) (false for zeros, subnormals, infinities
+  // and NaN): four compares with absolute-value operands
+  const double lo = 0x1p-500, hi = 0x1p500;
+  const double fa = __builtin_fabs(a), fb = __builtin_fabs(b);
+  return (fa >= lo) & (fa < hi) & (fb >= lo) & (fb < hi);
 }
 
 CCD double div_rd(double a, double b) {
@@ -337,9 +338,9 @@ CCD double sqrt_rd_fast(double x) {
 static __device__ __noinline__ double sqrt_rd_slow(double x);
 
 CCD bool sqrt_fast_ok(double x) {
-  // x in [2^-960, +inf): biased exponent in [63, 2046] and sign clear
-  uint64_t b = bits(x);
-  return (b >> 52) - 63u < 1984u;
+  // x in [2^-960, DBL_MAX] (false for NaN): biased exponent in [63, 2046],
+  // sign clear
+  return (x >= 0x1p-960) & (x <= __DBL_MAX__);
 }
 CCD double sqrt_rd(double x) { return sqrt_fast_ok(x) ? sqrt_rd_fast(x) : sqrt_rd_slow(x); }
 CCD double sqrt_ru(double x) {

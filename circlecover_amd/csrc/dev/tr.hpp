@@ -101,8 +101,10 @@ CCD iv div_pt_fast(double n, double d) {  // n, d nonzero, div_fast_ok(n, d)
 // corner of x*y is RU(x.hi*y.hi) or RU(x.hi*y.lo); b = la/4 + 1/(2 la) only
 // occurs for la > be_lo > 0.
 CCD bool crit_hi_ok(iv la) {
-  const uint32_t el = (uint32_t)((bits(la.lo) >> 52) & 0x7ff), eh = (uint32_t)((bits(la.hi) >> 52) & 0x7ff);
-  return (el - 523u) < 1000u && (eh - 523u) < 1000u && ((bits(la.lo) ^ bits(la.hi)) >> 63) == 0;
+  // |lo|, |hi| in [2^-500, 2^500) and one sign: compares with absolute-value
+  // operands (false for zeros, subnormals, infinities and NaN)
+  const double fl = __builtin_fabs(la.lo), fh = __builtin_fabs(la.hi);
+  return (fl >= 0x1p-500) & (fl < 0x1p500) & (fh >= 0x1p-500) & (fh < 0x1p500) & ((la.lo > 0.0) == (la.hi > 0.0));
 }
 CCD double crit_ratio_hi_fast(const Konst& K, iv la) {
   const bool lo_side = la.hi <= K.be_lo;
@@ -140,8 +142,7 @@ CCD iv weight_for(const Konst& K, iv w, iv h) { return crit_ratio(K, w, h) * w *
 // weight_for(K, pt(w), pt(h)).hi: for w, h > 0 the products' upper corners
 // are RU(RU(crit.hi * w) * h).
 CCD double weight_for_pt_hi(const Konst& K, double w, double h) {
-  const uint32_t ew = (uint32_t)(bits(w) >> 52), eh = (uint32_t)(bits(h) >> 52);  // sign and exponent
-  if ((ew - 823u) < 400u && (eh - 823u) < 400u)  // w, h in [2^-200, 2^200)
+  if ((w >= 0x1p-200) & (w < 0x1p200) & (h >= 0x1p-200) & (h < 0x1p200))  // w, h in [2^-200, 2^200)
     return mul_ru(mul_ru(crit_ratio_pt_hi(K, w, h), w), h);
   return weight_for(K, pt(w), pt(h)).hi;
 }
