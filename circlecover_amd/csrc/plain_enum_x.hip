@@ -16,7 +16,9 @@ extern "C" __global__ void __launch_bounds__(256) ccrd_tr_plain_x(const double* 
   tr_plain_cell(cells, n1, n2, n3, n4, K, leaves, cap, leaf_cell, leaf_piece, count, trap);
 }
 
-extern "C" __global__ void __launch_bounds__(256) ccrd_rsb_plain_x(const rsb_rec* __restrict__ cells, int n3, int n4,
+// two waves per SIMD (at most 256 VGPRs): the loop nest needs about that
+// many, and one register above it would halve the occupancy
+extern "C" __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) ccrd_rsb_plain_x(const rsb_rec* __restrict__ cells, int n3, int n4,
                                                                    int n5, int n6, const Konst* __restrict__ kp,
                                                                    rsb_rec* __restrict__ leaves, uint64_t cap,
                                                                    unsigned long long* __restrict__ count,
