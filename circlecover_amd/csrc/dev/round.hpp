@@ -258,15 +258,32 @@ CCD double rsq_seed(double x) {
 // corrected quotient q + r*y then misses a/b by at most 2^-48 ulp, so q is
 // RD(a/b) or a neighbour of it: one residual test each way settles it
 // (sign(a - q*b) == sign(a/b - q), exact from one FMA here).
+// The dependent chain of the quotient, q and its residual r = a - q*b, in
+// one asm block: the same instructions, operands and modifiers as the
+// one-operation primitives above (seed, fnma_rd(b, y, 1), fma_rd(y, e, y),
+// mul_rd(a, y), fnma_rd(b, q, a), fma_rd(r0, y, q), fnma_rd(q, b, a)).
+// Between separate asm blocks the compiler inserts a wait state for a
+// result-forwarding hazard of sub-dword destinations, which these
+// instructions do not have; on a dependent chain those waits add up.
+CCD void div_core(double a, double b, double& q, double& r) {
+  double y, e, r0;
+  asm volatile(
+      "v_rcp_f64 %[y], %[b]\n\t"
+      "s_nop 1\n\t"
+      "v_fma_f64 %[e], -%[b], %[y], 1.0\n\t"
+      "v_fma_f64 %[y], %[y], %[e], %[y]\n\t"
+      "v_mul_f64 %[q], %[a], %[y]\n\t"
+      "v_fma_f64 %[r0], -%[b], %[q], %[a]\n\t"
+      "v_fma_f64 %[q], %[r0], %[y], %[q]\n\t"
+      "v_fma_f64 %[r], -%[q], %[b], %[a]"
+      : [y] "=&v"(y), [e] "=&v"(e), [q] "=&v"(q), [r0] "=&v"(r0), [r] "=&v"(r)
+      : [a] "v"(a), [b] "v"(b));
+}
 CCD double div_rd_fast_pos(double a, double b) {
-  double y = rcp_seed(b);
-  const double e = fnma_rd(b, y, 1.0);
-  y = fma_rd(y, e, y);
-  double q = mul_rd(a, y);
-  const double r0 = fnma_rd(b, q, a);
-  q = fma_rd(r0, y, q);
+  double q, r;
+  div_core(a, b, q, r);
   const double qu = next_up(q);
-  const double r = fnma_rd(q, b, a), ru = fnma_rd(qu, b, a);
+  const double ru = fnma_rd(qu, b, a);
   return r < 0.0 ? next_down(q) : (ru >= 0.0 ? qu : q);
 }
 
@@ -321,17 +338,31 @@ static __device__ __noinline__ double div_rd_slow(double a, double b) {
 // RD(sqrt x) or a neighbour of it (a non-square's root lies at least 2^-54
 // ulp from every double, so both neighbours can occur, rarely): one residual
 // test each way settles it (sign(x - g*g) == sign(sqrt(x) - g); g > 0).
+// The dependent chain of the root, g and its residual d = x - g*g, in one
+// asm block (as div_core: the instructions of rsq_seed, mul_rd(x, y),
+// mul_rd(y, 0.5), fnma_rd(g, h, 0.5), fma_rd(g, r, g), fma_rd(h, r, h),
+// fnma_rd(g, g, x), fma_rd(d0, h, g), fnma_rd(g, g, x)).
+CCD void sqrt_core(double x, double& g, double& d) {
+  double y, h, r, d0;
+  asm volatile(
+      "v_rsq_f64 %[y], %[x]\n\t"
+      "s_nop 1\n\t"
+      "v_mul_f64 %[g], %[x], %[y]\n\t"
+      "v_mul_f64 %[h], %[y], 0.5\n\t"
+      "v_fma_f64 %[r], -%[g], %[h], 0.5\n\t"
+      "v_fma_f64 %[g], %[g], %[r], %[g]\n\t"
+      "v_fma_f64 %[h], %[h], %[r], %[h]\n\t"
+      "v_fma_f64 %[d0], -%[g], %[g], %[x]\n\t"
+      "v_fma_f64 %[g], %[d0], %[h], %[g]\n\t"
+      "v_fma_f64 %[d], -%[g], %[g], %[x]"
+      : [y] "=&v"(y), [h] "=&v"(h), [r] "=&v"(r), [d0] "=&v"(d0), [g] "=&v"(g), [d] "=&v"(d)
+      : [x] "v"(x));
+}
 CCD double sqrt_rd_fast(double x) {
-  const double y = rsq_seed(x);
-  double g = mul_rd(x, y);
-  double h = mul_rd(y, 0.5);
-  const double r = fnma_rd(g, h, 0.5);
-  g = fma_rd(g, r, g);
-  h = fma_rd(h, r, h);
-  const double d0 = fnma_rd(g, g, x);
-  g = fma_rd(d0, h, g);
+  double g, d;
+  sqrt_core(x, g, d);
   const double gu = from_bits(bits(g) + 1u);
-  const double d = fnma_rd(g, g, x), du = fnma_rd(gu, gu, x);
+  const double du = fnma_rd(gu, gu, x);
   return d < 0.0 ? from_bits(bits(g) - 1u) : (du >= 0.0 ? gu : g);
 }
 
