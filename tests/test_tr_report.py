@@ -85,7 +85,7 @@ def test_gpu_tr_report_empty_set(prover):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("n", [1, 2, 63, 255, 256, 257, 1000, 5000])
+@pytest.mark.parametrize("n", [1, 2, 63, 255, 256, 257, 1000, 5000, 100000])
 def test_gpu_tr_report_mixed_sets_match_oracle(prover, n):
     rng = np.random.default_rng(n)
     cells = np.concatenate([two_disk_cells(rng, n), three_disk_cells(rng, n)])
